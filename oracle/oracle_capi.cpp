@@ -1,0 +1,340 @@
+// ORACLE C-ABI — test infrastructure only (loaded by tests/ and bench.py's cpu_baseline leg).
+// Mirrors the product's kyn_* entry points one for one (prefix orc_) so that the same
+// workload builder can drive either library.
+#include <algorithm>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "kyn_oracle.hpp"
+
+using namespace kyn_oracle;
+
+namespace {
+thread_local std::string g_err;
+struct OrcSystem {
+    System sys;
+    StepParameters params;
+    OrcSystem(System s, StepParameters p) : sys(std::move(s)), params(p) {}
+};
+}  // namespace
+
+extern "C" {
+
+const char* orc_last_error() { return g_err.c_str(); }
+
+void* orc_model_create() { return new ModelIn(); }
+void orc_model_destroy(void* m) { delete static_cast<ModelIn*>(m); }
+int orc_model_set_gravity(void* m, const double* g) {
+    static_cast<ModelIn*>(m)->gravity = {g[0], g[1], g[2]};
+    return 0;
+}
+long orc_model_add_node(void* m, const double* x0, const double* u, const double* v,
+                        const double* vd, double s) {
+    auto* mo = static_cast<ModelIn*>(m);
+    NodeIn n;
+    std::copy(x0, x0 + 7, n.x0.begin());
+    std::copy(u, u + 7, n.u.begin());
+    std::copy(v, v + 6, n.v.begin());
+    std::copy(vd, vd + 6, n.vd.begin());
+    n.s = s;
+    mo->nodes.push_back(n);
+    return static_cast<long>(mo->nodes.size() - 1);
+}
+long orc_model_add_beam_element(void* m, size_t nn, const size_t* nodes, size_t ns,
+                                const double* sec_s, const double* sec_M, const double* sec_C,
+                                size_t nq, const double* qp) {
+    auto* mo = static_cast<ModelIn*>(m);
+    BeamIn b;
+    b.nodes.assign(nodes, nodes + nn);
+    for (size_t k = 0; k < ns; ++k) {
+        SectionIn sec{};
+        sec.s = sec_s[k];
+        std::copy(sec_M + k * 36, sec_M + k * 36 + 36, sec.M.begin());
+        std::copy(sec_C + k * 36, sec_C + k * 36 + 36, sec.C.begin());
+        b.sections.push_back(sec);
+    }
+    for (size_t q = 0; q < nq; ++q) b.quad.push_back({qp[2 * q], qp[2 * q + 1]});
+    mo->beams.push_back(std::move(b));
+    return static_cast<long>(mo->beams.size() - 1);
+}
+long orc_model_add_constraint(void* m, int type, size_t base, size_t target, const double* axis,
+                              const double* init) {
+    auto* mo = static_cast<ModelIn*>(m);
+    ConstraintIn c{static_cast<CType>(type), base, target};
+    if (axis) std::copy(axis, axis + 3, c.axis.begin());
+    if (init) std::copy(init, init + 7, c.init.begin());
+    mo->constraints.push_back(c);
+    return static_cast<long>(mo->constraints.size() - 1);
+}
+
+void* orc_system_create(void* m, int is_dynamic, size_t max_iter, double h, double rho_inf,
+                        double atol, double rtol) {
+    try {
+        auto sys = CreateSystem(*static_cast<ModelIn*>(m));
+        return new OrcSystem(std::move(sys), StepParameters(is_dynamic != 0, max_iter, h, rho_inf, atol, rtol));
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return nullptr;
+    }
+}
+void orc_system_destroy(void* s) { delete static_cast<OrcSystem*>(s); }
+
+int orc_step(void* s) {
+    auto* o = static_cast<OrcSystem*>(s);
+    return Step(o->params, o->sys) ? 0 : 1;
+}
+// Steps several independent systems in parallel (OpenMP over systems).
+void orc_step_many(void** s, size_t n, int* status) {
+#pragma omp parallel for schedule(dynamic)
+    for (long i = 0; i < static_cast<long>(n); ++i) {
+        auto* o = static_cast<OrcSystem*>(s[i]);
+        status[i] = Step(o->params, o->sys) ? 0 : 1;
+    }
+}
+
+// Runs the first assembly of a step without solving (the sequence of
+// tests/regression_tests/regression/test_solver.cpp:97-112).
+void orc_debug_assemble(void* s) {
+    auto* o = static_cast<OrcSystem*>(s);
+    PredictNextState(o->params, o->sys);
+    ResetConstraints(o->sys);
+    ResetSolver(o->sys);
+    UpdateTangentOperator(o->params, o->sys);
+    UpdateSystemVariablesBeams(o->params, o->sys);
+    AssembleSystemMatrix(o->params, o->sys);
+    AssembleSystemResidual(o->sys);
+    UpdateConstraintVariables(o->sys);
+    AssembleConstraintsMatrix(o->sys);
+    AssembleConstraintsResidual(o->sys);
+}
+
+size_t orc_num_nodes(void* s) { return static_cast<OrcSystem*>(s)->sys.num_nodes; }
+size_t orc_num_dofs(void* s) { return static_cast<OrcSystem*>(s)->sys.num_dofs; }
+size_t orc_num_system_dofs(void* s) { return static_cast<OrcSystem*>(s)->sys.num_system_dofs; }
+size_t orc_num_blocks(void* s) { return static_cast<OrcSystem*>(s)->sys.blocks.size(); }
+
+// field: 0 x0, 1 x, 2 q_prev, 3 q (x7); 4 q_delta, 5 v, 6 vd, 7 a, 8 f (x6); 9 tangent (x36)
+static std::vector<double>* field_ptr(System& s, int field, int& width) {
+    switch (field) {
+        case 0: width = 7; return &s.x0;
+        case 1: width = 7; return &s.x;
+        case 2: width = 7; return &s.q_prev;
+        case 3: width = 7; return &s.q;
+        case 4: width = 6; return &s.q_delta;
+        case 5: width = 6; return &s.v;
+        case 6: width = 6; return &s.vd;
+        case 7: width = 6; return &s.a;
+        case 8: width = 6; return &s.f;
+        case 9: width = 36; return &s.tangent;
+        default: return nullptr;
+    }
+}
+int orc_get_state(void* s, int field, double* out) {
+    int w = 0;
+    auto* v = field_ptr(static_cast<OrcSystem*>(s)->sys, field, w);
+    if (!v) return -1;
+    std::copy(v->begin(), v->end(), out);
+    return 0;
+}
+int orc_set_state(void* s, int field, const double* in) {
+    int w = 0;
+    auto* v = field_ptr(static_cast<OrcSystem*>(s)->sys, field, w);
+    if (!v) return -1;
+    std::copy(in, in + v->size(), v->begin());
+    return 0;
+}
+size_t orc_time_step(void* s) { return static_cast<OrcSystem*>(s)->sys.time_step; }
+int orc_set_nodal_loads(void* s, const double* f) {
+    auto& sy = static_cast<OrcSystem*>(s)->sys;
+    std::copy(f, f + sy.num_nodes * 6, sy.f.begin());
+    return 0;
+}
+int orc_set_element_node_loads(void* s, size_t elem, const double* fx) {
+    auto& e = static_cast<OrcSystem*>(s)->sys.beams.at(elem);
+    std::copy(fx, fx + e.N * 6, e.node_FX.begin());
+    return 0;
+}
+int orc_set_constraint_displacement(void* s, size_t id, const double* d) {
+    auto& c = static_cast<OrcSystem*>(s)->sys.cons.at(id);
+    std::copy(d, d + 7, c.input.begin());
+    return 0;
+}
+int orc_set_control(void* s, size_t id, double v) {
+    static_cast<OrcSystem*>(s)->sys.cons.at(id).control = v;
+    return 0;
+}
+size_t orc_get_convergence(void* s, double* out, size_t cap) {
+    auto& e = static_cast<OrcSystem*>(s)->sys.convergence_err;
+    const size_t n = std::min(cap, e.size());
+    std::copy(e.begin(), e.begin() + static_cast<long>(n), out);
+    return e.size();
+}
+int orc_get_rhs(void* s, double* out) {
+    auto& sy = static_cast<OrcSystem*>(s)->sys;
+    std::copy(sy.b.begin(), sy.b.end(), out);
+    return 0;
+}
+int orc_get_solution(void* s, double* out) {
+    auto& sy = static_cast<OrcSystem*>(s)->sys;
+    std::copy(sy.xsol.begin(), sy.xsol.end(), out);
+    return 0;
+}
+// which: 0 lambda, 1 residual_terms, 2 output(3)
+int orc_get_constraint(void* s, size_t id, int which, double* out) {
+    auto& c = static_cast<OrcSystem*>(s)->sys.cons.at(id);
+    if (which == 0) std::copy(c.lambda.begin(), c.lambda.end(), out);
+    else if (which == 1) std::copy(c.res.begin(), c.res.end(), out);
+    else if (which == 2) std::copy(c.output.begin(), c.output.end(), out);
+    else return -1;
+    return 0;
+}
+// Element evaluation at the current state: residual [N][6], system matrix [N][N][36]
+int orc_element_terms(void* s, size_t elem, int update_tangent, double* resid, double* mat) {
+    auto* o = static_cast<OrcSystem*>(s);
+    if (update_tangent) UpdateTangentOperator(o->params, o->sys);
+    auto& e = o->sys.beams.at(elem);
+    CalculateQuadraturePointValues(o->params.beta_prime, o->params.gamma_prime, o->sys, e);
+    if (resid) std::copy(e.residual.begin(), e.residual.end(), resid);
+    if (mat) std::copy(e.sysmat.begin(), e.sysmat.end(), mat);
+    return 0;
+}
+// Element QP setup views: which 0 weight, 1 jacobian, 2 shape_interp, 3 shape_deriv,
+// 4 qp_x0, 5 qp_x0_prime, 6 qp_r0, 7 qp_Mstar, 8 qp_Cstar
+int orc_element_setup(void* s, size_t elem, int which, double* out) {
+    auto& e = static_cast<OrcSystem*>(s)->sys.beams.at(elem);
+    const std::vector<double>* v = nullptr;
+    switch (which) {
+        case 0: v = &e.weight; break;
+        case 1: v = &e.jacobian; break;
+        case 2: v = &e.shape_interp; break;
+        case 3: v = &e.shape_deriv; break;
+        case 4: v = &e.qp_x0; break;
+        case 5: v = &e.qp_x0_prime; break;
+        case 6: v = &e.qp_r0; break;
+        case 7: v = &e.qp_Mstar; break;
+        case 8: v = &e.qp_Cstar; break;
+        default: return -1;
+    }
+    std::copy(v->begin(), v->end(), out);
+    return 0;
+}
+void orc_step_parameters(void* s, double* out) {
+    auto& p = static_cast<OrcSystem*>(s)->params;
+    const double v[] = {p.h, p.alpha_m, p.alpha_f, p.gamma, p.beta, p.gamma_prime, p.beta_prime, p.conditioner};
+    std::copy(v, v + 8, out);
+}
+
+// --- restart format (src/state/write_state_to_file.hpp:19-58, read_state_from_file.hpp:22-65)
+size_t orc_restart_size(void* s) {
+    const size_t n = static_cast<OrcSystem*>(s)->sys.num_nodes;
+    return sizeof(size_t) + n * (7 * 4 + 6 * 5) * sizeof(double);
+}
+int orc_write_restart(void* s, unsigned char* out) {
+    auto& sy = static_cast<OrcSystem*>(s)->sys;
+    size_t n = sy.num_nodes;
+    std::memcpy(out, &n, sizeof(size_t));
+    size_t off = sizeof(size_t);
+    const std::vector<double>* order[] = {&sy.x0, &sy.x, &sy.q_delta, &sy.q_prev, &sy.q,
+                                          &sy.v, &sy.vd, &sy.a, &sy.f};
+    for (auto* v : order) {
+        std::memcpy(out + off, v->data(), v->size() * sizeof(double));
+        off += v->size() * sizeof(double);
+    }
+    return 0;
+}
+
+// --- functor entry points for the reference's unit-test known answers
+void orc_fn_strain(const double* x0p, const double* up, const double* r, const double* rp, double* out) {
+    const auto e = CalculateStrain({x0p[0], x0p[1], x0p[2]}, {up[0], up[1], up[2]},
+                                   {r[0], r[1], r[2], r[3]}, {rp[0], rp[1], rp[2], rp[3]});
+    std::copy(e.begin(), e.end(), out);
+}
+void orc_fn_rotate_section(const double* q, const double* C, double* out) {
+    Mat6 c{};
+    std::copy(C, C + 36, c.begin());
+    const auto r = RotateSectionMatrix({q[0], q[1], q[2], q[3]}, c);
+    std::copy(r.begin(), r.end(), out);
+}
+void orc_fn_ouu_puu_quu(const double* C, const double* x0pupSS, const double* Mt, const double* Nt,
+                        double* O, double* P, double* Q) {
+    Mat6 c{};
+    Mat3 x{}, m{}, nn{};
+    std::copy(C, C + 36, c.begin());
+    std::copy(x0pupSS, x0pupSS + 9, x.begin());
+    std::copy(Mt, Mt + 9, m.begin());
+    std::copy(Nt, Nt + 9, nn.begin());
+    const auto o = CalculateOuu(c, x, m, nn);
+    const auto p = CalculatePuu(c, x, nn);
+    const auto q = CalculateQuu(c, x, nn);
+    std::copy(o.begin(), o.end(), O);
+    std::copy(p.begin(), p.end(), P);
+    std::copy(q.begin(), q.end(), Q);
+}
+void orc_fn_fd(const double* x0pupSS, const double* FC, double* out) {
+    Mat3 x{};
+    std::copy(x0pupSS, x0pupSS + 9, x.begin());
+    const auto r = CalculateForceFD(x, {FC[0], FC[1], FC[2], FC[3], FC[4], FC[5]});
+    std::copy(r.begin(), r.end(), out);
+}
+// inertial pieces: in = m, u_ddot[3], omega[3], omega_dot[3], eta[3], rho[9]
+void orc_fn_inertial(const double* in, double* FI, double* G, double* K) {
+    const double m = in[0];
+    const Vec3 ud{in[1], in[2], in[3]}, w{in[4], in[5], in[6]}, wd{in[7], in[8], in[9]},
+        eta{in[10], in[11], in[12]};
+    Mat3 rho{};
+    std::copy(in + 13, in + 22, rho.begin());
+    const Mat3 et = VecTilde(eta), wt = VecTilde(w), wdt = VecTilde(wd);
+    const auto f = CalculateInertialForce(m, ud, w, wd, eta, et, rho, wt, wdt);
+    const auto g = CalculateGyroscopicMatrix(m, w, eta, rho, wt);
+    const auto k = CalculateInertiaStiffnessMatrix(m, ud, w, wd, eta, rho, wt, wdt);
+    std::copy(f.begin(), f.end(), FI);
+    std::copy(g.begin(), g.end(), G);
+    std::copy(k.begin(), k.end(), K);
+}
+void orc_fn_gravity(double m, const double* g, const double* eta, double* out) {
+    const auto r = CalculateGravityForce(m, {g[0], g[1], g[2]}, VecTilde({eta[0], eta[1], eta[2]}));
+    std::copy(r.begin(), r.end(), out);
+}
+void orc_fn_tangent_operator(double h, const double* qd, double* out) {
+    const auto t = CalculateTangentOperatorT(h, qd);
+    std::copy(t.begin(), t.end(), out);
+}
+void orc_fn_rv_to_quat(const double* rv, double* out) {
+    const auto q = RotationVectorToQuaternion({rv[0], rv[1], rv[2]});
+    std::copy(q.begin(), q.end(), out);
+}
+void orc_fn_quat_to_rv(const double* q, double* out) {
+    const auto r = QuaternionToRotationVector({q[0], q[1], q[2], q[3]});
+    std::copy(r.begin(), r.end(), out);
+}
+void orc_fn_quat_compose(const double* a, const double* b, double* out) {
+    const auto q = QuaternionCompose({a[0], a[1], a[2], a[3]}, {b[0], b[1], b[2], b[3]});
+    std::copy(q.begin(), q.end(), out);
+}
+void orc_fn_gauss_legendre(size_t n, double* xi, double* w) {
+    std::vector<double> x, ww;
+    GaussLegendreRule(n, x, ww);
+    std::copy(x.begin(), x.end(), xi);
+    std::copy(ww.begin(), ww.end(), w);
+}
+void orc_fn_gll_points(size_t order, double* out) {
+    const auto p = GenerateGLLPoints(order);
+    std::copy(p.begin(), p.end(), out);
+}
+void orc_fn_lagrange(double x, size_t n, const double* xs, double* w, double* dw) {
+    std::vector<double> v(xs, xs + n), a, b;
+    LagrangePolynomialInterpWeights(x, v, a);
+    LagrangePolynomialDerivWeights(x, v, b);
+    std::copy(a.begin(), a.end(), w);
+    std::copy(b.begin(), b.end(), dw);
+}
+int orc_fn_lu_solve(size_t n, double* A, double* b) {
+    std::vector<int> piv(n);
+    const bool ok = LUFactor(n, A, piv.data());
+    LUSolve(n, A, piv.data(), b);
+    return ok ? 0 : 1;
+}
+
+}  // extern "C"

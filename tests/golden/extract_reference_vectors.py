@@ -1,0 +1,212 @@
+"""Extract the reference's golden vectors into JSON fixtures under tests/golden/.
+
+The reference keeps its known answers as numeric literals inside C++ test sources.  This
+script reads those sources as text under /root/reference (it is run in the build container
+only; the fixtures it writes are what the tests load) and stores the numbers, with the
+inputs they belong to, as plain JSON data.  Nothing of the reference's code is kept: only
+the inputs and the expected outputs.
+
+Sources (paths relative to the reference root):
+  tests/regression_tests/regression/test_solver.cpp         -> solver_regression.json
+  tests/regression_tests/regression/test_cantilever_beam.cpp -> cantilever_sine.json
+  tests/regression_tests/regression/test_rotor.cpp
+  tests/regression_tests/regression/iea15_rotor_data.hpp    -> iea15_rotor.json
+  tests/unit_tests/system/beams/test_calculate_strain.cpp   -> functor_known_answers.json
+  tests/unit_tests/system/masses/*.cpp                      -> functor_known_answers.json
+
+Usage: python tests/golden/extract_reference_vectors.py [/root/reference]
+"""
+
+import json
+import re
+import sys
+from pathlib import Path
+
+NUM = re.compile(r"(?<![\w.])-?(?:\d+\.\d*|\.\d+|\d+)(?:[eE][-+]?\d+)?")
+HERE = Path(__file__).resolve().parent
+
+
+def numbers(text):
+    return [float(t) for t in NUM.findall(text)]
+
+
+def balanced(text, start):
+    """Return the text between the parenthesis/brace opened at text[start] and its match."""
+    open_c = text[start]
+    close_c = {"(": ")", "{": "}", "[": "]"}[open_c]
+    depth = 0
+    for i in range(start, len(text)):
+        if text[i] == open_c:
+            depth += 1
+        elif text[i] == close_c:
+            depth -= 1
+            if depth == 0:
+                return text[start + 1 : i]
+    raise ValueError("unbalanced")
+
+
+def split_top(args):
+    out, depth, cur = [], 0, []
+    for ch in args:
+        if ch in "({[":
+            depth += 1
+        elif ch in ")}]":
+            depth -= 1
+        if ch == "," and depth == 0:
+            out.append("".join(cur))
+            cur = []
+        else:
+            cur.append(ch)
+    out.append("".join(cur))
+    return out
+
+
+def strip_comments(text):
+    text = re.sub(r"//[^\n]*", "", text)
+    return re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+
+
+def expectations(body):
+    """All expect_kokkos_view_{1D,2D}_equal calls in a function body, in order."""
+    res = []
+    for m in re.finditer(r"expect_kokkos_view_(1D|2D)_equal\s*\(", body):
+        args = split_top(balanced(body, m.end() - 1))
+        name = re.sub(r"\s+", "", args[0])
+        data = numbers(",".join(args[1:]))
+        res.append({"view": name, "values": data})
+    return res
+
+
+def function_body(text, name):
+    m = re.search(r"\b" + re.escape(name) + r"\s*\(\s*\)\s*\{", text)
+    return balanced(text, m.end() - 1)
+
+
+def solver_regression(root):
+    src = strip_comments((root / "tests/regression_tests/regression/test_solver.cpp").read_text())
+    out = {"source": "tests/regression_tests/regression/test_solver.cpp"}
+    for fn in ("SetUpSolverAndAssemble", "SetupAndTakeNoSteps", "SetupAndTakeTwoSteps"):
+        body = function_body(src, fn)
+        entry = {"expect": expectations(body)}
+        m = re.search(r"EXPECT_NEAR\(solver\.convergence_err\[0\],\s*([^,]+),", body)
+        if m:
+            entry["convergence_err0"] = float(m.group(1))
+        mass = numbers(balanced(body, body.index("{", body.index("mass_matrix"))))
+        stiff = numbers(balanced(body, body.index("{", body.index("stiffness_matrix"))))
+        node_s = numbers(balanced(body, body.index("{", body.index("node_s ="))))
+        quad = numbers(balanced(body, body.index("(", body.index("model.AddBeamElement"))).split("std::array{", 2)[-1])
+        # quadrature: the last std::array{...} group inside AddBeamElement holds xi,w pairs
+        qtext = balanced(body, body.index("(", body.index("model.AddBeamElement")))
+        qblock = qtext[qtext.rfind("std::array{\n") :]
+        quad = numbers(qblock)
+        entry["inputs"] = {
+            "mass_matrix": mass,
+            "stiffness_matrix": stiff,
+            "node_s": node_s,
+            "quadrature": quad,
+            "x_root_offset": 2.0 if "x0_root[0]" in body.split("AddNode")[0] + body.split("AddNode")[1][:200] else 0.0,
+            "omega": numbers(re.search(r"omega = ([^;]+);", body).group(1))[0],
+            "max_iter": numbers(re.search(r"max_iter = ([^;]+);", body).group(1))[0],
+            "step_size": numbers(re.search(r"step_size = ([^;]+);", body).group(1))[0],
+            "rho_inf": numbers(re.search(r"rho_inf = ([^;]+);", body).group(1))[0],
+            "bc_translation_follows_rotation": "u_trans" in body,
+        }
+        out[fn] = entry
+    return out
+
+
+def cantilever(root):
+    src = strip_comments((root / "tests/regression_tests/regression/test_cantilever_beam.cpp").read_text())
+    body = src[src.index("CantileverBeamSineLoad") :]
+    mass = numbers(balanced(body, body.index("{", body.index("mass_matrix ="))))
+    stiff = numbers(balanced(body, body.index("{", body.index("stiffness_matrix ="))))
+    node_s = numbers(balanced(body, body.index("{", body.index("node_s ="))))
+    qtext = balanced(body, body.index("(", body.index("model.AddBeamElement")))
+    quad = numbers(qtext[qtext.rfind("std::array{\n") :])
+    loads = [numbers(m.group(1)) for m in re.finditer(r"100\. \* std::sin\(([^)]*)\)", body)]
+    tips = [e["values"] for e in expectations(body)]
+    return {
+        "source": "tests/regression_tests/regression/test_cantilever_beam.cpp",
+        "mass_matrix": mass,
+        "stiffness_matrix": stiff,
+        "node_s": node_s,
+        "quadrature": quad,
+        "load_sin_args": loads,
+        "max_iter": numbers(re.search(r"max_iter\((\d+)\)", body).group(1))[0],
+        "step_size": numbers(re.search(r"step_size\(([^)]+)\)", body).group(1))[0],
+        "rho_inf": numbers(re.search(r"rho_inf\(([^)]+)\)", body).group(1))[0],
+        "tip_node": 4,
+        "tip_q_translation": tips,
+    }
+
+
+def iea15(root):
+    data = strip_comments((root / "tests/regression_tests/regression/iea15_rotor_data.hpp").read_text())
+    node_xi = numbers(balanced(data, data.index("{", data.index("node_xi ="))))
+    coords = numbers(balanced(data, data.index("{", data.index("node_coords ="))))
+    quad = numbers(balanced(data, data.index("{", data.index("trapz_quadrature ="))))
+    sec_text = balanced(data, data.index("{", data.index("material_sections =")))
+    sections = []
+    for m in re.finditer(r"BeamSection\s*\{", sec_text):
+        vals = numbers(balanced(sec_text, m.end() - 1))
+        assert len(vals) == 73, len(vals)
+        sections.append({"s": vals[0], "M": vals[1:37], "C": vals[37:73]})
+    test = strip_comments((root / "tests/regression_tests/regression/test_rotor.cpp").read_text())
+    body = test[test.index("TEST(RotorTest, IEA15Rotor)") :]
+    body = body[: body.index("TEST(", 10)] if "TEST(" in body[10:] else body
+    omega = numbers(re.search(r"omega = std::array\{([^}]*)\}", body).group(1))
+    hub = numbers(re.search(r"hub_rad = std::array\{([^}]*)\}", test).group(1))
+    gravity = numbers(re.search(r"SetGravity\(([^)]*)\)", test).group(1))
+    checks = []
+    for m in re.finditer(r"EXPECT_NEAR\(q\((\d+), (\d+)\),\s*(q\(\d+, \d+\)|[^,]+),\s*([^)]+)\)", body):
+        node, comp = int(m.group(1)), int(m.group(2))
+        rhs = m.group(3).strip()
+        tol = float(m.group(4))
+        ref = re.match(r"q\((\d+), (\d+)\)", rhs)
+        if ref:
+            checks.append({"node": node, "comp": comp, "equals": [int(ref.group(1)), int(ref.group(2))], "tol": tol})
+        else:
+            checks.append({"node": node, "comp": comp, "value": float(rhs), "tol": tol})
+    return {
+        "source": "tests/regression_tests/regression/test_rotor.cpp + iea15_rotor_data.hpp",
+        "node_xi": node_xi,
+        "node_coords": coords,
+        "quadrature": quad,
+        "sections": sections,
+        "omega": omega,
+        "hub_radius": hub,
+        "gravity": gravity,
+        "num_blades": 3,
+        "max_iter": numbers(re.search(r"max_iter\((\d+)\)", body).group(1))[0],
+        "step_size": numbers(re.search(r"step_size\(([^)]+)\)", body).group(1))[0],
+        "rho_inf": numbers(re.search(r"rho_inf\(([^)]+)\)", body).group(1))[0],
+        "t_end": numbers(re.search(r"t_end\(([^)]+)\)", body).group(1))[0],
+        "checks": checks,
+    }
+
+
+def functor_known_answers(root):
+    out = {}
+    t = strip_comments((root / "tests/unit_tests/system/beams/test_calculate_strain.cpp").read_text())
+    out["strain"] = {
+        "x0_prime": numbers(re.search(r'"x0_prime", std::array\{([^}]*)\}', t).group(1)),
+        "u_prime": numbers(re.search(r'"u_prime", std::array\{([^}]*)\}', t).group(1)),
+        "r": numbers(re.search(r'"r", std::array\{([^}]*)\}', t).group(1)),
+        "r_prime": numbers(re.search(r'"r_prime", std::array\{([^}]*)\}', t).group(1)),
+        "expected": numbers(re.search(r"strain_exact_data = std::array\{([^}]*)\}", t).group(1)),
+        "source": "tests/unit_tests/system/beams/test_calculate_strain.cpp",
+    }
+    return out
+
+
+def main():
+    root = Path(sys.argv[1] if len(sys.argv) > 1 else "/root/reference")
+    (HERE / "solver_regression.json").write_text(json.dumps(solver_regression(root), indent=1))
+    (HERE / "cantilever_sine.json").write_text(json.dumps(cantilever(root), indent=1))
+    (HERE / "iea15_rotor.json").write_text(json.dumps(iea15(root)))
+    (HERE / "functor_known_answers.json").write_text(json.dumps(functor_known_answers(root), indent=1))
+    print("wrote fixtures to", HERE)
+
+
+if __name__ == "__main__":
+    main()

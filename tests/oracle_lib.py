@@ -1,0 +1,210 @@
+"""ctypes adapter for the CPU oracle (oracle/liboracle_kyn.so).
+
+Test infrastructure: the oracle is the checker.  The adapter exposes the same method names
+as openturbine_amd.KynLib so openturbine_amd.workloads.build_model can drive either.
+"""
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parents[1]
+LIB_PATH = ROOT / "oracle" / "liboracle_kyn.so"
+
+_dp = C.POINTER(C.c_double)
+_sp = C.POINTER(C.c_size_t)
+
+FIELDS = {"x0": (0, 7), "x": (1, 7), "q_prev": (2, 7), "q": (3, 7), "q_delta": (4, 6), "v": (5, 6),
+          "vd": (6, 6), "a": (7, 6), "f": (8, 6), "tangent": (9, 36)}
+
+
+def _d(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a, a.ctypes.data_as(_dp)
+
+
+def build_oracle():
+    import subprocess
+
+    subprocess.run(["make", "-s", "-C", str(ROOT / "oracle")], check=True)
+
+
+class OracleLib:
+    def __init__(self, path=LIB_PATH):
+        if not Path(path).exists():
+            build_oracle()
+        L = C.CDLL(str(path))
+        self.L = L
+        L.orc_model_create.restype = C.c_void_p
+        L.orc_model_destroy.argtypes = [C.c_void_p]
+        L.orc_model_set_gravity.argtypes = [C.c_void_p, _dp]
+        L.orc_model_add_node.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, C.c_double]
+        L.orc_model_add_node.restype = C.c_long
+        L.orc_model_add_beam_element.argtypes = [C.c_void_p, C.c_size_t, _sp, C.c_size_t, _dp, _dp, _dp,
+                                                 C.c_size_t, _dp]
+        L.orc_model_add_beam_element.restype = C.c_long
+        L.orc_model_add_constraint.argtypes = [C.c_void_p, C.c_int, C.c_size_t, C.c_size_t, _dp, _dp]
+        L.orc_model_add_constraint.restype = C.c_long
+        L.orc_system_create.argtypes = [C.c_void_p, C.c_int, C.c_size_t, C.c_double, C.c_double,
+                                        C.c_double, C.c_double]
+        L.orc_system_create.restype = C.c_void_p
+        L.orc_system_destroy.argtypes = [C.c_void_p]
+        L.orc_step.argtypes = [C.c_void_p]
+        L.orc_step_many.argtypes = [C.POINTER(C.c_void_p), C.c_size_t, C.POINTER(C.c_int)]
+        L.orc_debug_assemble.argtypes = [C.c_void_p]
+        for f in ("orc_num_nodes", "orc_num_dofs", "orc_num_system_dofs", "orc_num_blocks", "orc_time_step",
+                  "orc_restart_size"):
+            getattr(L, f).argtypes = [C.c_void_p]
+            getattr(L, f).restype = C.c_size_t
+        L.orc_get_state.argtypes = [C.c_void_p, C.c_int, _dp]
+        L.orc_set_state.argtypes = [C.c_void_p, C.c_int, _dp]
+        L.orc_set_nodal_loads.argtypes = [C.c_void_p, _dp]
+        L.orc_set_element_node_loads.argtypes = [C.c_void_p, C.c_size_t, _dp]
+        L.orc_set_constraint_displacement.argtypes = [C.c_void_p, C.c_size_t, _dp]
+        L.orc_set_control.argtypes = [C.c_void_p, C.c_size_t, C.c_double]
+        L.orc_get_convergence.argtypes = [C.c_void_p, _dp, C.c_size_t]
+        L.orc_get_convergence.restype = C.c_size_t
+        L.orc_get_rhs.argtypes = [C.c_void_p, _dp]
+        L.orc_get_solution.argtypes = [C.c_void_p, _dp]
+        L.orc_get_constraint.argtypes = [C.c_void_p, C.c_size_t, C.c_int, _dp]
+        L.orc_element_terms.argtypes = [C.c_void_p, C.c_size_t, C.c_int, _dp, _dp]
+        L.orc_element_setup.argtypes = [C.c_void_p, C.c_size_t, C.c_int, _dp]
+        L.orc_step_parameters.argtypes = [C.c_void_p, _dp]
+        L.orc_write_restart.argtypes = [C.c_void_p, C.POINTER(C.c_ubyte)]
+        L.orc_last_error.restype = C.c_char_p
+
+    # ---- model entry points (same names as the product adapter)
+    def model_create(self):
+        return self.L.orc_model_create()
+
+    def model_destroy(self, m):
+        self.L.orc_model_destroy(m)
+
+    def model_set_gravity(self, m, g):
+        a, p = _d(g)
+        self.L.orc_model_set_gravity(m, p)
+
+    def model_add_node(self, m, x0, u, v, vd, s):
+        a1, p1 = _d(x0)
+        a2, p2 = _d(u)
+        a3, p3 = _d(v)
+        a4, p4 = _d(vd)
+        return self.L.orc_model_add_node(m, p1, p2, p3, p4, float(s))
+
+    def model_add_beam_element(self, m, nodes, sec_s, sec_M, sec_C, quad):
+        nodes = np.ascontiguousarray(nodes, dtype=np.uintp)
+        a1, p1 = _d(sec_s)
+        a2, p2 = _d(sec_M)
+        a3, p3 = _d(sec_C)
+        a4, p4 = _d(quad)
+        return self.L.orc_model_add_beam_element(m, len(nodes), nodes.ctypes.data_as(_sp), len(a1), p1, p2, p3,
+                                                 a4.size // 2, p4)
+
+    def model_add_constraint(self, m, ctype, base, target, axis, init):
+        a1, p1 = _d(axis)
+        a2, p2 = _d(init)
+        return self.L.orc_model_add_constraint(m, int(ctype), int(base), int(target), p1, p2)
+
+
+class OracleSystem:
+    """A CPU oracle system built from a ModelSpec."""
+
+    def __init__(self, lib: OracleLib, spec, *, max_iter=None, atol=None, rtol=None):
+        from openturbine_amd.workloads import build_model
+
+        self.lib = lib
+        self.spec = spec
+        m = build_model(lib, spec)
+        self.h = lib.L.orc_system_create(m, int(spec.is_dynamic), int(max_iter or spec.max_iter), spec.h,
+                                         spec.rho_inf, spec.atol if atol is None else atol,
+                                         spec.rtol if rtol is None else rtol)
+        lib.model_destroy(m)
+        if not self.h:
+            raise RuntimeError(lib.L.orc_last_error().decode())
+        self.n = lib.L.orc_num_nodes(self.h)
+        self.num_dofs = lib.L.orc_num_dofs(self.h)
+        self.step_index = 0
+
+    def __del__(self):
+        try:
+            self.lib.L.orc_system_destroy(self.h)
+        except Exception:
+            pass
+
+    def get(self, field):
+        fid, w = FIELDS[field]
+        out = np.zeros((self.n, w))
+        self.lib.L.orc_get_state(self.h, fid, out.ctypes.data_as(_dp))
+        return out
+
+    def set(self, field, arr):
+        fid, w = FIELDS[field]
+        a, p = _d(np.asarray(arr).reshape(self.n, w))
+        self.lib.L.orc_set_state(self.h, fid, p)
+
+    def apply_inputs(self, inputs):
+        for cid, d in inputs.get("constraint_disp", {}).items():
+            a, p = _d(d)
+            self.lib.L.orc_set_constraint_displacement(self.h, cid, p)
+        if "nodal_loads" in inputs:
+            a, p = _d(inputs["nodal_loads"])
+            self.lib.L.orc_set_nodal_loads(self.h, p)
+        for e, fx in inputs.get("element_node_loads", {}).items():
+            a, p = _d(fx)
+            self.lib.L.orc_set_element_node_loads(self.h, e, p)
+
+    def step(self, inputs=None):
+        if inputs is None and self.spec.step_inputs is not None:
+            inputs = self.spec.step_inputs(self.step_index, self.get)
+        if inputs:
+            self.apply_inputs(inputs)
+        st = self.lib.L.orc_step(self.h)
+        self.step_index += 1
+        return st
+
+    def convergence(self):
+        buf = np.zeros(256)
+        n = self.lib.L.orc_get_convergence(self.h, buf.ctypes.data_as(_dp), 256)
+        return buf[: min(n, 256)].copy()
+
+    def rhs(self):
+        out = np.zeros(self.num_dofs)
+        self.lib.L.orc_get_rhs(self.h, out.ctypes.data_as(_dp))
+        return out
+
+    def solution(self):
+        out = np.zeros(self.num_dofs)
+        self.lib.L.orc_get_solution(self.h, out.ctypes.data_as(_dp))
+        return out
+
+    def constraint(self, cid, which):
+        out = np.zeros(6)
+        self.lib.L.orc_get_constraint(self.h, cid, which, out.ctypes.data_as(_dp))
+        return out
+
+    def debug_assemble(self, inputs=None):
+        if inputs:
+            self.apply_inputs(inputs)
+        self.lib.L.orc_debug_assemble(self.h)
+
+    def element_terms(self, elem, N, update_tangent=True):
+        r = np.zeros((N, 6))
+        m = np.zeros((N, N, 36))
+        self.lib.L.orc_element_terms(self.h, elem, int(update_tangent), r.ctypes.data_as(_dp),
+                                     m.ctypes.data_as(_dp))
+        return r, m
+
+
+def step_many(lib, systems, inputs_list=None):
+    if inputs_list is not None:
+        for s, inp in zip(systems, inputs_list):
+            if inp:
+                s.apply_inputs(inp)
+    arr = (C.c_void_p * len(systems))(*[s.h for s in systems])
+    st = (C.c_int * len(systems))()
+    lib.L.orc_step_many(arr, len(systems), st)
+    for s in systems:
+        s.step_index += 1
+    return list(st)
