@@ -1,0 +1,179 @@
+/*
+ * kynema_amd.h — C-ABI of the MI355X (gfx950) beam Newton path.
+ *
+ * The reference exposes this path as compile-time C++ templates, not as an FFI:
+ *   Model::AddNode / AddBeamElement / Add*Constraint      src/model/model.hpp:112-169, 477-541
+ *   Model::CreateSystemWithSolver / CreateSolver           src/model/model.hpp:39-63, 647-656
+ *   StepParameters(is_dynamic, max_iter, h, rho_inf, ...)  src/step/step_parameters.hpp:37-52
+ *   bool Step(parameters, solver, elements, state, constraints)   src/step/step.hpp:42-124
+ *   Constraints::UpdateDisplacement / UpdateViews          src/constraints/constraints.hpp:338-353
+ *   State views q, v, vd, a, x, q_delta, f                 src/state/state.hpp:17-68
+ *   Solver::convergence_err                                src/solver/solver.hpp:83
+ *   CopyStateData / CloneState                             src/state/copy_state_data.hpp:20-34
+ *   WriteStateToFile / ReadStateFromFile                   src/state/write_state_to_file.hpp:19-58
+ * Each entry point below names the one it stands in for.
+ *
+ * Conventions: opaque handles own all device memory; host arrays are copied in and out
+ * (no pointer is retained); nothing throws across the ABI; failures return a negative
+ * code and set a thread-local message readable with kyn_last_error().  One HIP stream per
+ * system; a system is not thread-safe, distinct systems may be driven concurrently.
+ * All arrays are row-major FP64 unless stated.
+ */
+#ifndef KYNEMA_AMD_H
+#define KYNEMA_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---- */
+#define KYN_OK 0
+#define KYN_NOT_CONVERGED 1 /* Step returned false (src/step/step.hpp:55-57) */
+#define KYN_ERR_INVALID (-1)
+#define KYN_ERR_HIP (-2)
+#define KYN_ERR_UNSUPPORTED (-3)
+
+/* ---- constraint types (src/constraints/constraint_type.hpp:9-21) ---- */
+#define KYN_FIXED_BC 1
+#define KYN_PRESCRIBED_BC 2
+#define KYN_RIGID_JOINT 3
+#define KYN_REVOLUTE_JOINT 4
+
+/* ---- execution form ---- */
+#define KYN_EXEC_PLAIN 0 /* one kernel per stage, host reads the error every iteration */
+#define KYN_EXEC_FUSED 1 /* one kernel per time step, Newton loop on the device */
+
+/* ---- tangent precision ---- */
+#define KYN_TANGENT_FP64 0
+#define KYN_TANGENT_FP32 1 /* FP32 tangent + FP32 LU, FP64 residual/update, FP64 refinement */
+
+/* ---- state fields for kyn_get_state / kyn_set_state (src/state/state.hpp:29-38) ---- */
+#define KYN_X0 0      /* [n][7] */
+#define KYN_X 1       /* [n][7] */
+#define KYN_Q_PREV 2  /* [n][7] */
+#define KYN_Q 3       /* [n][7] */
+#define KYN_Q_DELTA 4 /* [n][6] */
+#define KYN_V 5       /* [n][6] */
+#define KYN_VD 6      /* [n][6] */
+#define KYN_A 7       /* [n][6] */
+#define KYN_F 8       /* [n][6] */
+#define KYN_TANGENT 9 /* [n][36] */
+
+typedef struct kyn_model_s* kyn_model;
+typedef struct kyn_system_s* kyn_system;
+
+typedef struct {
+    int is_dynamic_solve; /* StepParameters::is_dynamic_solve */
+    size_t max_iter;      /* StepParameters::max_iter */
+    double h;             /* time step */
+    double rho_inf;       /* generalized-alpha spectral radius */
+    double atol;          /* absolute_convergence_tol (reference default 1e-5) */
+    double rtol;          /* relative_convergence_tol (reference default 1e-3) */
+} kyn_step_params;
+
+typedef struct {
+    int exec_mode;     /* KYN_EXEC_PLAIN or KYN_EXEC_FUSED */
+    int precision;     /* KYN_TANGENT_FP64 or KYN_TANGENT_FP32 */
+    int refine_sweeps; /* FP64 iterative-refinement sweeps of the FP32 solve (0..8) */
+} kyn_exec_options;
+
+/* Last error message of the calling thread (empty if none). */
+const char* kyn_last_error(void);
+/* Device name and architecture of the current HIP device, e.g. "gfx950". */
+int kyn_device_info(char* name, size_t name_cap, char* arch, size_t arch_cap);
+
+/* ---- model building: src/model/model.hpp ---- */
+kyn_model kyn_model_create(void);                                 /* Model() */
+void kyn_model_destroy(kyn_model m);
+int kyn_model_set_gravity(kyn_model m, const double g[3]);        /* Model::SetGravity :97 */
+/* Model::AddNode + NodeBuilder (model.hpp:112-116, node.hpp:26-49): returns the node id */
+int64_t kyn_model_add_node(kyn_model m, const double x0[7], const double u[7], const double v[6],
+                           const double vd[6], double s);
+/* Model::AddBeamElement (model.hpp:161-169): sections {s, M*[36], C*[36]}, quadrature {xi, w} */
+int64_t kyn_model_add_beam_element(kyn_model m, size_t num_nodes, const size_t* node_ids,
+                                   size_t num_sections, const double* section_s,
+                                   const double* section_mass, const double* section_stiffness,
+                                   size_t num_qps, const double* qp_xi_w);
+/* Model::AddFixedBC / AddPrescribedBC / AddRigidJointConstraint / AddRevoluteJointConstraint
+ * (model.hpp:477-541). For one-node constraints `base` is ignored. axis and init may be NULL. */
+int64_t kyn_model_add_constraint(kyn_model m, int type, size_t base, size_t target,
+                                 const double axis[3], const double init_disp[7]);
+
+/* ---- system: Model::CreateSystemWithSolver (model.hpp:647-656) ----
+ * A system advances `num_models` independent models at once (each keeps the reference's own
+ * Step semantics: its own Newton loop, convergence test and history).  With num_models == 1
+ * it is exactly CreateSystemWithSolver of that model. */
+kyn_system kyn_system_create(const kyn_model* models, size_t num_models,
+                             const kyn_step_params* params, const kyn_exec_options* options);
+void kyn_system_destroy(kyn_system s);
+size_t kyn_system_num_models(kyn_system s);
+size_t kyn_system_num_nodes(kyn_system s);       /* all models, concatenated in order */
+size_t kyn_system_num_constraints(kyn_system s);
+size_t kyn_system_num_dofs(kyn_system s, size_t model);   /* Solver::num_dofs */
+size_t kyn_system_node_offset(kyn_system s, size_t model);
+size_t kyn_system_constraint_offset(kyn_system s, size_t model);
+
+/* ---- stepping: Step (step.hpp:42-124) ----
+ * Advances every model by one step.  Returns KYN_OK if every model converged,
+ * KYN_NOT_CONVERGED if any did not (per-model status via kyn_get_step_status). */
+int kyn_step(kyn_system s);
+/* per model: 0 converged / 1 not converged, and Newton iteration count of the last step */
+int kyn_get_step_status(kyn_system s, int* status, int* iterations);
+/* Solver::convergence_err of one model for the last step; returns the count */
+size_t kyn_get_convergence_history(kyn_system s, size_t model, double* err, size_t cap);
+/* Device time (ms) spent in the last kyn_step, measured with HIP events on the system stream */
+double kyn_last_step_ms(kyn_system s);
+/* Average duration (ms) of the step kernel (fused form) or of the element kernel (plain form)
+ * over the last kyn_step, from HIP events recorded around those launches */
+double kyn_last_kernel_ms(kyn_system s);
+
+/* ---- per-step inputs ---- */
+/* Constraints::UpdateDisplacement (constraints.hpp:338-343); id is system-wide */
+int kyn_set_constraint_displacement(kyn_system s, size_t id, const double disp[7]);
+int kyn_set_constraint_displacements(kyn_system s, size_t count, const size_t* ids,
+                                     const double* disp /* [count][7] */);
+/* control signal read at UpdateViews (constraints.hpp:346-353): revolute torque */
+int kyn_set_constraint_control(kyn_system s, size_t id, double value);
+/* State::f nodal loads (blade_interface.cpp:56-61), all nodes [n][6] */
+int kyn_set_nodal_loads(kyn_system s, const double* f);
+/* Beams::node_FX of one element [N][6] (test_cantilever_beam.cpp:105-107) */
+int kyn_set_element_node_loads(kyn_system s, size_t element, const double* fx);
+/* Device-resident input schedule: step k (counted from this call) takes
+ *   constraint displacements cdisp[k][num_constraints][7] (NULL: unchanged) and
+ *   nodal loads loads[k][num_loaded][6] on nodes load_nodes[num_loaded] (NULL: unchanged).
+ * Replaces per-step host calls for long runs; no counterpart in the reference. */
+int kyn_set_input_schedule(kyn_system s, size_t num_steps, const double* cdisp,
+                           size_t num_loaded, const size_t* load_nodes, const double* loads);
+
+/* ---- state access: State views (state.hpp:29-38) ---- */
+int kyn_get_state(kyn_system s, int field, double* out);
+int kyn_set_state(kyn_system s, int field, const double* in);
+/* constraint lambda (6), residual_terms (6) or output (3): which = 0, 1, 2 */
+int kyn_get_constraint_values(kyn_system s, size_t id, int which, double* out);
+/* Solver::b and Solver::x of one model in the reference's global dof order
+ * (b: as assembled, before ConditionR; x: after UnconditionSolution) */
+int kyn_get_rhs(kyn_system s, size_t model, double* b);
+int kyn_get_solution(kyn_system s, size_t model, double* x);
+
+/* ---- state management: CopyStateData (copy_state_data.hpp:20-34), restart files ---- */
+int kyn_save_state(kyn_system s);
+int kyn_restore_state(kyn_system s);
+/* byte image of WriteStateToFile (write_state_to_file.hpp:19-58) for one model */
+size_t kyn_restart_size(kyn_system s, size_t model);
+int kyn_write_restart(kyn_system s, size_t model, unsigned char* out);
+int kyn_read_restart(kyn_system s, size_t model, const unsigned char* in, size_t size);
+
+/* ---- element-level access (tests): first assembly of a step without solving
+ * (the sequence of test_solver.cpp:97-112), and the element terms it produced ---- */
+int kyn_debug_assemble(kyn_system s);
+int kyn_get_element_terms(kyn_system s, size_t element, double* residual /* [N][6] */,
+                          double* system_matrix /* [N][N][36] */);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* KYNEMA_AMD_H */

@@ -1,0 +1,1177 @@
+// Host side of the MI355X beam Newton path: the Model facade, one-time setup on the CPU
+// (shape functions, QP interpolation, DOF tables, independent blocks), upload to SoA device
+// buffers, the step drivers of both execution forms, and the C-ABI of include/kynema_amd.h.
+//
+// Setup mirrors src/model/model.hpp (CreateSystem, CreateSolver), src/elements/beams/
+// create_beams.hpp + populate_element_views.hpp + calculate_jacobian.hpp, src/constraints/
+// constraints.hpp:76-266 and src/dof_management/*.hpp.  It runs once per system.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <numeric>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/kynema_amd.h"
+#include "kyn_system.hpp"
+
+namespace kyn {
+size_t fused_lds_bytes(int Nm, int Qm, int nm, int precision);
+size_t element_lds_bytes(int Nm, int Qm);
+size_t solve_lds_bytes(int nm);
+hipError_t set_lds_limits(size_t bytes);
+void launch_fused_step(const DevSys& S, int Nm, int Qm, int nm, hipStream_t st);
+void launch_advance_slot(const DevSys& S, hipStream_t st);
+void launch_plain_predict(const DevSys& S, hipStream_t st);
+void launch_plain_iteration(const DevSys& S, int Nm, int Qm, int nm, int iter, bool solve,
+                            hipStream_t st, hipEvent_t e0, hipEvent_t e1);
+void launch_plain_epilogue(const DevSys& S, hipStream_t st);
+}  // namespace kyn
+
+namespace {
+
+thread_local std::string g_err;
+
+struct KynError : std::runtime_error {
+    int code;
+    KynError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw KynError(KYN_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// ---------------------------------------------------------------------------------------
+// Model (src/model/model.hpp, node.hpp, elements/beams/beam_element.hpp, constraint.hpp)
+// ---------------------------------------------------------------------------------------
+struct HNode {
+    std::array<double, 7> x0, u;
+    std::array<double, 6> v, vd;
+    double s;
+};
+struct HSection {
+    double s;
+    std::array<double, 36> M, C;
+};
+struct HBeam {
+    std::vector<size_t> nodes;
+    std::vector<HSection> sections;
+    std::vector<std::array<double, 2>> quad;
+};
+struct HConstraint {
+    int type;
+    size_t base, target;
+    std::array<double, 3> axis;
+    std::array<double, 7> init;
+};
+
+}  // namespace
+
+struct kyn_model_s {
+    std::vector<HNode> nodes;
+    std::vector<HBeam> beams;
+    std::vector<HConstraint> constraints;
+    std::array<double, 3> gravity{0., 0., 0.};
+};
+
+namespace {
+
+// ---------------------------------------------------------------------------------------
+// Setup math (host, once): Lagrange shape functions and derivatives at the QPs
+// (src/math/interpolation.hpp:69-123), linear section interpolation (:21-43)
+// ---------------------------------------------------------------------------------------
+void lagrange_values(double x, const std::vector<double>& xs, std::vector<double>& w) {
+    const size_t n = xs.size();
+    w.assign(n, 1.);
+    std::vector<double> dx(n);
+    for (size_t m = 0; m < n; ++m) dx[m] = x - xs[m];
+    for (size_t j = 0; j < n; ++j)
+        for (size_t m = 0; m < n; ++m)
+            if (m != j) w[j] *= dx[m] / (xs[j] - xs[m]);
+}
+void lagrange_derivs(double x, const std::vector<double>& xs, std::vector<double>& w) {
+    const size_t n = xs.size();
+    w.assign(n, 0.);
+    for (size_t i = 0; i < n; ++i) {
+        double sum = 0.;
+        for (size_t j = 0; j < n; ++j) {
+            if (j == i) continue;
+            double prod = 1.;
+            for (size_t k = 0; k < n; ++k)
+                if (k != i && k != j) prod *= (x - xs[k]) / (xs[i] - xs[k]);
+            sum += prod / (xs[i] - xs[j]);
+        }
+        w[i] = sum;
+    }
+}
+void linear_weights(double x, const std::vector<double>& xs, std::vector<double>& w) {
+    w.assign(xs.size(), 0.);
+    const auto it = std::lower_bound(xs.begin(), xs.end(), x);
+    if (it == xs.begin()) { w.front() = 1.; return; }
+    if (it == xs.end()) { w.back() = 1.; return; }
+    const size_t k = static_cast<size_t>(it - xs.begin());
+    const double t = (x - xs[k - 1]) / (xs[k] - xs[k - 1]);
+    w[k - 1] = 1. - t;
+    w[k] = t;
+}
+std::array<double, 4> qcompose(const std::array<double, 4>& a, const std::array<double, 4>& b) {
+    return {a[0] * b[0] - a[1] * b[1] - a[2] * b[2] - a[3] * b[3],
+            a[0] * b[1] + a[1] * b[0] + a[2] * b[3] - a[3] * b[2],
+            a[0] * b[2] - a[1] * b[3] + a[2] * b[0] + a[3] * b[1],
+            a[0] * b[3] + a[1] * b[2] - a[2] * b[1] + a[3] * b[0]};
+}
+std::array<double, 3> qrotate(const std::array<double, 4>& q, const std::array<double, 3>& v) {
+    return {(q[0] * q[0] + q[1] * q[1] - q[2] * q[2] - q[3] * q[3]) * v[0] +
+                2. * (q[1] * q[2] - q[0] * q[3]) * v[1] + 2. * (q[1] * q[3] + q[0] * q[2]) * v[2],
+            2. * (q[1] * q[2] + q[0] * q[3]) * v[0] +
+                (q[0] * q[0] - q[1] * q[1] + q[2] * q[2] - q[3] * q[3]) * v[1] +
+                2. * (q[2] * q[3] - q[0] * q[1]) * v[2],
+            2. * (q[1] * q[3] - q[0] * q[2]) * v[0] + 2. * (q[2] * q[3] + q[0] * q[1]) * v[1] +
+                (q[0] * q[0] - q[1] * q[1] - q[2] * q[2] + q[3] * q[3]) * v[2]};
+}
+
+size_t uf_root(std::vector<size_t>& p, size_t i) {
+    while (p[i] != i) {
+        p[i] = p[p[i]];
+        i = p[i];
+    }
+    return i;
+}
+
+// Device allocation helper: every buffer is a separate hipMalloc, freed with the system
+struct Arena {
+    std::vector<void*> ptrs;
+    template <typename T>
+    T* up(const std::vector<T>& h) {
+        T* d = nullptr;
+        const size_t bytes = std::max<size_t>(h.size(), 1) * sizeof(T);
+        hip_check(hipMalloc(&d, bytes), "hipMalloc");
+        ptrs.push_back(d);
+        if (!h.empty()) hip_check(hipMemcpy(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice), "upload");
+        return d;
+    }
+    template <typename T>
+    T* zeros(size_t n) {
+        T* d = nullptr;
+        hip_check(hipMalloc(&d, std::max<size_t>(n, 1) * sizeof(T)), "hipMalloc");
+        ptrs.push_back(d);
+        hip_check(hipMemset(d, 0, std::max<size_t>(n, 1) * sizeof(T)), "hipMemset");
+        return d;
+    }
+    ~Arena() {
+        for (void* p : ptrs) (void)hipFree(p);
+    }
+};
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------------
+// System
+// ---------------------------------------------------------------------------------------
+struct kyn_system_s {
+    kyn::DevSys S{};
+    Arena arena;
+    hipStream_t stream{};
+    hipEvent_t ev0{}, ev1{}, ek0{}, ek1{};
+    int exec_mode{KYN_EXEC_FUSED};
+    int64_t mat_total{0}, A_total{0};
+    int Nm{1}, Qm{1}, nm{1};
+    float last_step_ms{0.f}, last_kernel_ms{0.f};
+    // host copies of the layout
+    size_t num_models{}, num_nodes{}, num_cons{}, num_elems{}, num_blocks{};
+    std::vector<int> model_node_begin, model_node_end, model_con_begin, model_con_end;
+    std::vector<int> model_ndofs, model_nsys;
+    std::vector<int> elem_N, elem_node_off;
+    std::vector<int> block_n, block_vec_off, block_model;
+    std::vector<int> node_block, node_local;
+    std::vector<int> con_local, con_rows, con_model_row, con_block;
+    std::vector<std::vector<int>> model_gdof_to_block_vec;  // per model: global dof -> rhs index
+    // saved state (CopyStateData)
+    std::vector<double> saved;
+    bool has_saved{false};
+    // graph of one fused step
+    hipGraph_t graph{};
+    hipGraphExec_t graph_exec{};
+    bool graph_ready{false};
+
+    ~kyn_system_s() {
+        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (ek0) (void)hipEventDestroy(ek0);
+        if (ek1) (void)hipEventDestroy(ek1);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+struct Builder {
+    // concatenated host arrays (become DevSys arrays)
+    std::vector<double> x0, q, v, vd, a, f;
+    std::vector<int> model_node_begin, model_node_end, model_block_begin, model_block_end,
+        model_con_begin, model_con_end, model_ndofs;
+    std::vector<double> model_gravity;
+    std::vector<int> block_model, block_n, block_nsys, block_elem_begin, block_elem_end, block_elems,
+        block_con_begin, block_con_end, block_cons, block_node_begin, block_node_end, block_nodes,
+        block_vec_off;
+    std::vector<int64_t> block_A_off;
+    std::vector<int> node_block, node_local;
+    std::vector<int> elem_N, elem_Q, elem_node_off, elem_nodes, elem_qp_off, elem_shape_off;
+    std::vector<int64_t> elem_mat_off;
+    std::vector<double> qp_weight, qp_jac, qp_Mstar, qp_Cstar, qp_x0p, qp_r0, shape_interp, shape_deriv;
+    std::vector<int> con_type, con_base, con_target, con_rows, con_model_row, con_local, con_block;
+    std::vector<double> con_X0, con_axes, con_input;
+    std::map<std::string, int> shape_tables;
+    std::vector<std::vector<int>> gdof_maps;
+    std::vector<int> model_nsys;
+    int64_t A_total = 0;
+    int vec_total = 0;
+    int64_t mat_total = 0;
+};
+
+void add_model(Builder& B, const kyn_model_s& mo) {
+    const int node0 = static_cast<int>(B.node_block.size());
+    const int con0 = static_cast<int>(B.con_type.size());
+    const int elem0 = static_cast<int>(B.elem_N.size());
+    const int blk0 = static_cast<int>(B.block_n.size());
+    const size_t n = mo.nodes.size();
+    const int mid = static_cast<int>(B.model_node_begin.size());
+
+    // ---- state (copy_nodes_to_state.hpp:13-60)
+    for (const auto& nd : mo.nodes) {
+        for (int c = 0; c < 7; ++c) {
+            B.x0.push_back(nd.x0[c]);
+            B.q.push_back(nd.u[c]);
+        }
+        for (int c = 0; c < 6; ++c) {
+            B.v.push_back(nd.v[c]);
+            B.vd.push_back(nd.vd[c]);
+            B.a.push_back(nd.vd[c]);
+            B.f.push_back(0.);
+        }
+    }
+    for (size_t i = 0; i < n; ++i) {
+        B.node_block.push_back(-1);
+        B.node_local.push_back(-1);
+    }
+    // ---- validate
+    for (const auto& bm : mo.beams) {
+        if (bm.nodes.size() < 2 || bm.nodes.size() > static_cast<size_t>(kyn::kMaxElemNodes))
+            throw KynError(KYN_ERR_UNSUPPORTED, "beam element node count must be in [2, 21]");
+        if (bm.quad.empty() || bm.quad.size() > static_cast<size_t>(kyn::kMaxElemQps))
+            throw KynError(KYN_ERR_UNSUPPORTED, "beam element quadrature size must be in [1, 64]");
+        for (auto nid : bm.nodes)
+            if (nid >= n) throw KynError(KYN_ERR_INVALID, "beam element references an unknown node");
+        if (bm.sections.empty()) throw KynError(KYN_ERR_INVALID, "beam element without sections");
+    }
+    for (const auto& c : mo.constraints) {
+        if (c.type < 1 || c.type > 4) throw KynError(KYN_ERR_UNSUPPORTED, "unsupported constraint type");
+        if (c.target >= n || ((c.type == 3 || c.type == 4) && c.base >= n))
+            throw KynError(KYN_ERR_INVALID, "constraint references an unknown node");
+    }
+
+    // ---- DOF tables (assemble_node_freedom_allocation_table.hpp, compute_node_freedom_map_table.hpp)
+    std::vector<int> has(n, 0);
+    for (const auto& bm : mo.beams)
+        for (auto nid : bm.nodes) has[nid] = 1;
+    for (const auto& c : mo.constraints) {
+        has[c.target] = 1;
+        if (c.type == 3 || c.type == 4) has[c.base] = 1;
+    }
+    std::vector<int> dof_map(n, 0);
+    int nsys = 0;
+    for (size_t i = 0; i < n; ++i) {
+        dof_map[i] = nsys;
+        nsys += has[i] ? 6 : 0;
+    }
+    std::vector<int> row_first(mo.constraints.size());
+    int nrows = 0;
+    for (size_t k = 0; k < mo.constraints.size(); ++k) {
+        row_first[k] = nrows;
+        nrows += mo.constraints[k].type == 4 ? 5 : 6;
+    }
+    const int ndofs = nsys + nrows;
+
+    // ---- blocks (connected components)
+    std::vector<size_t> parent(n);
+    std::iota(parent.begin(), parent.end(), size_t{0});
+    auto unite = [&](size_t a, size_t b) {
+        a = uf_root(parent, a);
+        b = uf_root(parent, b);
+        if (a != b) parent[std::max(a, b)] = std::min(a, b);
+    };
+    for (const auto& bm : mo.beams)
+        for (size_t k = 1; k < bm.nodes.size(); ++k) unite(bm.nodes[0], bm.nodes[k]);
+    for (const auto& c : mo.constraints)
+        if (c.type == 3 || c.type == 4) unite(c.base, c.target);
+    std::map<size_t, int> root_block;
+    std::vector<std::vector<int>> bnodes, bcons, belems;
+    for (size_t i = 0; i < n; ++i) {
+        if (!has[i]) continue;
+        const size_t r = uf_root(parent, i);
+        if (!root_block.count(r)) {
+            root_block[r] = static_cast<int>(bnodes.size());
+            bnodes.emplace_back();
+            bcons.emplace_back();
+            belems.emplace_back();
+        }
+        bnodes[root_block[r]].push_back(static_cast<int>(i));
+    }
+    for (size_t k = 0; k < mo.constraints.size(); ++k)
+        bcons[root_block[uf_root(parent, mo.constraints[k].target)]].push_back(static_cast<int>(k));
+    for (size_t k = 0; k < mo.beams.size(); ++k)
+        belems[root_block[uf_root(parent, mo.beams[k].nodes[0])]].push_back(static_cast<int>(k));
+
+    std::vector<int> gmap(ndofs, -1);  // model dof -> rhs index
+    std::vector<int> con_loc(mo.constraints.size(), 0), con_blk(mo.constraints.size(), 0);
+    for (size_t bi = 0; bi < bnodes.size(); ++bi) {
+        const int gb = blk0 + static_cast<int>(bi);
+        int l = 0;
+        B.block_node_begin.push_back(static_cast<int>(B.block_nodes.size()));
+        for (int nd : bnodes[bi]) {
+            B.block_nodes.push_back(node0 + nd);
+            B.node_block[node0 + nd] = gb;
+            B.node_local[node0 + nd] = l;
+            for (int c = 0; c < 6; ++c) gmap[dof_map[nd] + c] = B.vec_total + l + c;
+            l += 6;
+        }
+        B.block_node_end.push_back(static_cast<int>(B.block_nodes.size()));
+        const int bnsys = l;
+        B.block_con_begin.push_back(static_cast<int>(B.block_cons.size()));
+        for (int ck : bcons[bi]) {
+            B.block_cons.push_back(con0 + ck);
+            con_loc[ck] = l;
+            con_blk[ck] = gb;
+            const int rows = mo.constraints[ck].type == 4 ? 5 : 6;
+            for (int r = 0; r < rows; ++r) gmap[nsys + row_first[ck] + r] = B.vec_total + l + r;
+            l += rows;
+        }
+        B.block_con_end.push_back(static_cast<int>(B.block_cons.size()));
+        B.block_elem_begin.push_back(static_cast<int>(B.block_elems.size()));
+        for (int ek : belems[bi]) B.block_elems.push_back(elem0 + ek);
+        B.block_elem_end.push_back(static_cast<int>(B.block_elems.size()));
+        B.block_model.push_back(mid);
+        B.block_n.push_back(l);
+        B.block_nsys.push_back(bnsys);
+        B.block_vec_off.push_back(B.vec_total);
+        B.block_A_off.push_back(B.A_total);
+        B.vec_total += l;
+        B.A_total += static_cast<int64_t>(l) * (l + 1);
+    }
+    B.gdof_maps.push_back(gmap);
+    B.model_nsys.push_back(nsys);
+
+    // ---- elements (create_beams.hpp:26-165)
+    for (const auto& bm : mo.beams) {
+        const int N = static_cast<int>(bm.nodes.size()), Q = static_cast<int>(bm.quad.size());
+        B.elem_N.push_back(N);
+        B.elem_Q.push_back(Q);
+        B.elem_node_off.push_back(static_cast<int>(B.elem_nodes.size()));
+        for (auto nid : bm.nodes) B.elem_nodes.push_back(node0 + static_cast<int>(nid));
+        B.elem_qp_off.push_back(static_cast<int>(B.qp_weight.size()));
+        B.elem_mat_off.push_back(B.mat_total);
+        B.mat_total += static_cast<int64_t>(N) * N * 36;
+        std::vector<double> node_xi(N), phi(N * Q), dphi(N * Q), w;
+        for (int k = 0; k < N; ++k) node_xi[k] = 2. * mo.nodes[bm.nodes[k]].s - 1.;
+        for (int q = 0; q < Q; ++q) {
+            lagrange_values(bm.quad[q][0], node_xi, w);
+            for (int k = 0; k < N; ++k) phi[k * Q + q] = w[k];
+            lagrange_derivs(bm.quad[q][0], node_xi, w);
+            for (int k = 0; k < N; ++k) dphi[k * Q + q] = w[k];
+        }
+        std::string key(reinterpret_cast<const char*>(node_xi.data()), node_xi.size() * sizeof(double));
+        for (const auto& qq : bm.quad) key.append(reinterpret_cast<const char*>(&qq[0]), sizeof(double));
+        auto it = B.shape_tables.find(key);
+        if (it == B.shape_tables.end()) {
+            const int off = static_cast<int>(B.shape_interp.size());
+            B.shape_interp.insert(B.shape_interp.end(), phi.begin(), phi.end());
+            B.shape_deriv.insert(B.shape_deriv.end(), dphi.begin(), dphi.end());
+            it = B.shape_tables.emplace(key, off).first;
+        }
+        B.elem_shape_off.push_back(it->second);
+        // sections (populate_element_views.hpp:89-141)
+        std::vector<double> sxi(bm.sections.size());
+        for (size_t k = 0; k < sxi.size(); ++k) sxi[k] = 2. * bm.sections[k].s - 1.;
+        for (int q = 0; q < Q; ++q) {
+            B.qp_weight.push_back(bm.quad[q][1]);
+            linear_weights(bm.quad[q][0], sxi, w);
+            std::array<double, 36> M{}, C{};
+            for (size_t k = 0; k < sxi.size(); ++k)
+                for (int c = 0; c < 36; ++c) {
+                    M[c] += bm.sections[k].M[c] * w[k];
+                    C[c] += bm.sections[k].C[c] * w[k];
+                }
+            B.qp_Mstar.insert(B.qp_Mstar.end(), M.begin(), M.end());
+            B.qp_Cstar.insert(B.qp_Cstar.end(), C.begin(), C.end());
+            // interpolate_QP_rotation.hpp / calculate_jacobian.hpp
+            double d[3] = {0., 0., 0.}, r[4] = {0., 0., 0., 0.};
+            for (int k = 0; k < N; ++k) {
+                const auto& x0 = mo.nodes[bm.nodes[k]].x0;
+                for (int c = 0; c < 3; ++c) d[c] += x0[c] * dphi[k * Q + q];
+                for (int c = 0; c < 4; ++c) r[c] += x0[3 + c] * phi[k * Q + q];
+            }
+            const double len = std::sqrt(std::pow(r[0], 2) + std::pow(r[1], 2) + std::pow(r[2], 2) + std::pow(r[3], 2));
+            if (len == 0.) {
+                r[0] = 1.;
+                r[1] = r[2] = r[3] = 0.;
+            } else
+                for (double& c : r) c /= len;
+            const double J = std::sqrt(std::pow(d[0], 2.) + std::pow(d[1], 2.) + std::pow(d[2], 2.));
+            B.qp_jac.push_back(J);
+            for (double c : d) B.qp_x0p.push_back(c / J);
+            for (double c : r) B.qp_r0.push_back(c);
+        }
+    }
+
+    // ---- constraints (constraints.hpp:76-266)
+    for (size_t k = 0; k < mo.constraints.size(); ++k) {
+        const auto& c = mo.constraints[k];
+        const bool two = (c.type == 3 || c.type == 4);
+        B.con_type.push_back(c.type);
+        B.con_base.push_back(two ? node0 + static_cast<int>(c.base) : node0 + static_cast<int>(c.target));
+        B.con_target.push_back(node0 + static_cast<int>(c.target));
+        B.con_rows.push_back(c.type == 4 ? 5 : 6);
+        B.con_model_row.push_back(row_first[k]);
+        B.con_local.push_back(con_loc[k]);
+        B.con_block.push_back(con_blk[k]);
+        std::array<double, 3> X0{0., 0., 0.};
+        if (c.type != 2) {
+            const auto& t = mo.nodes[c.target].x0;
+            if (!two) X0 = {t[0] - c.axis[0], t[1] - c.axis[1], t[2] - c.axis[2]};
+            else {
+                const auto& bb = mo.nodes[c.base].x0;
+                X0 = {t[0] - bb[0], t[1] - bb[1], t[2] - bb[2]};
+            }
+        }
+        for (double v : X0) B.con_X0.push_back(v);
+        std::array<double, 9> ax{};
+        if (c.type == 4) {
+            const double na = std::sqrt(c.axis[0] * c.axis[0] + c.axis[1] * c.axis[1] + c.axis[2] * c.axis[2]);
+            std::array<double, 3> xh{};
+            if (na != 0.) xh = {c.axis[0] / na, c.axis[1] / na, c.axis[2] / na};
+            else {
+                const double nx = std::sqrt(X0[0] * X0[0] + X0[1] * X0[1] + X0[2] * X0[2]);
+                if (nx == 0.) throw KynError(KYN_ERR_INVALID, "Cannot normalize a zero vector");
+                xh = {X0[0] / nx, X0[1] / nx, X0[2] / nx};
+            }
+            const std::array<double, 3> cp{0. * xh[2] - 0. * xh[1], 0. * xh[0] - 1. * xh[2], 1. * xh[1] - 0. * xh[0]};
+            const double dp = xh[0];
+            const double kk = 1. / (1. + dp);
+            ax = {cp[0] * cp[0] * kk + dp, cp[1] * cp[0] * kk + cp[2], cp[2] * cp[0] * kk - cp[1],
+                  cp[0] * cp[1] * kk - cp[2], cp[1] * cp[1] * kk + dp, cp[2] * cp[1] * kk + cp[0],
+                  cp[0] * cp[2] * kk + cp[1], cp[1] * cp[2] * kk - cp[0], cp[2] * cp[2] * kk + dp};
+        } else {
+            ax[0] = c.axis[0];
+            ax[1] = c.axis[1];
+            ax[2] = c.axis[2];
+        }
+        for (double v : ax) B.con_axes.push_back(v);
+        std::array<double, 7> in{0., 0., 0., 1., 0., 0., 0.};
+        if (c.type == 2) in = c.init;
+        for (double v : in) B.con_input.push_back(v);
+    }
+
+    B.model_node_begin.push_back(node0);
+    B.model_node_end.push_back(node0 + static_cast<int>(n));
+    B.model_block_begin.push_back(blk0);
+    B.model_block_end.push_back(static_cast<int>(B.block_n.size()));
+    B.model_con_begin.push_back(con0);
+    B.model_con_end.push_back(con0 + static_cast<int>(mo.constraints.size()));
+    B.model_ndofs.push_back(ndofs);
+    for (double g : mo.gravity) B.model_gravity.push_back(g);
+}
+
+template <typename F>
+int guarded(F&& f) {
+    try {
+        g_err.clear();
+        return f();
+    } catch (const KynError& e) {
+        g_err = e.what();
+        return e.code;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return KYN_ERR_INVALID;
+    }
+}
+
+double* field_ptr(kyn_system_s* s, int field, int& width) {
+    auto& S = s->S;
+    switch (field) {
+        case KYN_X0: width = 7; return S.x0;
+        case KYN_X: width = 7; return S.x;
+        case KYN_Q_PREV: width = 7; return S.q_prev;
+        case KYN_Q: width = 7; return S.q;
+        case KYN_Q_DELTA: width = 6; return S.q_delta;
+        case KYN_V: width = 6; return S.v;
+        case KYN_VD: width = 6; return S.vd;
+        case KYN_A: width = 6; return S.a;
+        case KYN_F: width = 6; return S.f;
+        default: return nullptr;
+    }
+}
+
+int run_step(kyn_system_s* s, bool solve = true) {
+    auto& S = s->S;
+    hip_check(hipEventRecord(s->ev0, s->stream), "event");
+    float kms = 0.f;
+    if (s->exec_mode == KYN_EXEC_FUSED && solve) {
+        hip_check(hipEventRecord(s->ek0, s->stream), "event");
+        kyn::launch_fused_step(S, s->Nm, s->Qm, s->nm, s->stream);
+        hip_check(hipGetLastError(), "k_fused_step launch");
+        hip_check(hipEventRecord(s->ek1, s->stream), "event");
+        kyn::launch_advance_slot(S, s->stream);
+    } else {
+        kyn::launch_plain_predict(S, s->stream);
+        hip_check(hipGetLastError(), "k_plain_predict launch");
+        int count = 0;
+        for (int iter = 0; iter < S.p.max_iter; ++iter) {
+            kyn::launch_plain_iteration(S, s->Nm, s->Qm, s->nm, iter, solve, s->stream, s->ek0, s->ek1);
+            hip_check(hipGetLastError(), "plain iteration launch");
+            if (!solve) break;
+            int any = 0;
+            hip_check(hipMemcpyAsync(&any, S.any_active, sizeof(int), hipMemcpyDeviceToHost, s->stream), "D2H");
+            hip_check(hipStreamSynchronize(s->stream), "sync");
+            float ms = 0.f;
+            hip_check(hipEventElapsedTime(&ms, s->ek0, s->ek1), "elapsed");
+            kms += ms;
+            ++count;
+            if (!any) break;
+        }
+        if (solve) {
+            kyn::launch_plain_epilogue(S, s->stream);
+            kyn::launch_advance_slot(S, s->stream);
+        }
+        if (count > 0) kms /= static_cast<float>(count);
+    }
+    hip_check(hipEventRecord(s->ev1, s->stream), "event");
+    hip_check(hipStreamSynchronize(s->stream), "sync");
+    float ms = 0.f;
+    hip_check(hipEventElapsedTime(&ms, s->ev0, s->ev1), "elapsed");
+    s->last_step_ms = ms;
+    if (s->exec_mode == KYN_EXEC_FUSED && solve) {
+        hip_check(hipEventElapsedTime(&kms, s->ek0, s->ek1), "elapsed");
+    }
+    s->last_kernel_ms = kms;
+    if (!solve) return KYN_OK;
+    std::vector<int> st(s->num_models);
+    hip_check(hipMemcpy(st.data(), S.model_status, st.size() * sizeof(int), hipMemcpyDeviceToHost), "D2H");
+    for (int v : st)
+        if (v != 0) return KYN_NOT_CONVERGED;
+    return KYN_OK;
+}
+
+}  // namespace
+
+// =======================================================================================
+// C-ABI
+// =======================================================================================
+extern "C" {
+
+const char* kyn_last_error(void) { return g_err.c_str(); }
+
+int kyn_device_info(char* name, size_t name_cap, char* arch, size_t arch_cap) {
+    return guarded([&] {
+        int dev = 0;
+        hip_check(hipGetDevice(&dev), "hipGetDevice");
+        hipDeviceProp_t prop;
+        hip_check(hipGetDeviceProperties(&prop, dev), "hipGetDeviceProperties");
+        if (name && name_cap) {
+            std::strncpy(name, prop.name, name_cap - 1);
+            name[name_cap - 1] = 0;
+        }
+        if (arch && arch_cap) {
+            std::strncpy(arch, prop.gcnArchName, arch_cap - 1);
+            arch[arch_cap - 1] = 0;
+        }
+        return KYN_OK;
+    });
+}
+
+kyn_model kyn_model_create(void) { return new kyn_model_s(); }
+void kyn_model_destroy(kyn_model m) { delete m; }
+int kyn_model_set_gravity(kyn_model m, const double g[3]) {
+    if (!m || !g) return KYN_ERR_INVALID;
+    m->gravity = {g[0], g[1], g[2]};
+    return KYN_OK;
+}
+int64_t kyn_model_add_node(kyn_model m, const double x0[7], const double u[7], const double v[6],
+                           const double vd[6], double s) {
+    if (!m || !x0) return KYN_ERR_INVALID;
+    HNode n{};
+    std::copy(x0, x0 + 7, n.x0.begin());
+    if (u) std::copy(u, u + 7, n.u.begin());
+    else n.u = {0., 0., 0., 1., 0., 0., 0.};
+    if (v) std::copy(v, v + 6, n.v.begin());
+    if (vd) std::copy(vd, vd + 6, n.vd.begin());
+    n.s = s;
+    m->nodes.push_back(n);
+    return static_cast<int64_t>(m->nodes.size() - 1);
+}
+int64_t kyn_model_add_beam_element(kyn_model m, size_t num_nodes, const size_t* node_ids,
+                                   size_t num_sections, const double* section_s,
+                                   const double* section_mass, const double* section_stiffness,
+                                   size_t num_qps, const double* qp_xi_w) {
+    if (!m || !node_ids || !section_s || !section_mass || !section_stiffness || !qp_xi_w)
+        return KYN_ERR_INVALID;
+    HBeam b;
+    b.nodes.assign(node_ids, node_ids + num_nodes);
+    for (size_t k = 0; k < num_sections; ++k) {
+        HSection sec{};
+        sec.s = section_s[k];
+        std::copy(section_mass + 36 * k, section_mass + 36 * k + 36, sec.M.begin());
+        std::copy(section_stiffness + 36 * k, section_stiffness + 36 * k + 36, sec.C.begin());
+        b.sections.push_back(sec);
+    }
+    for (size_t q = 0; q < num_qps; ++q) b.quad.push_back({qp_xi_w[2 * q], qp_xi_w[2 * q + 1]});
+    m->beams.push_back(std::move(b));
+    return static_cast<int64_t>(m->beams.size() - 1);
+}
+int64_t kyn_model_add_constraint(kyn_model m, int type, size_t base, size_t target, const double axis[3],
+                                 const double init_disp[7]) {
+    if (!m) return KYN_ERR_INVALID;
+    HConstraint c{};
+    c.type = type;
+    c.base = base;
+    c.target = target;
+    c.axis = {0., 0., 0.};
+    if (axis) std::copy(axis, axis + 3, c.axis.begin());
+    c.init = {0., 0., 0., 1., 0., 0., 0.};
+    if (init_disp) std::copy(init_disp, init_disp + 7, c.init.begin());
+    m->constraints.push_back(c);
+    return static_cast<int64_t>(m->constraints.size() - 1);
+}
+
+kyn_system kyn_system_create(const kyn_model* models, size_t num_models, const kyn_step_params* params,
+                             const kyn_exec_options* options) {
+    kyn_system result = nullptr;
+    const int rc = guarded([&] {
+        if (!models || num_models == 0 || !params) throw KynError(KYN_ERR_INVALID, "null argument");
+        auto sys = std::make_unique<kyn_system_s>();
+        int dev_count = 0;
+        hip_check(hipGetDeviceCount(&dev_count), "hipGetDeviceCount");
+        if (dev_count < 1) throw KynError(KYN_ERR_HIP, "no HIP device");
+        Builder B;
+        for (size_t k = 0; k < num_models; ++k) add_model(B, *models[k]);
+        auto& S = sys->S;
+        // ---- step parameters (step_parameters.hpp:37-52)
+        const double rho = params->rho_inf, h = params->h;
+        const bool dyn = params->is_dynamic_solve != 0;
+        S.p.h = h;
+        S.p.alpha_m = (2. * rho - 1.) / (rho + 1.);
+        S.p.alpha_f = rho / (rho + 1.);
+        S.p.gamma = 0.5 + S.p.alpha_f - S.p.alpha_m;
+        S.p.beta = 0.25 * (S.p.gamma + 0.5) * (S.p.gamma + 0.5);
+        S.p.gamma_prime = dyn ? S.p.gamma / (h * S.p.beta) : 0.;
+        S.p.beta_prime = dyn ? (1. - S.p.alpha_m) / (h * h * S.p.beta * (1. - S.p.alpha_f)) : 0.;
+        S.p.conditioner = S.p.beta * h * h;
+        S.p.atol = params->atol;
+        S.p.rtol = params->rtol;
+        S.p.is_dynamic = dyn ? 1 : 0;
+        S.p.max_iter = static_cast<int>(params->max_iter);
+        if (S.p.max_iter < 1) throw KynError(KYN_ERR_INVALID, "max_iter must be >= 1");
+        sys->exec_mode = options ? options->exec_mode : KYN_EXEC_FUSED;
+        S.precision = options ? options->precision : KYN_TANGENT_FP64;
+        S.refine_sweeps = options ? options->refine_sweeps : 0;
+        if (S.refine_sweeps < 0 || S.refine_sweeps > 8) throw KynError(KYN_ERR_INVALID, "refine_sweeps in [0, 8]");
+        if (sys->exec_mode == KYN_EXEC_PLAIN && S.precision != KYN_TANGENT_FP64)
+            throw KynError(KYN_ERR_UNSUPPORTED, "the plain form computes the tangent in FP64 only");
+
+        // ---- sizes and eligibility
+        S.num_models = static_cast<int>(num_models);
+        S.num_blocks = static_cast<int>(B.block_n.size());
+        S.num_elems = static_cast<int>(B.elem_N.size());
+        S.num_nodes = static_cast<int>(B.node_block.size());
+        S.num_cons = static_cast<int>(B.con_type.size());
+        int Nm = 2, Qm = 1, nm = 1;
+        for (int N : B.elem_N) Nm = std::max(Nm, N);
+        for (int Q : B.elem_Q) Qm = std::max(Qm, Q);
+        for (int n : B.block_n) nm = std::max(nm, n);
+        if (nm > 192) throw KynError(KYN_ERR_UNSUPPORTED, "independent KKT block larger than 192 dofs");
+        sys->Nm = Nm;
+        sys->Qm = Qm;
+        sys->nm = nm;
+        int dev = 0;
+        hip_check(hipGetDevice(&dev), "hipGetDevice");
+        int max_lds = 0;
+        hip_check(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev), "attr");
+        size_t need = 0;
+        if (sys->exec_mode == KYN_EXEC_FUSED) {
+            for (int b = 0; b < S.num_blocks; ++b) {
+                if (B.block_elem_end[b] - B.block_elem_begin[b] != 1)
+                    throw KynError(KYN_ERR_UNSUPPORTED, "the fused form needs exactly one beam element per independent block");
+                if (B.block_con_end[b] - B.block_con_begin[b] > 8)
+                    throw KynError(KYN_ERR_UNSUPPORTED, "the fused form allows at most 8 constraints per block");
+            }
+            need = kyn::fused_lds_bytes(Nm, Qm, nm, S.precision);
+        } else {
+            need = std::max(kyn::element_lds_bytes(Nm, Qm), kyn::solve_lds_bytes(nm));
+        }
+        if (need > static_cast<size_t>(max_lds))
+            throw KynError(KYN_ERR_UNSUPPORTED, "problem needs " + std::to_string(need) + " bytes of LDS, device offers " +
+                                                    std::to_string(max_lds));
+        hip_check(kyn::set_lds_limits(static_cast<size_t>(max_lds)), "hipFuncSetAttribute");
+        for (int b = 0; b < S.num_blocks; ++b)
+            if (B.block_n[b] > 0 && B.block_elem_end[b] == B.block_elem_begin[b])
+                throw KynError(KYN_ERR_UNSUPPORTED, "independent block without beam elements");
+
+        // ---- upload
+        auto& Ar = sys->arena;
+        const size_t n = B.node_block.size();
+        S.x0 = Ar.up(B.x0);
+        S.x = Ar.zeros<double>(n * 7);
+        S.q_prev = Ar.up(B.q);
+        S.q = Ar.up(B.q);
+        S.q_delta = Ar.zeros<double>(n * 6);
+        S.v = Ar.up(B.v);
+        S.vd = Ar.up(B.vd);
+        S.a = Ar.up(B.a);
+        S.f = Ar.up(B.f);
+        S.tan_rot = Ar.zeros<double>(n * 9);
+        S.model_node_begin = Ar.up(B.model_node_begin);
+        S.model_node_end = Ar.up(B.model_node_end);
+        S.model_block_begin = Ar.up(B.model_block_begin);
+        S.model_block_end = Ar.up(B.model_block_end);
+        S.model_con_begin = Ar.up(B.model_con_begin);
+        S.model_con_end = Ar.up(B.model_con_end);
+        S.model_ndofs = Ar.up(B.model_ndofs);
+        S.model_gravity = Ar.up(B.model_gravity);
+        S.model_load_begin = Ar.zeros<int>(num_models + 1);
+        S.block_model = Ar.up(B.block_model);
+        S.block_n = Ar.up(B.block_n);
+        S.block_nsys = Ar.up(B.block_nsys);
+        S.block_elem_begin = Ar.up(B.block_elem_begin);
+        S.block_elem_end = Ar.up(B.block_elem_end);
+        S.block_elems = Ar.up(B.block_elems);
+        S.block_con_begin = Ar.up(B.block_con_begin);
+        S.block_con_end = Ar.up(B.block_con_end);
+        S.block_cons = Ar.up(B.block_cons);
+        S.block_node_begin = Ar.up(B.block_node_begin);
+        S.block_node_end = Ar.up(B.block_node_end);
+        S.block_nodes = Ar.up(B.block_nodes);
+        S.block_A_off = Ar.up(B.block_A_off);
+        S.block_vec_off = Ar.up(B.block_vec_off);
+        S.node_block = Ar.up(B.node_block);
+        S.node_local = Ar.up(B.node_local);
+        S.elem_N = Ar.up(B.elem_N);
+        S.elem_Q = Ar.up(B.elem_Q);
+        S.elem_node_off = Ar.up(B.elem_node_off);
+        S.elem_nodes = Ar.up(B.elem_nodes);
+        S.elem_qp_off = Ar.up(B.elem_qp_off);
+        S.elem_shape_off = Ar.up(B.elem_shape_off);
+        S.elem_mat_off = Ar.up(B.elem_mat_off);
+        S.qp_weight = Ar.up(B.qp_weight);
+        S.qp_jac = Ar.up(B.qp_jac);
+        S.qp_Mstar = Ar.up(B.qp_Mstar);
+        S.qp_Cstar = Ar.up(B.qp_Cstar);
+        S.qp_x0p = Ar.up(B.qp_x0p);
+        S.qp_r0 = Ar.up(B.qp_r0);
+        S.shape_interp = Ar.up(B.shape_interp);
+        S.shape_deriv = Ar.up(B.shape_deriv);
+        S.node_FX = Ar.zeros<double>(B.elem_nodes.size() * 6);
+        S.elem_res = Ar.zeros<double>(B.elem_nodes.size() * 6);
+        // element matrices and dense KKT blocks in HBM exist only in the plain form (the fused
+        // form keeps them on chip); kyn_debug_assemble allocates them on demand
+        S.elem_mat = sys->exec_mode == KYN_EXEC_PLAIN ? Ar.zeros<double>(static_cast<size_t>(B.mat_total)) : nullptr;
+        sys->mat_total = B.mat_total;
+        sys->A_total = B.A_total;
+        const size_t nc = B.con_type.size();
+        S.con_type = Ar.up(B.con_type);
+        S.con_base = Ar.up(B.con_base);
+        S.con_target = Ar.up(B.con_target);
+        S.con_rows = Ar.up(B.con_rows);
+        S.con_model_row = Ar.up(B.con_model_row);
+        S.con_local = Ar.up(B.con_local);
+        S.con_X0 = Ar.up(B.con_X0);
+        S.con_axes = Ar.up(B.con_axes);
+        S.con_input = Ar.up(B.con_input);
+        S.con_control = Ar.zeros<double>(nc);
+        S.con_lambda = Ar.zeros<double>(nc * 6);
+        S.con_res = Ar.zeros<double>(nc * 6);
+        S.con_sys_res = Ar.zeros<double>(nc * 6);
+        S.con_output = Ar.zeros<double>(nc * 3);
+        S.con_tgrad = Ar.zeros<double>(nc * 36);
+        S.con_bgrad = Ar.zeros<double>(nc * 36);
+        S.con_tgradT = Ar.zeros<double>(nc * 36);
+        S.con_bgradT = Ar.zeros<double>(nc * 36);
+        S.con_tlres = Ar.zeros<double>(nc * 6);
+        S.con_blres = Ar.zeros<double>(nc * 6);
+        const size_t tt = S.precision ? sizeof(float) : sizeof(double);
+        S.A = sys->exec_mode == KYN_EXEC_PLAIN ? Ar.zeros<double>(static_cast<size_t>(B.A_total)) : nullptr;
+        S.Acopy = (S.refine_sweeps > 0 && sys->exec_mode == KYN_EXEC_FUSED)
+                      ? static_cast<void*>(Ar.zeros<char>(static_cast<size_t>(B.A_total) * tt))
+                      : nullptr;
+        S.rhs = Ar.zeros<double>(B.vec_total);
+        S.xsol = Ar.zeros<double>(B.vec_total);
+        S.rhs_raw = Ar.zeros<double>(B.vec_total);
+        S.err_hist = Ar.zeros<double>(num_models * static_cast<size_t>(S.p.max_iter));
+        S.model_iters = Ar.zeros<int>(num_models);
+        S.model_status = Ar.zeros<int>(num_models);
+        S.model_active = Ar.zeros<int>(num_models);
+        S.any_active = Ar.zeros<int>(1);
+        S.sched_slot = Ar.zeros<int>(1);
+        S.sched_steps = 0;
+
+        // ---- host copies of the layout
+        sys->num_models = num_models;
+        sys->num_nodes = n;
+        sys->num_cons = nc;
+        sys->num_elems = B.elem_N.size();
+        sys->num_blocks = B.block_n.size();
+        sys->model_node_begin = B.model_node_begin;
+        sys->model_node_end = B.model_node_end;
+        sys->model_con_begin = B.model_con_begin;
+        sys->model_con_end = B.model_con_end;
+        sys->model_ndofs = B.model_ndofs;
+        sys->model_nsys = B.model_nsys;
+        sys->elem_N = B.elem_N;
+        sys->elem_node_off = B.elem_node_off;
+        sys->block_n = B.block_n;
+        sys->block_vec_off = B.block_vec_off;
+        sys->block_model = B.block_model;
+        sys->node_block = B.node_block;
+        sys->node_local = B.node_local;
+        sys->con_local = B.con_local;
+        sys->con_rows = B.con_rows;
+        sys->con_model_row = B.con_model_row;
+        sys->con_block = B.con_block;
+        sys->model_gdof_to_block_vec = B.gdof_maps;
+
+        hip_check(hipStreamCreateWithFlags(&sys->stream, hipStreamNonBlocking), "hipStreamCreate");
+        hip_check(hipEventCreate(&sys->ev0), "event");
+        hip_check(hipEventCreate(&sys->ev1), "event");
+        hip_check(hipEventCreate(&sys->ek0), "event");
+        hip_check(hipEventCreate(&sys->ek1), "event");
+        // UpdateGlobalPosition at setup (copy_nodes_to_state.hpp:52-58) via the epilogue's x
+        {
+            std::vector<double> x(n * 7);
+            for (size_t i = 0; i < n; ++i) {
+                const double* q = &B.q[i * 7];
+                const double* x0 = &B.x0[i * 7];
+                x[i * 7 + 0] = x0[0] + q[0];
+                x[i * 7 + 1] = x0[1] + q[1];
+                x[i * 7 + 2] = x0[2] + q[2];
+                const auto r = qcompose({q[3], q[4], q[5], q[6]}, {x0[3], x0[4], x0[5], x0[6]});
+                for (int c = 0; c < 4; ++c) x[i * 7 + 3 + c] = r[c];
+            }
+            hip_check(hipMemcpy(S.x, x.data(), x.size() * sizeof(double), hipMemcpyHostToDevice), "upload");
+        }
+        result = sys.release();
+        return KYN_OK;
+    });
+    (void)rc;
+    return result;
+}
+
+void kyn_system_destroy(kyn_system s) { delete s; }
+size_t kyn_system_num_models(kyn_system s) { return s ? s->num_models : 0; }
+size_t kyn_system_num_nodes(kyn_system s) { return s ? s->num_nodes : 0; }
+size_t kyn_system_num_constraints(kyn_system s) { return s ? s->num_cons : 0; }
+size_t kyn_system_num_dofs(kyn_system s, size_t model) {
+    return (s && model < s->num_models) ? static_cast<size_t>(s->model_ndofs[model]) : 0;
+}
+size_t kyn_system_node_offset(kyn_system s, size_t model) {
+    return (s && model < s->num_models) ? static_cast<size_t>(s->model_node_begin[model]) : 0;
+}
+size_t kyn_system_constraint_offset(kyn_system s, size_t model) {
+    return (s && model < s->num_models) ? static_cast<size_t>(s->model_con_begin[model]) : 0;
+}
+
+int kyn_step(kyn_system s) {
+    return guarded([&] {
+        if (!s) throw KynError(KYN_ERR_INVALID, "null system");
+        return run_step(s);
+    });
+}
+
+int kyn_get_step_status(kyn_system s, int* status, int* iterations) {
+    return guarded([&] {
+        if (!s) throw KynError(KYN_ERR_INVALID, "null system");
+        if (status)
+            hip_check(hipMemcpy(status, s->S.model_status, s->num_models * sizeof(int), hipMemcpyDeviceToHost), "D2H");
+        if (iterations)
+            hip_check(hipMemcpy(iterations, s->S.model_iters, s->num_models * sizeof(int), hipMemcpyDeviceToHost), "D2H");
+        return KYN_OK;
+    });
+}
+
+size_t kyn_get_convergence_history(kyn_system s, size_t model, double* err, size_t cap) {
+    size_t count = 0;
+    guarded([&] {
+        if (!s || model >= s->num_models) throw KynError(KYN_ERR_INVALID, "bad model");
+        int it = 0;
+        hip_check(hipMemcpy(&it, s->S.model_iters + model, sizeof(int), hipMemcpyDeviceToHost), "D2H");
+        count = static_cast<size_t>(it);
+        if (err && cap) {
+            const size_t k = std::min(cap, count);
+            hip_check(hipMemcpy(err, s->S.err_hist + model * static_cast<size_t>(s->S.p.max_iter), k * sizeof(double),
+                                hipMemcpyDeviceToHost),
+                      "D2H");
+        }
+        return KYN_OK;
+    });
+    return count;
+}
+
+double kyn_last_step_ms(kyn_system s) { return s ? static_cast<double>(s->last_step_ms) : 0.; }
+double kyn_last_kernel_ms(kyn_system s) { return s ? static_cast<double>(s->last_kernel_ms) : 0.; }
+
+int kyn_set_constraint_displacement(kyn_system s, size_t id, const double disp[7]) {
+    return kyn_set_constraint_displacements(s, 1, &id, disp);
+}
+int kyn_set_constraint_displacements(kyn_system s, size_t count, const size_t* ids, const double* disp) {
+    return guarded([&] {
+        if (!s || !ids || !disp) throw KynError(KYN_ERR_INVALID, "null argument");
+        for (size_t k = 0; k < count; ++k) {
+            if (ids[k] >= s->num_cons) throw KynError(KYN_ERR_INVALID, "constraint id out of range");
+            hip_check(hipMemcpyAsync(s->S.con_input + ids[k] * 7, disp + 7 * k, 7 * sizeof(double),
+                                     hipMemcpyHostToDevice, s->stream),
+                      "H2D");
+        }
+        hip_check(hipStreamSynchronize(s->stream), "sync");
+        return KYN_OK;
+    });
+}
+int kyn_set_constraint_control(kyn_system s, size_t id, double value) {
+    return guarded([&] {
+        if (!s || id >= s->num_cons) throw KynError(KYN_ERR_INVALID, "constraint id out of range");
+        hip_check(hipMemcpy(s->S.con_control + id, &value, sizeof(double), hipMemcpyHostToDevice), "H2D");
+        return KYN_OK;
+    });
+}
+int kyn_set_nodal_loads(kyn_system s, const double* f) {
+    return guarded([&] {
+        if (!s || !f) throw KynError(KYN_ERR_INVALID, "null argument");
+        hip_check(hipMemcpy(s->S.f, f, s->num_nodes * 6 * sizeof(double), hipMemcpyHostToDevice), "H2D");
+        return KYN_OK;
+    });
+}
+int kyn_set_element_node_loads(kyn_system s, size_t element, const double* fx) {
+    return guarded([&] {
+        if (!s || !fx || element >= s->num_elems) throw KynError(KYN_ERR_INVALID, "bad element");
+        hip_check(hipMemcpy(s->S.node_FX + static_cast<size_t>(s->elem_node_off[element]) * 6, fx,
+                            static_cast<size_t>(s->elem_N[element]) * 6 * sizeof(double), hipMemcpyHostToDevice),
+                  "H2D");
+        return KYN_OK;
+    });
+}
+int kyn_set_input_schedule(kyn_system s, size_t num_steps, const double* cdisp, size_t num_loaded,
+                           const size_t* load_nodes, const double* loads) {
+    return guarded([&] {
+        if (!s) throw KynError(KYN_ERR_INVALID, "null system");
+        auto& S = s->S;
+        auto& Ar = s->arena;
+        S.sched_steps = static_cast<int>(num_steps);
+        S.sched_cdisp = nullptr;
+        S.sched_loads = nullptr;
+        S.sched_num_loaded = 0;
+        if (cdisp && s->num_cons > 0)
+            S.sched_cdisp = Ar.up(std::vector<double>(cdisp, cdisp + num_steps * s->num_cons * 7));
+        std::vector<int> begin(s->num_models + 1, 0);
+        if (loads && num_loaded > 0) {
+            std::vector<size_t> order(num_loaded);
+            std::iota(order.begin(), order.end(), size_t{0});
+            std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return load_nodes[a] < load_nodes[b]; });
+            std::vector<int> nodes(num_loaded);
+            std::vector<double> vals(num_steps * num_loaded * 6);
+            for (size_t k = 0; k < num_loaded; ++k) {
+                if (load_nodes[order[k]] >= s->num_nodes) throw KynError(KYN_ERR_INVALID, "load node out of range");
+                nodes[k] = static_cast<int>(load_nodes[order[k]]);
+                for (size_t st = 0; st < num_steps; ++st)
+                    for (int c = 0; c < 6; ++c)
+                        vals[(st * num_loaded + k) * 6 + c] = loads[(st * num_loaded + order[k]) * 6 + c];
+            }
+            size_t k = 0;
+            for (size_t m = 0; m < s->num_models; ++m) {
+                begin[m] = static_cast<int>(k);
+                while (k < num_loaded && nodes[k] < s->model_node_end[m]) ++k;
+            }
+            begin[s->num_models] = static_cast<int>(num_loaded);
+            S.sched_load_nodes = Ar.up(nodes);
+            S.sched_loads = Ar.up(vals);
+            S.sched_num_loaded = static_cast<int>(num_loaded);
+        }
+        hip_check(hipMemcpy(const_cast<int*>(S.model_load_begin), begin.data(), begin.size() * sizeof(int),
+                            hipMemcpyHostToDevice),
+                  "H2D");
+        const int zero = 0;
+        hip_check(hipMemcpy(S.sched_slot, &zero, sizeof(int), hipMemcpyHostToDevice), "H2D");
+        s->graph_ready = false;
+        return KYN_OK;
+    });
+}
+
+int kyn_get_state(kyn_system s, int field, double* out) {
+    return guarded([&] {
+        if (!s || !out) throw KynError(KYN_ERR_INVALID, "null argument");
+        if (field == KYN_TANGENT) {
+            std::vector<double> tr(s->num_nodes * 9);
+            hip_check(hipMemcpy(tr.data(), s->S.tan_rot, tr.size() * sizeof(double), hipMemcpyDeviceToHost), "D2H");
+            for (size_t i = 0; i < s->num_nodes; ++i) {
+                double* T = out + i * 36;
+                std::fill(T, T + 36, 0.);
+                for (int k = 0; k < 3; ++k) T[k * 6 + k] = 1.;
+                for (int r = 0; r < 3; ++r)
+                    for (int c = 0; c < 3; ++c) T[(3 + r) * 6 + 3 + c] = tr[i * 9 + r * 3 + c];
+            }
+            return KYN_OK;
+        }
+        int w = 0;
+        double* p = field_ptr(s, field, w);
+        if (!p) throw KynError(KYN_ERR_INVALID, "unknown field");
+        hip_check(hipMemcpy(out, p, s->num_nodes * w * sizeof(double), hipMemcpyDeviceToHost), "D2H");
+        return KYN_OK;
+    });
+}
+int kyn_set_state(kyn_system s, int field, const double* in) {
+    return guarded([&] {
+        if (!s || !in) throw KynError(KYN_ERR_INVALID, "null argument");
+        int w = 0;
+        double* p = field_ptr(s, field, w);
+        if (!p) throw KynError(KYN_ERR_INVALID, "unknown field");
+        hip_check(hipMemcpy(p, in, s->num_nodes * w * sizeof(double), hipMemcpyHostToDevice), "H2D");
+        return KYN_OK;
+    });
+}
+int kyn_get_constraint_values(kyn_system s, size_t id, int which, double* out) {
+    return guarded([&] {
+        if (!s || !out || id >= s->num_cons) throw KynError(KYN_ERR_INVALID, "bad constraint");
+        const double* src = which == 0 ? s->S.con_lambda + id * 6
+                            : which == 1 ? s->S.con_res + id * 6
+                            : which == 2 ? s->S.con_output + id * 3
+                                         : nullptr;
+        if (!src) throw KynError(KYN_ERR_INVALID, "bad selector");
+        hip_check(hipMemcpy(out, src, (which == 2 ? 3 : 6) * sizeof(double), hipMemcpyDeviceToHost), "D2H");
+        return KYN_OK;
+    });
+}
+static int get_vec(kyn_system s, size_t model, double* out, const double* dev) {
+    return guarded([&] {
+        if (!s || !out || model >= s->num_models) throw KynError(KYN_ERR_INVALID, "bad model");
+        std::vector<double> all(std::accumulate(s->block_n.begin(), s->block_n.end(), size_t{0}));
+        hip_check(hipMemcpy(all.data(), dev, all.size() * sizeof(double), hipMemcpyDeviceToHost), "D2H");
+        const auto& g = s->model_gdof_to_block_vec[model];
+        for (size_t i = 0; i < g.size(); ++i) out[i] = g[i] >= 0 ? all[static_cast<size_t>(g[i])] : 0.;
+        return KYN_OK;
+    });
+}
+int kyn_get_rhs(kyn_system s, size_t model, double* b) { return get_vec(s, model, b, s ? s->S.rhs_raw : nullptr); }
+int kyn_get_solution(kyn_system s, size_t model, double* x) { return get_vec(s, model, x, s ? s->S.xsol : nullptr); }
+
+// CopyStateData (copy_state_data.hpp:20-34): x, q_delta, q_prev, q, v, vd, a, f
+int kyn_save_state(kyn_system s) {
+    return guarded([&] {
+        if (!s) throw KynError(KYN_ERR_INVALID, "null system");
+        const size_t n = s->num_nodes;
+        s->saved.assign(n * (7 * 3 + 6 * 5) + 1, 0.);
+        double* o = s->saved.data();
+        const int fields[] = {KYN_X, KYN_Q_DELTA, KYN_Q_PREV, KYN_Q, KYN_V, KYN_VD, KYN_A, KYN_F};
+        for (int f : fields) {
+            int w = 0;
+            double* p = field_ptr(s, f, w);
+            hip_check(hipMemcpy(o, p, n * w * sizeof(double), hipMemcpyDeviceToHost), "D2H");
+            o += n * w;
+        }
+        int slot = 0;
+        hip_check(hipMemcpy(&slot, s->S.sched_slot, sizeof(int), hipMemcpyDeviceToHost), "D2H");
+        *o = static_cast<double>(slot);
+        s->has_saved = true;
+        return KYN_OK;
+    });
+}
+int kyn_restore_state(kyn_system s) {
+    return guarded([&] {
+        if (!s || !s->has_saved) throw KynError(KYN_ERR_INVALID, "no saved state");
+        const size_t n = s->num_nodes;
+        const double* o = s->saved.data();
+        const int fields[] = {KYN_X, KYN_Q_DELTA, KYN_Q_PREV, KYN_Q, KYN_V, KYN_VD, KYN_A, KYN_F};
+        for (int f : fields) {
+            int w = 0;
+            double* p = field_ptr(s, f, w);
+            hip_check(hipMemcpy(p, o, n * w * sizeof(double), hipMemcpyHostToDevice), "H2D");
+            o += n * w;
+        }
+        const int slot = static_cast<int>(*o);
+        hip_check(hipMemcpy(s->S.sched_slot, &slot, sizeof(int), hipMemcpyHostToDevice), "H2D");
+        return KYN_OK;
+    });
+}
+// WriteStateToFile / ReadStateFromFile byte image (write_state_to_file.hpp:19-58)
+size_t kyn_restart_size(kyn_system s, size_t model) {
+    if (!s || model >= s->num_models) return 0;
+    const size_t n = static_cast<size_t>(s->model_node_end[model] - s->model_node_begin[model]);
+    return sizeof(size_t) + n * (7 * 4 + 6 * 5) * sizeof(double);
+}
+int kyn_write_restart(kyn_system s, size_t model, unsigned char* out) {
+    return guarded([&] {
+        if (!s || !out || model >= s->num_models) throw KynError(KYN_ERR_INVALID, "bad model");
+        size_t n = static_cast<size_t>(s->model_node_end[model] - s->model_node_begin[model]);
+        const size_t n0 = static_cast<size_t>(s->model_node_begin[model]);
+        std::memcpy(out, &n, sizeof(size_t));
+        size_t off = sizeof(size_t);
+        const int fields[] = {KYN_X0, KYN_X, KYN_Q_DELTA, KYN_Q_PREV, KYN_Q, KYN_V, KYN_VD, KYN_A, KYN_F};
+        for (int f : fields) {
+            int w = 0;
+            double* p = field_ptr(s, f, w);
+            hip_check(hipMemcpy(out + off, p + n0 * w, n * w * sizeof(double), hipMemcpyDeviceToHost), "D2H");
+            off += n * w * sizeof(double);
+        }
+        return KYN_OK;
+    });
+}
+int kyn_read_restart(kyn_system s, size_t model, const unsigned char* in, size_t size) {
+    return guarded([&] {
+        if (!s || !in || model >= s->num_models) throw KynError(KYN_ERR_INVALID, "bad model");
+        size_t n = 0;
+        std::memcpy(&n, in, sizeof(size_t));
+        const size_t nm = static_cast<size_t>(s->model_node_end[model] - s->model_node_begin[model]);
+        if (n != nm) throw KynError(KYN_ERR_INVALID, "Number of system nodes in file is not the same as in model");
+        if (size < kyn_restart_size(s, model)) throw KynError(KYN_ERR_INVALID, "restart image too short");
+        const size_t n0 = static_cast<size_t>(s->model_node_begin[model]);
+        size_t off = sizeof(size_t);
+        const int fields[] = {KYN_X0, KYN_X, KYN_Q_DELTA, KYN_Q_PREV, KYN_Q, KYN_V, KYN_VD, KYN_A, KYN_F};
+        for (int f : fields) {
+            int w = 0;
+            double* p = field_ptr(s, f, w);
+            hip_check(hipMemcpy(p + n0 * w, in + off, n * w * sizeof(double), hipMemcpyHostToDevice), "H2D");
+            off += n * w * sizeof(double);
+        }
+        return KYN_OK;
+    });
+}
+
+int kyn_debug_assemble(kyn_system s) {
+    return guarded([&] {
+        if (!s) throw KynError(KYN_ERR_INVALID, "null system");
+        if (!s->S.elem_mat) s->S.elem_mat = s->arena.zeros<double>(static_cast<size_t>(s->mat_total));
+        if (!s->S.A) s->S.A = s->arena.zeros<double>(static_cast<size_t>(s->A_total));
+        return run_step(s, false);
+    });
+}
+int kyn_get_element_terms(kyn_system s, size_t element, double* residual, double* system_matrix) {
+    return guarded([&] {
+        if (!s || element >= s->num_elems) throw KynError(KYN_ERR_INVALID, "bad element");
+        if (!s->S.elem_mat) throw KynError(KYN_ERR_INVALID, "no element terms: run kyn_debug_assemble or the plain form first");
+        const size_t N = static_cast<size_t>(s->elem_N[element]);
+        const size_t off = static_cast<size_t>(s->elem_node_off[element]);
+        if (residual)
+            hip_check(hipMemcpy(residual, s->S.elem_res + off * 6, N * 6 * sizeof(double), hipMemcpyDeviceToHost), "D2H");
+        if (system_matrix) {
+            int64_t moff = 0;
+            hip_check(hipMemcpy(&moff, s->S.elem_mat_off + element, sizeof(int64_t), hipMemcpyDeviceToHost), "D2H");
+            hip_check(hipMemcpy(system_matrix, s->S.elem_mat + moff, N * N * 36 * sizeof(double), hipMemcpyDeviceToHost),
+                      "D2H");
+        }
+        return KYN_OK;
+    });
+}
+
+}  // extern "C"

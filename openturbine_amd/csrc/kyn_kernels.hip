@@ -1,0 +1,1180 @@
+// HIP kernels of the beam Newton path for gfx950 (MI355X).
+//
+// Two execution forms share every device routine:
+//   plain  — the reference's launch structure (src/step/step.hpp:54-83): predict, then per
+//            Newton iteration tangent -> element -> constraints -> assemble -> LU/solve ->
+//            update, with the convergence error read by the host after every iteration.
+//   fused  — one workgroup per model runs the whole time step (predict, Newton loop,
+//            epilogue) in one launch; the KKT block lives in LDS, the element tangent goes
+//            from registers straight into it, nothing but state round-trips through HBM.
+//
+// Element contraction (the reference's integrate_{stiffness,inertia}_matrix.hpp and
+// calculate_system_matrix.hpp): S_ij = sum_q phi_i R0_q[j] + phi'_i R1_q[j] where
+//   R0_q[j] = (phi_j wJ(Kuu+Quu) + phi'_j w Puu) T_j + phi_j wJ(beta' Muu + gamma' Guu)
+//   R1_q[j] = (phi_j w Ouu + phi'_j (w/J) Cuu) T_j
+// i.e. an (N x 2Q) x (2Q x 36N) product.  R is built per QP chunk in LDS; each of 252
+// threads owns one 6x6 component c and the rows i = ig, ig+7, ig+14 for all j (a 3 x 21
+// register tile).
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "kyn_device.hpp"
+#include "kyn_system.hpp"
+
+namespace kyn {
+
+constexpr int kQpChunk = 2;
+constexpr int kContractThreads = 252;  // 7 row groups x 36 components
+
+// ---------------------------------------------------------------------------------------
+// LDS layout (identical on host and device)
+// ---------------------------------------------------------------------------------------
+struct Lay {
+    // persistent
+    int nq, nv, nvd, tr, loc, rhs, xs, xs2, piv, red, cres, misc;
+    // overlay: element view
+    int phi, dphi, w, J, qpF, qpM, R;
+    // overlay: matrix view
+    int A;
+    int lda;
+    int total;
+};
+
+__host__ __device__ inline int align16(int x) { return (x + 15) & ~15; }
+
+__host__ __device__ inline Lay make_layout(int Nm, int Qm, int nm, int tt_size, bool with_matrix) {
+    Lay L{};
+    int o = 0;
+    auto take = [&](int bytes) {
+        const int r = o;
+        o = align16(o + bytes);
+        return r;
+    };
+    L.nq = take(Nm * 7 * 8);
+    L.nv = take(Nm * 6 * 8);
+    L.nvd = take(Nm * 6 * 8);
+    L.tr = take(Nm * 9 * 8);
+    L.loc = take(Nm * 4);
+    L.rhs = take(nm * 8);
+    L.xs = take(nm * 8);
+    L.xs2 = take(nm * 8);
+    L.piv = take(nm * 4);
+    L.red = take(16 * 8);
+    L.cres = take(8 * 24 * 8);  // up to 8 constraints per block: res, lres_b, lres_t, sys_res
+    L.misc = take(16 * 4);
+    const int overlay = o;
+    L.phi = take(Nm * Qm * 8);
+    L.dphi = take(Nm * Qm * 8);
+    L.w = take(Qm * 8);
+    L.J = take(Qm * 8);
+    L.qpF = take(Qm * 24 * 8);
+    L.qpM = take(Qm * 6 * 36 * tt_size);
+    L.R = take(kQpChunk * 2 * Nm * 36 * tt_size);
+    const int elem_end = o;
+    L.lda = nm + 1;
+    L.A = overlay;
+    const int mat_end = with_matrix ? align16(overlay + nm * L.lda * tt_size) : overlay;
+    L.total = elem_end > mat_end ? elem_end : mat_end;
+    return L;
+}
+
+// ---------------------------------------------------------------------------------------
+// Small block-wide helpers
+// ---------------------------------------------------------------------------------------
+__device__ __forceinline__ double block_sum(double v, double* red) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) red[wave] = v;
+    __syncthreads();
+    double t = 0.;
+    const int nw = (blockDim.x + 63) >> 6;
+    for (int k = 0; k < nw; ++k) t += red[k];
+    __syncthreads();
+    return t;
+}
+
+template <typename T>
+__device__ __forceinline__ T tabs(T v) { return v < T(0) ? -v : v; }
+
+// ---------------------------------------------------------------------------------------
+// Element phase: gather, QP values, residual, contraction.
+//   kFused: residual goes into the block rhs (-f + R) and the tangent into LDS matrix A
+//           (times the conditioner, at block-local dofs); otherwise into elem_res/elem_mat.
+// ---------------------------------------------------------------------------------------
+template <typename TT, bool kFused>
+__device__ void element_phase(const DevSys& S, int e, char* sm, const Lay& L, int n) {
+    const int tid = threadIdx.x;
+    const int N = S.elem_N[e], Q = S.elem_Q[e];
+    const int noff = S.elem_node_off[e];
+    const int qoff = S.elem_qp_off[e];
+    const int soff = S.elem_shape_off[e];
+    const double* grav = S.model_gravity + 3 * S.block_model[S.node_block[S.elem_nodes[noff]]];
+    double* nq = reinterpret_cast<double*>(sm + L.nq);
+    double* nv = reinterpret_cast<double*>(sm + L.nv);
+    double* nvd = reinterpret_cast<double*>(sm + L.nvd);
+    double* tr = reinterpret_cast<double*>(sm + L.tr);
+    int* loc = reinterpret_cast<int*>(sm + L.loc);
+    double* phi = reinterpret_cast<double*>(sm + L.phi);
+    double* dphi = reinterpret_cast<double*>(sm + L.dphi);
+    double* wq = reinterpret_cast<double*>(sm + L.w);
+    double* Jq = reinterpret_cast<double*>(sm + L.J);
+    double* qpF = reinterpret_cast<double*>(sm + L.qpF);
+    TT* qpM = reinterpret_cast<TT*>(sm + L.qpM);
+    TT* Rb = reinterpret_cast<TT*>(sm + L.R);
+    double* rhs = reinterpret_cast<double*>(sm + L.rhs);
+    const double h = S.p.h;
+
+    // ---- gather node state (update_node_state.hpp:26-35) and tangent blocks
+    for (int k = tid; k < N; k += blockDim.x) {
+        const int node = S.elem_nodes[noff + k];
+#pragma unroll
+        for (int c = 0; c < 7; ++c) nq[k * 7 + c] = S.q[node * 7 + c];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+            nv[k * 6 + c] = S.v[node * 6 + c];
+            nvd[k * 6 + c] = S.vd[node * 6 + c];
+        }
+        if (kFused) {
+            // UpdateTangentOperator (update_tangent_operator.hpp:13-26) folded in
+            double qd[6];
+#pragma unroll
+            for (int c = 0; c < 6; ++c) qd[c] = S.q_delta[node * 6 + c];
+            tangent_rot_block(h, qd, tr + k * 9);
+        } else {
+#pragma unroll
+            for (int c = 0; c < 9; ++c) tr[k * 9 + c] = S.tan_rot[node * 9 + c];
+        }
+        loc[k] = kFused ? S.node_local[node] : 0;
+    }
+    for (int k = tid; k < N * Q; k += blockDim.x) {
+        phi[k] = S.shape_interp[soff + k];
+        dphi[k] = S.shape_deriv[soff + k];
+    }
+    for (int q = tid; q < Q; q += blockDim.x) {
+        wq[q] = S.qp_weight[qoff + q];
+        Jq[q] = S.qp_jac[qoff + q];
+    }
+    __syncthreads();
+
+    // ---- QP values: two lanes per QP (inertial | stiffness)
+    for (int t = tid; t < 2 * Q; t += blockDim.x) {
+        const int q = t >> 1;
+        const int half = t & 1;
+        const double w = wq[q], J = Jq[q];
+        const double* r0 = S.qp_r0 + (size_t)(qoff + q) * 4;
+        TT* M6 = qpM + (size_t)q * 6 * 36;  // A1, A2, Mi, P, O, C
+        double* F = qpF + q * 24;          // Fc, Fd, Fi, Fg
+        if (half == 0) {
+            // interpolate_to_quadrature_point_for_inertia.hpp:19-40
+            double r[4] = {0., 0., 0., 0.}, ud[3] = {0., 0., 0.}, om[3] = {0., 0., 0.},
+                   omd[3] = {0., 0., 0.};
+            for (int k = 0; k < N; ++k) {
+                const double p = phi[k * Q + q];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    ud[c] += nvd[k * 6 + c] * p;
+                    om[c] += nv[k * 6 + 3 + c] * p;
+                    omd[c] += nvd[k * 6 + 3 + c] * p;
+                }
+#pragma unroll
+                for (int c = 0; c < 4; ++c) r[c] += nq[k * 7 + 3 + c] * p;
+            }
+            double Muu[36], Guu[36], Kuu[36], Fi[6], Fg[6];
+            double r0l[4] = {r0[0], r0[1], r0[2], r0[3]};
+            qp_inertial(r, ud, om, omd, r0l, S.qp_Mstar + (size_t)(qoff + q) * 36, grav, Muu, Guu,
+                        Kuu, Fi, Fg);
+            const double wJ = w * J;
+            const double bp = S.p.beta_prime, gp = S.p.gamma_prime;
+#pragma unroll
+            for (int c = 0; c < 36; ++c) {
+                M6[c] = (TT)(wJ * Kuu[c]);
+                M6[72 + c] = (TT)(wJ * (bp * Muu[c] + gp * Guu[c]));
+            }
+#pragma unroll
+            for (int c = 0; c < 6; ++c) {
+                F[12 + c] = Fi[c];
+                F[18 + c] = Fg[c];
+            }
+        } else {
+            // interpolate_to_quadrature_point_for_stiffness.hpp:19-41
+            double r[4] = {0., 0., 0., 0.}, rp[4] = {0., 0., 0., 0.}, up[3] = {0., 0., 0.};
+            for (int k = 0; k < N; ++k) {
+                const double p = phi[k * Q + q];
+                const double dp = dphi[k * Q + q] / J;
+#pragma unroll
+                for (int c = 0; c < 3; ++c) up[c] += nq[k * 7 + c] * dp;
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    r[c] += nq[k * 7 + 3 + c] * p;
+                    rp[c] += nq[k * 7 + 3 + c] * dp;
+                }
+            }
+            double Cuu[36], Ouu[36], Puu[36], Quu[36], Fc[6], Fd[6];
+            const double r0l[4] = {r0[0], r0[1], r0[2], r0[3]};
+            const double* xp = S.qp_x0p + (size_t)(qoff + q) * 3;
+            const double x0p[3] = {xp[0], xp[1], xp[2]};
+            qp_stiffness(r, rp, up, r0l, x0p, S.qp_Cstar + (size_t)(qoff + q) * 36, Cuu, Ouu, Puu, Quu,
+                         Fc, Fd);
+            const double wJ = w * J, wdJ = w / J;
+#pragma unroll
+            for (int c = 0; c < 36; ++c) {
+                M6[36 + c] = (TT)(wJ * Quu[c]);
+                M6[108 + c] = (TT)(w * Puu[c]);
+                M6[144 + c] = (TT)(w * Ouu[c]);
+                M6[180 + c] = (TT)(wdJ * Cuu[c]);
+            }
+#pragma unroll
+            for (int c = 0; c < 6; ++c) {
+                F[c] = Fc[c];
+                F[6 + c] = Fd[c];
+            }
+        }
+    }
+    __syncthreads();
+
+    // ---- residual (integrate_residual_vector.hpp:32-50)
+    for (int t = tid; t < 6 * N; t += blockDim.x) {
+        const int k = t / 6, c = t - 6 * (t / 6);
+        double acc = 0.;
+        for (int q = 0; q < Q; ++q) {
+            const double w = wq[q];
+            const double cc = w * dphi[k * Q + q];
+            const double cd = w * Jq[q] * phi[k * Q + q];
+            const double* F = qpF + q * 24;
+            acc += cc * F[c] + cd * ((F[6 + c] + F[12 + c]) - F[18 + c]);
+        }
+        const double R = acc - S.node_FX[(size_t)(noff + k) * 6 + c];
+        if (kFused) {
+            const int node = S.elem_nodes[noff + k];
+            rhs[loc[k] + c] = -S.f[node * 6 + c] + R;
+        } else {
+            S.elem_res[(size_t)(noff + k) * 6 + c] = R;
+        }
+    }
+
+    // ---- contraction over QP chunks
+    const int ig = tid / 36;
+    const int cc = tid - 36 * (tid / 36);
+    const bool active = tid < kContractThreads;
+    const int r_ = cc / 6, s_ = cc - 6 * (cc / 6);
+    TT acc[3][kMaxElemNodes];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int j = 0; j < kMaxElemNodes; ++j) acc[a][j] = TT(0);
+
+    for (int q0 = 0; q0 < Q; q0 += kQpChunk) {
+        const int qn = (Q - q0) < kQpChunk ? (Q - q0) : kQpChunk;
+        // build R chunk: items (qq, k, j, c)
+        const int items = qn * 2 * N * 36;
+        for (int t = tid; t < items; t += blockDim.x) {
+            const int c = t % 36;
+            const int j = (t / 36) % N;
+            const int k = (t / (36 * N)) & 1;
+            const int qq = t / (72 * N);
+            const int q = q0 + qq;
+            const int r = c / 6, s = c - 6 * (c / 6);
+            const TT* M6 = qpM + (size_t)q * 6 * 36;
+            const TT pj = (TT)phi[j * Q + q], dj = (TT)dphi[j * Q + q];
+            TT val;
+            if (k == 0) {
+                // (phi_j (A1+A2) + phi'_j P) T_j + phi_j Mi
+                if (s < 3) {
+                    val = pj * (M6[r * 6 + s] + M6[36 + r * 6 + s]) + dj * M6[108 + r * 6 + s];
+                } else {
+                    val = TT(0);
+#pragma unroll
+                    for (int u = 0; u < 3; ++u) {
+                        const TT x = pj * (M6[r * 6 + 3 + u] + M6[36 + r * 6 + 3 + u]) + dj * M6[108 + r * 6 + 3 + u];
+                        val += x * (TT)tr[j * 9 + u * 3 + (s - 3)];
+                    }
+                }
+                val += pj * M6[72 + r * 6 + s];
+            } else {
+                // (phi_j O + phi'_j C) T_j
+                if (s < 3) {
+                    val = pj * M6[144 + r * 6 + s] + dj * M6[180 + r * 6 + s];
+                } else {
+                    val = TT(0);
+#pragma unroll
+                    for (int u = 0; u < 3; ++u) {
+                        const TT x = pj * M6[144 + r * 6 + 3 + u] + dj * M6[180 + r * 6 + 3 + u];
+                        val += x * (TT)tr[j * 9 + u * 3 + (s - 3)];
+                    }
+                }
+            }
+            Rb[((qq * 2 + k) * N + j) * 36 + c] = val;
+        }
+        __syncthreads();
+        if (active) {
+            for (int qq = 0; qq < qn; ++qq) {
+                const int q = q0 + qq;
+                TT pi[3], di[3];
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    const int i = ig + 7 * a;
+                    pi[a] = i < N ? (TT)phi[i * Q + q] : TT(0);
+                    di[a] = i < N ? (TT)dphi[i * Q + q] : TT(0);
+                }
+                const TT* R0 = Rb + (size_t)(qq * 2 + 0) * N * 36 + cc;
+                const TT* R1 = Rb + (size_t)(qq * 2 + 1) * N * 36 + cc;
+#pragma unroll
+                for (int j = 0; j < kMaxElemNodes; ++j) {
+                    if (j < N) {
+                        const TT x0 = R0[j * 36], x1 = R1[j * 36];
+#pragma unroll
+                        for (int a = 0; a < 3; ++a) acc[a][j] += pi[a] * x0 + di[a] * x1;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+
+    // ---- write out
+    if (kFused) {
+        // the overlay is free after the barrier above: zero the matrix view, then write
+        TT* A = reinterpret_cast<TT*>(sm + L.A);
+        for (int k = tid; k < n * L.lda; k += blockDim.x) A[k] = TT(0);
+        __syncthreads();
+        const TT cond = (TT)S.p.conditioner;
+        if (active) {
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                const int i = ig + 7 * a;
+                if (i < N) {
+                    const int row = loc[i] + r_;
+#pragma unroll
+                    for (int j = 0; j < kMaxElemNodes; ++j)
+                        if (j < N) A[row * L.lda + loc[j] + s_] = acc[a][j] * cond;
+                }
+            }
+        }
+        __syncthreads();
+    } else {
+        if (active) {
+            double* out = S.elem_mat + S.elem_mat_off[e];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                const int i = ig + 7 * a;
+                if (i < N) {
+#pragma unroll
+                    for (int j = 0; j < kMaxElemNodes; ++j)
+                        if (j < N) out[(size_t)(i * N + j) * 36 + cc] = (double)acc[a][j];
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// LU with partial pivoting on an LDS matrix (replaces dss::numeric_factorization/solve,
+// src/step/solve_system.hpp:27-31).  Row-major, leading dimension lda.  Pivot: largest
+// |a_ik|, lowest index on ties.
+// ---------------------------------------------------------------------------------------
+template <typename TT>
+__device__ void lu_factor(TT* A, int n, int lda, int* piv, int* misc) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int nt = blockDim.x;
+    for (int k = 0; k < n; ++k) {
+        if (wave == 0) {
+            TT best = TT(-1);
+            int bi = n;
+            for (int i = k + lane; i < n; i += 64) {
+                const TT v = tabs(A[i * lda + k]);
+                if (v > best) {
+                    best = v;
+                    bi = i;
+                }
+            }
+            for (int off = 32; off > 0; off >>= 1) {
+                const TT ob = __shfl_xor(best, off, 64);
+                const int oi = __shfl_xor(bi, off, 64);
+                if (ob > best || (ob == best && oi < bi)) {
+                    best = ob;
+                    bi = oi;
+                }
+            }
+            if (lane == 0) {
+                if (bi >= n) bi = k;  // column of NaNs: keep the row (no out-of-range swap)
+                piv[k] = bi;
+                misc[0] = bi;
+            }
+        }
+        __syncthreads();
+        const int p = misc[0];
+        if (p != k)
+            for (int j = tid; j < n; j += nt) {
+                const TT t = A[k * lda + j];
+                A[k * lda + j] = A[p * lda + j];
+                A[p * lda + j] = t;
+            }
+        __syncthreads();
+        const TT d = A[k * lda + k];
+        const int m = n - k - 1;
+        if (d != TT(0) && m > 0) {
+            const TT inv = TT(1) / d;
+            for (int i = k + 1 + tid; i < n; i += nt) A[i * lda + k] = A[i * lda + k] * inv;
+            __syncthreads();
+            for (int idx = tid; idx < m * m; idx += nt) {
+                const int ii = idx / m, jj = idx - ii * m;
+                const int i = k + 1 + ii, j = k + 1 + jj;
+                A[i * lda + j] -= A[i * lda + k] * A[k * lda + j];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+__device__ __forceinline__ double sel3(const double* r, int a) { return a == 0 ? r[0] : (a == 1 ? r[1] : r[2]); }
+
+// Triangular solves with the LDS factors, FP64 accumulation, executed by wave 0
+// (lane owns rows lane, lane+64, lane+128; n <= 192).  b: LDS, pivoted here; x -> xs.
+template <typename TT>
+__device__ void lu_solve(const TT* A, int n, int lda, const int* piv, double* b, double* xs) {
+    const int tid = threadIdx.x;
+    if (tid == 0)
+        for (int k = 0; k < n; ++k) {
+            const int p = piv[k];
+            if (p != k) {
+                const double t = b[k];
+                b[k] = b[p];
+                b[p] = t;
+            }
+        }
+    __syncthreads();
+    if (tid < 64) {
+        const int lane = tid;
+        double r[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) r[a] = (lane + 64 * a) < n ? b[lane + 64 * a] : 0.;
+        for (int k = 0; k < n; ++k) {
+            const double bk = __shfl(sel3(r, k >> 6), k & 63, 64);
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                const int i = lane + 64 * a;
+                if (i > k && i < n) r[a] -= (double)A[i * lda + k] * bk;
+            }
+        }
+        for (int k = n - 1; k >= 0; --k) {
+            const int ka = k >> 6;
+            if (lane == (k & 63)) {
+                const double d = (double)A[k * lda + k];
+                if (ka == 0) r[0] = r[0] / d;
+                else if (ka == 1) r[1] = r[1] / d;
+                else r[2] = r[2] / d;
+            }
+            const double xk = __shfl(sel3(r, ka), k & 63, 64);
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                const int i = lane + 64 * a;
+                if (i < k) r[a] -= (double)A[i * lda + k] * xk;
+            }
+        }
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+            if (lane + 64 * a < n) xs[lane + 64 * a] = r[a];
+    }
+    __syncthreads();
+}
+
+// ---------------------------------------------------------------------------------------
+// Node-level kernels shared by both forms
+// ---------------------------------------------------------------------------------------
+// PredictNextState (src/step/predict_next_state.hpp:14-47, calculate_next_state.hpp:26-37)
+__device__ __forceinline__ void predict_node(const DevSys& S, int node) {
+    const StepConsts& p = S.p;
+#pragma unroll
+    for (int c = 0; c < 7; ++c) S.q_prev[node * 7 + c] = S.q[node * 7 + c];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+        const int k = node * 6 + c;
+        const double v_p = S.v[k], vd_p = S.vd[k], a_p = S.a[k];
+        const double an = (p.alpha_f * vd_p - p.alpha_m * a_p) / (1. - p.alpha_m);
+        S.vd[k] = 0.;
+        S.a[k] = an;
+        S.v[k] = v_p + p.h * (1. - p.gamma) * a_p + p.gamma * p.h * an;
+        S.q_delta[k] = v_p + (0.5 - p.beta) * p.h * a_p + p.beta * p.h * an;
+    }
+    double qd[6], qp[7], qn[7];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) qd[c] = S.q_delta[node * 6 + c];
+#pragma unroll
+    for (int c = 0; c < 7; ++c) qp[c] = S.q_prev[node * 7 + c];
+    calc_displacement(p.h, qd, qp, qn);
+#pragma unroll
+    for (int c = 0; c < 7; ++c) S.q[node * 7 + c] = qn[c];
+}
+
+__device__ __forceinline__ void displacement_node(const DevSys& S, int node) {
+    double qd[6], qp[7], qn[7];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) qd[c] = S.q_delta[node * 6 + c];
+#pragma unroll
+    for (int c = 0; c < 7; ++c) qp[c] = S.q_prev[node * 7 + c];
+    calc_displacement(S.p.h, qd, qp, qn);
+#pragma unroll
+    for (int c = 0; c < 7; ++c) S.q[node * 7 + c] = qn[c];
+}
+
+// Step epilogue (src/step/step.hpp:85-121)
+__device__ __forceinline__ void epilogue_node(const DevSys& S, int node) {
+    const double f = (1. - S.p.alpha_f) / (1. - S.p.alpha_m);
+#pragma unroll
+    for (int c = 0; c < 6; ++c) S.a[node * 6 + c] += f * S.vd[node * 6 + c];
+    const double* q = S.q + node * 7;
+    const double* x0 = S.x0 + node * 7;
+    double* x = S.x + node * 7;
+    x[0] = x0[0] + q[0];
+    x[1] = x0[1] + q[1];
+    x[2] = x0[2] + q[2];
+    const double a[4] = {q[3], q[4], q[5], q[6]}, b[4] = {x0[3], x0[4], x0[5], x0[6]};
+    double r[4];
+    quat_compose(a, b, r);
+    x[3] = r[0];
+    x[4] = r[1];
+    x[5] = r[2];
+    x[6] = r[3];
+}
+
+// calculate_revolute_joint_output.hpp:30-86
+__device__ __forceinline__ void constraint_output(const DevSys& S, int c) {
+    if (S.con_type[c] != 4) return;
+    const int node = S.con_target[c];
+    const double* q = S.q + node * 7;
+    const double R[4] = {q[3], q[4], q[5], q[6]};
+    const double theta = 2. * acos(R[0]);
+    const double shh = sqrt(1. - R[0] * R[0]);
+    double rv[3] = {0., 0., 0.};
+    if (shh > 1e-12) {
+        rv[0] = theta * R[1] / shh;
+        rv[1] = theta * R[2] / shh;
+        rv[2] = theta * R[3] / shh;
+    }
+    double ax[3];
+    quat_rotate(R, S.con_axes + c * 9, ax);
+    const double* v = S.v + node * 6;
+    const double* vd = S.vd + node * 6;
+    S.con_output[c * 3 + 0] = ax[0] * rv[0] + ax[1] * rv[1] + ax[2] * rv[2];
+    S.con_output[c * 3 + 1] = ax[0] * v[3] + ax[1] * v[4] + ax[2] * v[5];
+    S.con_output[c * 3 + 2] = ax[0] * vd[3] + ax[1] * vd[4] + ax[2] * vd[5];
+}
+
+// Constraint gradient times stored tangent: G T with T = [I 0; 0 Trr]
+__device__ __forceinline__ void times_tangent(const double* g, const double* trr, double* out) {
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+#pragma unroll
+        for (int j = 0; j < 3; ++j) out[i * 6 + j] = g[i * 6 + j];
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            out[i * 6 + 3 + j] = g[i * 6 + 3] * trr[j] + g[i * 6 + 4] * trr[3 + j] + g[i * 6 + 5] * trr[6 + j];
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Schedule application (device-resident per-step inputs)
+// ---------------------------------------------------------------------------------------
+__device__ __forceinline__ void apply_schedule_model(const DevSys& S, int m) {
+    if (S.sched_steps <= 0) return;
+    const int slot = *S.sched_slot;
+    if (slot >= S.sched_steps) return;
+    if (S.sched_cdisp)
+        for (int c = S.model_con_begin[m] + (int)threadIdx.x; c < S.model_con_end[m]; c += blockDim.x)
+#pragma unroll
+            for (int k = 0; k < 7; ++k)
+                S.con_input[c * 7 + k] = S.sched_cdisp[((size_t)slot * S.num_cons + c) * 7 + k];
+    if (S.sched_loads)
+        for (int k = S.model_load_begin[m] + (int)threadIdx.x; k < S.model_load_begin[m + 1]; k += blockDim.x) {
+            const int node = S.sched_load_nodes[k];
+#pragma unroll
+            for (int c = 0; c < 6; ++c)
+                S.f[node * 6 + c] = S.sched_loads[((size_t)slot * S.sched_num_loaded + k) * 6 + c];
+        }
+}
+
+__global__ void k_advance_slot(DevSys S) {
+    if (threadIdx.x == 0 && blockIdx.x == 0 && S.sched_steps > 0) *S.sched_slot += 1;
+}
+
+// ---------------------------------------------------------------------------------------
+// Block phase helpers (fused form): constraints into the LDS KKT block
+// ---------------------------------------------------------------------------------------
+template <typename TT>
+__device__ void block_constraints(const DevSys& S, int b, char* sm, const Lay& L) {
+    TT* A = reinterpret_cast<TT*>(sm + L.A);
+    double* cres = reinterpret_cast<double*>(sm + L.cres);
+    const int c0 = S.block_con_begin[b], c1 = S.block_con_end[b];
+    const int nc = c1 - c0;
+    const int t = threadIdx.x;
+    if (t < nc) {
+        const int c = S.block_cons[c0 + t];
+        const int type = S.con_type[c];
+        const bool two = (type == 3 || type == 4);
+        const int tn = S.con_target[c], bn = S.con_base[c];
+        double input[7];
+#pragma unroll
+        for (int k = 0; k < 7; ++k) input[k] = S.con_input[c * 7 + k];
+        if (type == 4) input[0] = S.con_control[c];  // Constraints::UpdateViews
+        double tq[7], bq[7] = {0., 0., 0., 1., 0., 0., 0.};
+#pragma unroll
+        for (int k = 0; k < 7; ++k) tq[k] = S.q[tn * 7 + k];
+        if (two)
+#pragma unroll
+            for (int k = 0; k < 7; ++k) bq[k] = S.q[bn * 7 + k];
+        double res[6], bg[36], tg[36], sres[6] = {0., 0., 0., 0., 0., 0.};
+        constraint_eval(type, S.con_X0 + c * 3, S.con_axes + c * 9, input, bq, tq, res, bg, tg, sres);
+        const int rows = S.con_rows[c];
+        const int lr = S.con_local[c];
+        double trr[9], qd[6], gt[36];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) qd[k] = S.q_delta[tn * 6 + k];
+        tangent_rot_block(S.p.h, qd, trr);
+        times_tangent(tg, trr, gt);
+        const int lt = S.node_local[tn];
+        const double* lam = S.con_lambda + c * 6;
+        double* cr = cres + t * 24;
+#pragma unroll
+        for (int i = 0; i < 6; ++i)
+            if (i < rows)
+#pragma unroll
+                for (int j = 0; j < 6; ++j) {
+                    A[(lr + i) * L.lda + lt + j] = (TT)gt[i * 6 + j];
+                    A[(lt + j) * L.lda + lr + i] = (TT)tg[i * 6 + j];
+                }
+#pragma unroll
+        for (int i = 0; i < 6; ++i) {
+            double acc = 0.;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) acc += tg[k * 6 + i] * lam[k];
+            cr[12 + i] = acc;  // target lambda residual
+        }
+        if (two) {
+#pragma unroll
+            for (int k = 0; k < 6; ++k) qd[k] = S.q_delta[bn * 6 + k];
+            tangent_rot_block(S.p.h, qd, trr);
+            times_tangent(bg, trr, gt);
+            const int lb = S.node_local[bn];
+#pragma unroll
+            for (int i = 0; i < 6; ++i)
+                if (i < rows)
+#pragma unroll
+                    for (int j = 0; j < 6; ++j) {
+                        A[(lr + i) * L.lda + lb + j] = (TT)gt[i * 6 + j];
+                        A[(lb + j) * L.lda + lr + i] = (TT)bg[i * 6 + j];
+                    }
+#pragma unroll
+            for (int i = 0; i < 6; ++i) {
+                double acc = 0.;
+#pragma unroll
+                for (int k = 0; k < 6; ++k) acc += bg[k * 6 + i] * lam[k];
+                cr[6 + i] = acc;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            cr[k] = res[k];
+            cr[18 + k] = sres[k];
+            S.con_res[c * 6 + k] = res[k];
+            S.con_sys_res[c * 6 + k] = sres[k];
+        }
+    }
+    __syncthreads();
+    // rhs in the reference's order (assemble_constraints_residual.hpp:27-50)
+    if (t == 0 && nc > 0) {
+        double* rhs = reinterpret_cast<double*>(sm + L.rhs);
+        for (int k = 0; k < nc; ++k) {
+            const int c = S.block_cons[c0 + k];
+            const int lt = S.node_local[S.con_target[c]];
+            for (int j = 0; j < 6; ++j) rhs[lt + j] += cres[k * 24 + 18 + j];
+        }
+        for (int k = 0; k < nc; ++k) {
+            const int c = S.block_cons[c0 + k];
+            const int type = S.con_type[c];
+            if (type == 3 || type == 4) {
+                const int lb = S.node_local[S.con_base[c]];
+                for (int j = 0; j < 6; ++j) rhs[lb + j] += cres[k * 24 + 6 + j];
+            }
+            const int lt = S.node_local[S.con_target[c]];
+            for (int j = 0; j < 6; ++j) rhs[lt + j] += cres[k * 24 + 12 + j];
+        }
+        for (int k = 0; k < nc; ++k) {
+            const int c = S.block_cons[c0 + k];
+            const int lr = S.con_local[c];
+            for (int i = 0; i < S.con_rows[c]; ++i) rhs[lr + i] = 0. + cres[k * 24 + i];
+        }
+    }
+    __syncthreads();
+}
+
+// Convergence-error terms and state update of one block after its solve
+// (calculate_convergence_error.hpp, update_state_prediction.hpp, update_lambda_prediction.hpp)
+__device__ double block_error_and_update(const DevSys& S, int b, const double* xs, double* red) {
+    const StepConsts& p = S.p;
+    const int n = S.block_n[b], nsys = S.block_nsys[b];
+    const int nb0 = S.block_node_begin[b];
+    const int t = threadIdx.x;
+    double term = 0.;
+    for (int i = t; i < n; i += blockDim.x) {
+        double e;
+        if (i < nsys) {
+            const int node = S.block_nodes[nb0 + i / 6];
+            const double xi = S.q_delta[node * 6 + i % 6] * p.h;
+            e = xs[i] / (p.atol + p.rtol * fabs(xi));
+        } else {
+            // constraint row: find constraint by local row
+            e = 0.;
+            for (int k = S.block_con_begin[b]; k < S.block_con_end[b]; ++k) {
+                const int c = S.block_cons[k];
+                const int r = i - S.con_local[c];
+                if (r >= 0 && r < S.con_rows[c]) {
+                    e = xs[i] / (p.atol + p.rtol * fabs(S.con_lambda[c * 6 + r]));
+                }
+            }
+        }
+        term += e * e;
+    }
+    const double sum = block_sum(term, red);
+    // update (after every thread has read q_delta / lambda for the error)
+    for (int i = t; i < nsys; i += blockDim.x) {
+        const int node = S.block_nodes[nb0 + i / 6];
+        const int k = node * 6 + i % 6;
+        const double dx = xs[i];
+        S.q_delta[k] += dx / p.h;
+        if (p.is_dynamic) {
+            S.v[k] += p.gamma_prime * dx;
+            S.vd[k] += p.beta_prime * dx;
+        }
+    }
+    for (int k = S.block_con_begin[b] + t; k < S.block_con_end[b]; k += blockDim.x) {
+        const int c = S.block_cons[k];
+        for (int r = 0; r < S.con_rows[c]; ++r) S.con_lambda[c * 6 + r] += xs[S.con_local[c] + r];
+    }
+    __syncthreads();
+    for (int k = nb0 + t; k < S.block_node_end[b]; k += blockDim.x) displacement_node(S, S.block_nodes[k]);
+    __syncthreads();
+    return sum;
+}
+
+
+// ---------------------------------------------------------------------------------------
+// Conditioning (src/solver/condition_system.hpp:10-31, solve_system.hpp:20-25): keep the
+// assembled b for kyn_get_rhs, scale system rows by beta*h^2, negate.
+// ---------------------------------------------------------------------------------------
+__device__ __forceinline__ void condition_rhs(const DevSys& S, int b, double* rhs) {
+    const int n = S.block_n[b], nsys = S.block_nsys[b], off = S.block_vec_off[b];
+    const double cond = S.p.conditioner;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const double raw = rhs[i];
+        S.rhs_raw[off + i] = raw;
+        const double c = i < nsys ? raw * cond : raw;
+        rhs[i] = -1. * c;
+        S.rhs[off + i] = rhs[i];
+    }
+    __syncthreads();
+}
+
+// Solve + optional FP64 iterative refinement against the unfactored matrix (FP32 tangent),
+// then UnconditionSolution.  A: LDS factors; Ac: unfactored copy in HBM (row-major, lda).
+template <typename TT>
+__device__ void solve_refine_uncondition(const DevSys& S, int b, TT* A, const TT* Ac, int lda,
+                                         int* piv, double* rhs, double* xs, double* xs2) {
+    const int n = S.block_n[b], nsys = S.block_nsys[b], off = S.block_vec_off[b];
+    lu_solve<TT>(A, n, lda, piv, rhs, xs);
+    for (int sweep = 0; Ac != nullptr && sweep < S.refine_sweeps; ++sweep) {
+        for (int i = threadIdx.x; i < n; i += blockDim.x) {
+            double r = S.rhs[off + i];
+            const TT* row = Ac + (size_t)i * lda;
+            for (int j = 0; j < n; ++j) r -= (double)row[j] * xs[j];
+            rhs[i] = r;
+        }
+        __syncthreads();
+        lu_solve<TT>(A, n, lda, piv, rhs, xs2);
+        for (int i = threadIdx.x; i < n; i += blockDim.x) xs[i] += xs2[i];
+        __syncthreads();
+    }
+    const double cond = S.p.conditioner;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        if (i >= nsys) xs[i] /= cond;
+        S.xsol[off + i] = xs[i];
+    }
+    __syncthreads();
+}
+
+// ---------------------------------------------------------------------------------------
+// Fused form: one workgroup per model, one launch per time step
+// ---------------------------------------------------------------------------------------
+template <typename TT>
+__global__ void __launch_bounds__(kThreads) k_fused_step(DevSys S, int Nm, int Qm, int nm) {
+    extern __shared__ __align__(16) char sm[];
+    const Lay L = make_layout(Nm, Qm, nm, (int)sizeof(TT), true);
+    const int m = blockIdx.x;
+    const int t = threadIdx.x;
+    double* red = reinterpret_cast<double*>(sm + L.red);
+    int* misc = reinterpret_cast<int*>(sm + L.misc);
+    double* rhs = reinterpret_cast<double*>(sm + L.rhs);
+    double* xs = reinterpret_cast<double*>(sm + L.xs);
+    double* xs2 = reinterpret_cast<double*>(sm + L.xs2);
+    int* piv = reinterpret_cast<int*>(sm + L.piv);
+    TT* A = reinterpret_cast<TT*>(sm + L.A);
+    const StepConsts& p = S.p;
+
+    // inputs, PredictNextState, ResetConstraints (step.hpp:49-50)
+    apply_schedule_model(S, m);
+    __syncthreads();
+    for (int node = S.model_node_begin[m] + t; node < S.model_node_end[m]; node += blockDim.x)
+        predict_node(S, node);
+    for (int c = S.model_con_begin[m] + t; c < S.model_con_end[m]; c += blockDim.x)
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            S.con_lambda[c * 6 + k] = 0.;
+            S.con_sys_res[c * 6 + k] = 0.;
+        }
+    __syncthreads();
+
+    const double ndofs = (double)S.model_ndofs[m];
+    int converged = 0;
+    int iter = 0;
+    for (; iter < p.max_iter; ++iter) {
+        double err_sum = 0.;
+        for (int b = S.model_block_begin[m]; b < S.model_block_end[m]; ++b) {
+            const int n = S.block_n[b];
+            const int e = S.block_elems[S.block_elem_begin[b]];
+            element_phase<TT, true>(S, e, sm, L, n);
+            block_constraints<TT>(S, b, sm, L);
+            condition_rhs(S, b, rhs);
+            TT* Ac = nullptr;
+            if (S.Acopy != nullptr && S.refine_sweeps > 0) {
+                Ac = reinterpret_cast<TT*>(S.Acopy) + S.block_A_off[b];
+                for (int k = t; k < n * L.lda; k += blockDim.x) Ac[k] = A[k];
+                __syncthreads();
+            }
+            lu_factor<TT>(A, n, L.lda, piv, misc);
+            solve_refine_uncondition<TT>(S, b, A, Ac, L.lda, piv, rhs, xs, xs2);
+            err_sum += block_error_and_update(S, b, xs, red);
+        }
+        const double err = sqrt(err_sum / ndofs);
+        if (t == 0) S.err_hist[(size_t)m * p.max_iter + iter] = err;
+        if (!(err > 1.)) {
+            converged = 1;
+            ++iter;
+            break;
+        }
+    }
+    if (converged) {
+        for (int node = S.model_node_begin[m] + t; node < S.model_node_end[m]; node += blockDim.x)
+            epilogue_node(S, node);
+        for (int c = S.model_con_begin[m] + t; c < S.model_con_end[m]; c += blockDim.x) constraint_output(S, c);
+    }
+    if (t == 0) {
+        S.model_status[m] = converged ? 0 : 1;
+        S.model_iters[m] = iter;
+        S.model_active[m] = 0;
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Plain form: the reference's kernel sequence, one launch per stage
+// ---------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(kThreads) k_plain_predict(DevSys S) {
+    const int m = blockIdx.x, t = threadIdx.x;
+    apply_schedule_model(S, m);
+    __syncthreads();
+    for (int node = S.model_node_begin[m] + t; node < S.model_node_end[m]; node += blockDim.x)
+        predict_node(S, node);
+    for (int c = S.model_con_begin[m] + t; c < S.model_con_end[m]; c += blockDim.x)
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            S.con_lambda[c * 6 + k] = 0.;
+            S.con_sys_res[c * 6 + k] = 0.;
+        }
+    if (t == 0) {
+        S.model_status[m] = 2;
+        S.model_iters[m] = 0;
+        S.model_active[m] = 1;
+    }
+}
+
+__global__ void k_plain_clear_flag(DevSys S) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *S.any_active = 0;
+}
+
+// node -> model through the block table; nodes without dofs take no part in the iteration
+__device__ __forceinline__ int node_model(const DevSys& S, int node) {
+    const int b = S.node_block[node];
+    return b < 0 ? -1 : S.block_model[b];
+}
+
+// UpdateTangentOperator (src/step/update_tangent_operator.hpp:13-26)
+__global__ void k_plain_tangent(DevSys S) {
+    const int node = blockIdx.x * blockDim.x + threadIdx.x;
+    if (node >= S.num_nodes) return;
+    const int m = node_model(S, node);
+    if (m < 0 || !S.model_active[m]) return;
+    double qd[6];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) qd[c] = S.q_delta[node * 6 + c];
+    tangent_rot_block(S.p.h, qd, S.tan_rot + node * 9);
+}
+
+// UpdateSystemVariablesBeams (src/step/update_system_variables_beams.hpp:13-69)
+__global__ void __launch_bounds__(kThreads) k_plain_element(DevSys S, int Nm, int Qm) {
+    extern __shared__ __align__(16) char sm[];
+    const Lay L = make_layout(Nm, Qm, 1, 8, false);
+    const int e = blockIdx.x;
+    const int m = S.block_model[S.node_block[S.elem_nodes[S.elem_node_off[e]]]];
+    if (!S.model_active[m]) return;
+    element_phase<double, false>(S, e, sm, L, 0);
+}
+
+// UpdateConstraintVariables (src/step/update_constraint_variables.hpp:13-37)
+__global__ void k_plain_constraints(DevSys S) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= S.num_cons) return;
+    const int tn = S.con_target[c], bn = S.con_base[c];
+    const int m = node_model(S, tn);
+    if (m < 0 || !S.model_active[m]) return;
+    const int type = S.con_type[c];
+    const bool two = (type == 3 || type == 4);
+    double input[7];
+#pragma unroll
+    for (int k = 0; k < 7; ++k) input[k] = S.con_input[c * 7 + k];
+    if (type == 4) input[0] = S.con_control[c];
+    double tq[7], bq[7] = {0., 0., 0., 1., 0., 0., 0.};
+#pragma unroll
+    for (int k = 0; k < 7; ++k) tq[k] = S.q[tn * 7 + k];
+    if (two)
+#pragma unroll
+        for (int k = 0; k < 7; ++k) bq[k] = S.q[bn * 7 + k];
+    double res[6], bg[36], tg[36], sres[6] = {0., 0., 0., 0., 0., 0.};
+    constraint_eval(type, S.con_X0 + c * 3, S.con_axes + c * 9, input, bq, tq, res, bg, tg, sres);
+    const double* lam = S.con_lambda + c * 6;
+    double gt[36];
+    times_tangent(tg, S.tan_rot + tn * 9, gt);
+#pragma unroll
+    for (int i = 0; i < 36; ++i) {
+        S.con_tgrad[c * 36 + i] = gt[i];
+        S.con_tgradT[c * 36 + i] = tg[(i % 6) * 6 + i / 6];
+    }
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+        double acc = 0.;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) acc += tg[k * 6 + i] * lam[k];
+        S.con_tlres[c * 6 + i] = acc;
+    }
+    if (two) {
+        times_tangent(bg, S.tan_rot + bn * 9, gt);
+#pragma unroll
+        for (int i = 0; i < 36; ++i) {
+            S.con_bgrad[c * 36 + i] = gt[i];
+            S.con_bgradT[c * 36 + i] = bg[(i % 6) * 6 + i / 6];
+        }
+#pragma unroll
+        for (int i = 0; i < 6; ++i) {
+            double acc = 0.;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) acc += bg[k * 6 + i] * lam[k];
+            S.con_blres[c * 6 + i] = acc;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        S.con_res[c * 6 + k] = res[k];
+        if (type == 4) S.con_sys_res[c * 6 + k] = sres[k];
+    }
+}
+
+// AssembleSystemResidual / AssembleSystemMatrix / AssembleConstraints{Matrix,Residual}
+// (src/step/assemble_*.hpp): dense block, deterministic (elements in order, no atomics)
+__global__ void __launch_bounds__(kThreads) k_plain_assemble(DevSys S) {
+    __shared__ double rhs[512];
+    const int b = blockIdx.x, t = threadIdx.x;
+    const int m = S.block_model[b];
+    if (!S.model_active[m]) return;
+    const int n = S.block_n[b];
+    double* A = reinterpret_cast<double*>(S.A) + S.block_A_off[b];
+    const int lda = n;
+    const double cond = S.p.conditioner;
+    for (int k = t; k < n * n; k += blockDim.x) A[k] = 0.;
+    for (int i = t; i < n; i += blockDim.x) rhs[i] = 0.;
+    __syncthreads();
+    // ContributeForcesToVector: b -= f
+    for (int k = S.block_node_begin[b] + t; k < S.block_node_end[b]; k += blockDim.x) {
+        const int node = S.block_nodes[k];
+        const int l = S.node_local[node];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) rhs[l + c] -= S.f[node * 6 + c];
+    }
+    __syncthreads();
+    // ContributeBeamsToVector / ContributeBeamsToSparseMatrix, elements in order
+    for (int ke = S.block_elem_begin[b]; ke < S.block_elem_end[b]; ++ke) {
+        const int e = S.block_elems[ke];
+        const int N = S.elem_N[e], noff = S.elem_node_off[e];
+        for (int k = t; k < 6 * N; k += blockDim.x) {
+            const int i = k / 6, c = k % 6;
+            rhs[S.node_local[S.elem_nodes[noff + i]] + c] += S.elem_res[(size_t)(noff + i) * 6 + c];
+        }
+        const double* Me = S.elem_mat + S.elem_mat_off[e];
+        for (int k = t; k < N * N * 36; k += blockDim.x) {
+            const int c = k % 36, ij = k / 36;
+            const int i = ij / N, j = ij % N;
+            const int row = S.node_local[S.elem_nodes[noff + i]] + c / 6;
+            const int col = S.node_local[S.elem_nodes[noff + j]] + c % 6;
+            A[(size_t)row * lda + col] += Me[k] * cond;
+        }
+        __syncthreads();
+    }
+    // constraints (copy_constraints_to_sparse_matrix.hpp, ..._transpose_...)
+    const int c0 = S.block_con_begin[b], c1 = S.block_con_end[b];
+    for (int k = c0; k < c1; ++k) {
+        const int c = S.block_cons[k];
+        const int type = S.con_type[c];
+        const bool two = (type == 3 || type == 4);
+        const int rows = S.con_rows[c], lr = S.con_local[c];
+        const int lt = S.node_local[S.con_target[c]];
+        const int lb = two ? S.node_local[S.con_base[c]] : 0;
+        for (int q = t; q < rows * 6; q += blockDim.x) {
+            const int i = q / 6, j = q % 6;
+            A[(size_t)(lr + i) * lda + lt + j] += S.con_tgrad[c * 36 + i * 6 + j];
+            A[(size_t)(lt + j) * lda + lr + i] += S.con_tgradT[c * 36 + j * 6 + i];
+            if (two) {
+                A[(size_t)(lr + i) * lda + lb + j] += S.con_bgrad[c * 36 + i * 6 + j];
+                A[(size_t)(lb + j) * lda + lr + i] += S.con_bgradT[c * 36 + j * 6 + i];
+            }
+        }
+    }
+    __syncthreads();
+    if (t == 0) {
+        for (int k = c0; k < c1; ++k) {
+            const int c = S.block_cons[k];
+            const int lt = S.node_local[S.con_target[c]];
+            for (int j = 0; j < 6; ++j) rhs[lt + j] += S.con_sys_res[c * 6 + j];
+        }
+        for (int k = c0; k < c1; ++k) {
+            const int c = S.block_cons[k];
+            const int type = S.con_type[c];
+            if (type == 3 || type == 4) {
+                const int lb = S.node_local[S.con_base[c]];
+                for (int j = 0; j < 6; ++j) rhs[lb + j] += S.con_blres[c * 6 + j];
+            }
+            const int lt = S.node_local[S.con_target[c]];
+            for (int j = 0; j < 6; ++j) rhs[lt + j] += S.con_tlres[c * 6 + j];
+        }
+        for (int k = c0; k < c1; ++k) {
+            const int c = S.block_cons[k];
+            const int lr = S.con_local[c];
+            for (int i = 0; i < S.con_rows[c]; ++i) rhs[lr + i] += S.con_res[c * 6 + i];
+        }
+    }
+    __syncthreads();
+    condition_rhs(S, b, rhs);
+}
+
+// SolveSystem (src/step/solve_system.hpp:14-39): LU + solve per block in LDS
+__global__ void __launch_bounds__(kThreads) k_plain_solve(DevSys S, int nm) {
+    extern __shared__ __align__(16) char sm[];
+    const int b = blockIdx.x, t = threadIdx.x;
+    const int m = S.block_model[b];
+    if (!S.model_active[m]) return;
+    const int n = S.block_n[b];
+    const int lda = nm + 1;
+    double* A = reinterpret_cast<double*>(sm);
+    double* rhs = A + (size_t)nm * lda;
+    double* xs = rhs + nm;
+    int* piv = reinterpret_cast<int*>(xs + nm);
+    int* misc = piv + nm;
+    const double* Ag = reinterpret_cast<const double*>(S.A) + S.block_A_off[b];
+    for (int k = t; k < n * n; k += blockDim.x) A[(k / n) * lda + (k % n)] = Ag[k];
+    for (int i = t; i < n; i += blockDim.x) rhs[i] = S.rhs[S.block_vec_off[b] + i];
+    __syncthreads();
+    lu_factor<double>(A, n, lda, piv, misc);
+    solve_refine_uncondition<double>(S, b, A, nullptr, lda, piv, rhs, xs, xs);
+}
+
+// CalculateConvergenceError + UpdateStatePrediction + UpdateConstraintPrediction
+__global__ void __launch_bounds__(kThreads) k_plain_update(DevSys S, int iter) {
+    __shared__ double red[16];
+    const int m = blockIdx.x, t = threadIdx.x;
+    if (!S.model_active[m]) return;
+    double err_sum = 0.;
+    for (int b = S.model_block_begin[m]; b < S.model_block_end[m]; ++b)
+        err_sum += block_error_and_update(S, b, S.xsol + S.block_vec_off[b], red);
+    const double err = sqrt(err_sum / (double)S.model_ndofs[m]);
+    if (t == 0) {
+        S.err_hist[(size_t)m * S.p.max_iter + iter] = err;
+        S.model_iters[m] = iter + 1;
+        if (!(err > 1.)) {
+            S.model_status[m] = 0;
+            S.model_active[m] = 0;
+        } else if (iter + 1 >= S.p.max_iter) {
+            S.model_status[m] = 1;
+            S.model_active[m] = 0;
+        } else {
+            *S.any_active = 1;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(kThreads) k_plain_epilogue(DevSys S) {
+    const int m = blockIdx.x, t = threadIdx.x;
+    if (S.model_status[m] != 0) return;
+    for (int node = S.model_node_begin[m] + t; node < S.model_node_end[m]; node += blockDim.x)
+        epilogue_node(S, node);
+    for (int c = S.model_con_begin[m] + t; c < S.model_con_end[m]; c += blockDim.x) constraint_output(S, c);
+}
+
+// ---------------------------------------------------------------------------------------
+// Host-side launchers
+// ---------------------------------------------------------------------------------------
+size_t fused_lds_bytes(int Nm, int Qm, int nm, int precision) {
+    return (size_t)make_layout(Nm, Qm, nm, precision ? 4 : 8, true).total;
+}
+size_t element_lds_bytes(int Nm, int Qm) { return (size_t)make_layout(Nm, Qm, 1, 8, false).total; }
+size_t solve_lds_bytes(int nm) { return (size_t)nm * (nm + 1) * 8 + (size_t)nm * 16 + (size_t)nm * 4 + 64; }
+
+hipError_t set_lds_limits(size_t bytes) {
+    hipError_t e;
+    e = hipFuncSetAttribute((const void*)k_fused_step<double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (e != hipSuccess) return e;
+    e = hipFuncSetAttribute((const void*)k_fused_step<float>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (e != hipSuccess) return e;
+    e = hipFuncSetAttribute((const void*)k_plain_element, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (e != hipSuccess) return e;
+    return hipFuncSetAttribute((const void*)k_plain_solve, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+
+void launch_fused_step(const DevSys& S, int Nm, int Qm, int nm, hipStream_t st) {
+    const size_t lds = fused_lds_bytes(Nm, Qm, nm, S.precision);
+    if (S.precision)
+        hipLaunchKernelGGL(k_fused_step<float>, dim3(S.num_models), dim3(kThreads), lds, st, S, Nm, Qm, nm);
+    else
+        hipLaunchKernelGGL(k_fused_step<double>, dim3(S.num_models), dim3(kThreads), lds, st, S, Nm, Qm, nm);
+}
+void launch_advance_slot(const DevSys& S, hipStream_t st) {
+    hipLaunchKernelGGL(k_advance_slot, dim3(1), dim3(64), 0, st, S);
+}
+void launch_plain_predict(const DevSys& S, hipStream_t st) {
+    hipLaunchKernelGGL(k_plain_predict, dim3(S.num_models), dim3(kThreads), 0, st, S);
+}
+void launch_plain_iteration(const DevSys& S, int Nm, int Qm, int nm, int iter, bool solve,
+                            hipStream_t st, hipEvent_t e0, hipEvent_t e1) {
+    hipLaunchKernelGGL(k_plain_clear_flag, dim3(1), dim3(64), 0, st, S);
+    hipLaunchKernelGGL(k_plain_tangent, dim3((S.num_nodes + 255) / 256), dim3(256), 0, st, S);
+    if (e0) (void)hipEventRecord(e0, st);
+    hipLaunchKernelGGL(k_plain_element, dim3(S.num_elems), dim3(kThreads), element_lds_bytes(Nm, Qm), st, S, Nm, Qm);
+    if (e1) (void)hipEventRecord(e1, st);
+    if (S.num_cons > 0)
+        hipLaunchKernelGGL(k_plain_constraints, dim3((S.num_cons + 63) / 64), dim3(64), 0, st, S);
+    hipLaunchKernelGGL(k_plain_assemble, dim3(S.num_blocks), dim3(kThreads), 0, st, S);
+    if (!solve) return;
+    hipLaunchKernelGGL(k_plain_solve, dim3(S.num_blocks), dim3(kThreads), solve_lds_bytes(nm), st, S, nm);
+    hipLaunchKernelGGL(k_plain_update, dim3(S.num_models), dim3(kThreads), 0, st, S, iter);
+}
+void launch_plain_epilogue(const DevSys& S, hipStream_t st) {
+    hipLaunchKernelGGL(k_plain_epilogue, dim3(S.num_models), dim3(kThreads), 0, st, S);
+}
+
+}  // namespace kyn

@@ -1,0 +1,106 @@
+// Device-resident system layout shared by the host setup (kyn_host.cpp) and the kernels
+// (kyn_kernels.hip).  All arrays are structure-of-arrays in HBM, grouped so that one
+// workgroup reads one element's / one block's data contiguously.
+//
+// Vocabulary (reference names):
+//   model      one Model of the reference: its own Step, Newton loop and convergence_err
+//   block      one independent connected component of a model's KKT system (blade + root BC)
+//   element    one beam element (src/elements/beams/beams.hpp)
+//   node       one State row (src/state/state.hpp)
+//   constraint one Constraints row (src/constraints/constraints.hpp)
+#pragma once
+
+#include <cstdint>
+
+namespace kyn {
+
+constexpr int kMaxElemNodes = 21;  // register tile of the element contraction (3 x 21)
+constexpr int kMaxElemQps = 64;
+constexpr int kThreads = 256;      // workgroup size of every per-element / per-block kernel
+
+struct StepConsts {
+    double h, alpha_m, alpha_f, gamma, beta, gamma_prime, beta_prime, conditioner, atol, rtol;
+    int is_dynamic;
+    int max_iter;
+};
+
+struct DevSys {
+    StepConsts p;
+    int num_models, num_blocks, num_elems, num_nodes, num_cons;
+    int precision;      // 0 FP64 tangent, 1 FP32 tangent
+    int refine_sweeps;  // FP64 iterative refinement sweeps (FP32 tangent)
+
+    // ---- state (src/state/state.hpp), [n][7] / [n][6]
+    double *x0, *x, *q_prev, *q, *q_delta, *v, *vd, *a, *f;
+    double* tan_rot;  // [n][9]: rotational block of the stored tangent (T^T)
+
+    // ---- models
+    const int *model_node_begin, *model_node_end;
+    const int *model_block_begin, *model_block_end;
+    const int *model_con_begin, *model_con_end;
+    const int* model_ndofs;
+    const double* model_gravity;  // [m][3]
+    const int* model_load_begin;  // into sched_load_nodes (sorted by node), [m+1]
+
+    // ---- blocks
+    const int* block_model;
+    const int* block_n;           // KKT size
+    const int* block_nsys;        // system dofs of the block (6 x nodes)
+    const int *block_elem_begin, *block_elem_end;  // into block_elems
+    const int* block_elems;
+    const int *block_con_begin, *block_con_end;    // into block_cons
+    const int* block_cons;
+    const int *block_node_begin, *block_node_end;  // into block_nodes (ascending)
+    const int* block_nodes;
+    const int64_t* block_A_off;   // into A (elements of the storage type)
+    const int* block_vec_off;     // into rhs / xsol
+
+    // ---- node -> block-local dof
+    const int* node_block;
+    const int* node_local;        // -1 if the node has no dofs
+
+    // ---- elements (src/elements/beams/beams.hpp)
+    const int *elem_N, *elem_Q;
+    const int* elem_node_off;     // into elem_nodes and node_FX rows
+    const int* elem_nodes;
+    const int* elem_qp_off;       // into qp arrays (rows)
+    const int* elem_shape_off;    // into shape tables (N*Q entries)
+    const int64_t* elem_mat_off;  // into elem_mat (N*N*36 entries)
+    const double *qp_weight, *qp_jac, *qp_Mstar, *qp_Cstar, *qp_x0p, *qp_r0;
+    const double *shape_interp, *shape_deriv;  // [N][Q] per table
+    double* node_FX;                           // [sum N][6]
+    double* elem_res;                          // [sum N][6]
+    double* elem_mat;                          // [sum N^2][36] (plain form)
+
+    // ---- constraints (src/constraints/constraints.hpp)
+    const int *con_type, *con_base, *con_target, *con_rows, *con_model_row;
+    const int* con_local;  // block-local index of the first constraint row
+    const double *con_X0, *con_axes;  // [c][3], [c][9]
+    double *con_input, *con_control;  // [c][7], [c]
+    double *con_lambda, *con_res, *con_sys_res, *con_output;  // [c][6],[c][6],[c][6],[c][3]
+    double *con_tgrad, *con_bgrad, *con_tgradT, *con_bgradT;  // [c][36] (plain form)
+    double *con_tlres, *con_blres;                            // [c][6]
+
+    // ---- solver
+    void* A;              // blocks' dense KKT matrices (double, or float for FP32 tangent)
+    void* Acopy;          // unfactored copy for refinement (FP32 tangent with sweeps > 0)
+    double *rhs, *xsol;   // [sum n]
+    double* rhs_raw;      // assembled b before conditioning (for kyn_get_rhs)
+
+    // ---- per-model Newton bookkeeping
+    double* err_hist;     // [m][max_iter]
+    int* model_iters;
+    int* model_status;    // 0 converged, 1 not converged, 2 iterating
+    int* model_active;
+    int* any_active;      // plain form: set by k_update when any model keeps iterating
+
+    // ---- device-resident input schedule
+    const double* sched_cdisp;   // [steps][num_cons][7] or null
+    const int* sched_load_nodes; // [nl]
+    const double* sched_loads;   // [steps][nl][6] or null
+    int sched_num_loaded;
+    int sched_steps;
+    int* sched_slot;             // device counter: schedule row of the next step
+};
+
+}  // namespace kyn

@@ -152,17 +152,22 @@ int orc_set_nodal_loads(void* s, const double* f) {
     return 0;
 }
 int orc_set_element_node_loads(void* s, size_t elem, const double* fx) {
-    auto& e = static_cast<OrcSystem*>(s)->sys.beams.at(elem);
+    auto& sy = static_cast<OrcSystem*>(s)->sys;
+    if (elem >= sy.beams.size()) return -1;
+    auto& e = sy.beams[elem];
     std::copy(fx, fx + e.N * 6, e.node_FX.begin());
     return 0;
 }
 int orc_set_constraint_displacement(void* s, size_t id, const double* d) {
-    auto& c = static_cast<OrcSystem*>(s)->sys.cons.at(id);
-    std::copy(d, d + 7, c.input.begin());
+    auto& sy = static_cast<OrcSystem*>(s)->sys;
+    if (id >= sy.cons.size()) return -1;
+    std::copy(d, d + 7, sy.cons[id].input.begin());
     return 0;
 }
 int orc_set_control(void* s, size_t id, double v) {
-    static_cast<OrcSystem*>(s)->sys.cons.at(id).control = v;
+    auto& sy = static_cast<OrcSystem*>(s)->sys;
+    if (id >= sy.cons.size()) return -1;
+    sy.cons[id].control = v;
     return 0;
 }
 size_t orc_get_convergence(void* s, double* out, size_t cap) {
@@ -183,7 +188,9 @@ int orc_get_solution(void* s, double* out) {
 }
 // which: 0 lambda, 1 residual_terms, 2 output(3)
 int orc_get_constraint(void* s, size_t id, int which, double* out) {
-    auto& c = static_cast<OrcSystem*>(s)->sys.cons.at(id);
+    auto& sy = static_cast<OrcSystem*>(s)->sys;
+    if (id >= sy.cons.size()) return -1;
+    auto& c = sy.cons[id];
     if (which == 0) std::copy(c.lambda.begin(), c.lambda.end(), out);
     else if (which == 1) std::copy(c.res.begin(), c.res.end(), out);
     else if (which == 2) std::copy(c.output.begin(), c.output.end(), out);
@@ -193,8 +200,9 @@ int orc_get_constraint(void* s, size_t id, int which, double* out) {
 // Element evaluation at the current state: residual [N][6], system matrix [N][N][36]
 int orc_element_terms(void* s, size_t elem, int update_tangent, double* resid, double* mat) {
     auto* o = static_cast<OrcSystem*>(s);
+    if (elem >= o->sys.beams.size()) return -1;
     if (update_tangent) UpdateTangentOperator(o->params, o->sys);
-    auto& e = o->sys.beams.at(elem);
+    auto& e = o->sys.beams[elem];
     CalculateQuadraturePointValues(o->params.beta_prime, o->params.gamma_prime, o->sys, e);
     if (resid) std::copy(e.residual.begin(), e.residual.end(), resid);
     if (mat) std::copy(e.sysmat.begin(), e.sysmat.end(), mat);
@@ -203,7 +211,9 @@ int orc_element_terms(void* s, size_t elem, int update_tangent, double* resid, d
 // Element QP setup views: which 0 weight, 1 jacobian, 2 shape_interp, 3 shape_deriv,
 // 4 qp_x0, 5 qp_x0_prime, 6 qp_r0, 7 qp_Mstar, 8 qp_Cstar
 int orc_element_setup(void* s, size_t elem, int which, double* out) {
-    auto& e = static_cast<OrcSystem*>(s)->sys.beams.at(elem);
+    auto& sy = static_cast<OrcSystem*>(s)->sys;
+    if (elem >= sy.beams.size()) return -1;
+    auto& e = sy.beams[elem];
     const std::vector<double>* v = nullptr;
     switch (which) {
         case 0: v = &e.weight; break;

@@ -1,0 +1,164 @@
+"""GPU parity: the HIP library (through the C-ABI) against the CPU oracle and the
+reference's golden vectors.
+
+Bars: the FP64 GPU forms match the FP64 oracle to <= 1e-12 relative in
+residuals and state (relative = max |gpu - oracle| / max |oracle| per field) and reproduce
+the oracle's Newton iteration counts exactly; reference golden vectors are met at the
+reference's own tolerances.  The FP32-tangent form is checked against the FP64 oracle at
+<= 1e-9 relative final state.
+"""
+
+import numpy as np
+import pytest
+
+import openturbine_amd as ota
+from golden_data import B_VIEW, X_VIEW, load, solver_case
+from openturbine_amd import workloads as W
+from oracle_lib import OracleLib, OracleSystem
+
+pytestmark = pytest.mark.gpu
+
+MODES = [pytest.param(ota.EXEC_PLAIN, id="plain"), pytest.param(ota.EXEC_FUSED, id="fused")]
+STATE_FIELDS = ("q", "q_delta", "v", "vd", "a", "x")
+
+
+@pytest.fixture(scope="module")
+def kyn():
+    return ota.KynLib()
+
+
+@pytest.fixture(scope="module")
+def orc():
+    return OracleLib()
+
+
+def rel(a, b):
+    scale = np.max(np.abs(b))
+    return float(np.max(np.abs(np.asarray(a) - np.asarray(b))) / (scale if scale > 0 else 1.0))
+
+
+def test_device_is_gfx950(kyn):
+    name, arch = kyn.device_info()
+    print("device:", name, arch, flush=True)
+    assert arch.startswith("gfx950")
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_solver_assembly(kyn, orc, mode):
+    d, exp = solver_case("SetUpSolverAndAssemble")
+    spec = W.solver_regression_spec(d["inputs"])
+    g = ota.KynSystem(kyn, spec, exec_mode=mode)
+    g.debug_assemble(spec.step_inputs(0))
+    o = OracleSystem(orc, spec)
+    o.debug_assemble(spec.step_inputs(0))
+    np.testing.assert_allclose(g.rhs(), exp[B_VIEW], atol=1e-6)
+    assert rel(g.rhs(), o.rhs()) < 1e-12
+    for f in ("q", "q_delta", "v"):
+        assert rel(g.get(f), o.get(f)) < 1e-14, f
+    r_g, m_g = g.element_terms(0, 6)
+    r_o, m_o = o.element_terms(0, 6, update_tangent=True)
+    assert rel(r_g, r_o) < 1e-12
+    assert rel(m_g, m_o) < 1e-12
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_solver_one_iteration(kyn, orc, mode):
+    d, exp = solver_case("SetupAndTakeNoSteps")
+    spec = W.solver_regression_spec(d["inputs"])
+    g = ota.KynSystem(kyn, spec, exec_mode=mode)
+    o = OracleSystem(orc, spec)
+    assert g.step() == ota.KYN_NOT_CONVERGED and o.step() == 1
+    np.testing.assert_allclose(g.convergence()[0], d["convergence_err0"], atol=1e-6)
+    assert abs(g.convergence()[0] - o.convergence()[0]) / o.convergence()[0] < 1e-12
+    np.testing.assert_allclose(g.solution(), exp[X_VIEW], atol=1e-6)
+    assert rel(g.solution(), o.solution()) < 1e-10
+    for f in ("q_delta", "v", "vd", "q"):
+        np.testing.assert_allclose(g.get(f).ravel(), exp["state." + f], atol=1e-6)
+        assert rel(g.get(f), o.get(f)) < 1e-12, f
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_cantilever_sine(kyn, orc, mode):
+    fx = load("cantilever_sine.json")
+    spec = W.cantilever_sine_spec(fx)
+    g = ota.KynSystem(kyn, spec, exec_mode=mode)
+    o = OracleSystem(orc, spec)
+    for k in range(3):
+        assert g.step() == 0 and o.step() == 0
+        np.testing.assert_allclose(g.get("q")[fx["tip_node"], :3], fx["tip_q_translation"][k], atol=1e-6)
+        assert len(g.convergence()) == len(o.convergence())
+        for f in STATE_FIELDS:
+            assert rel(g.get(f), o.get(f)) < 1e-12, f
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_iea15_rotor(kyn, orc, mode):
+    fx = load("iea15_rotor.json")
+    spec = W.iea15_rotor_spec(fx)
+    g = ota.KynSystem(kyn, spec, exec_mode=mode)
+    o = OracleSystem(orc, spec)
+    for _ in range(spec.num_steps):
+        assert g.step() == 0 and o.step() == 0
+        assert len(g.convergence()) == len(o.convergence())
+    q = g.get("q")
+    for ch in fx["checks"]:
+        ref = q[ch["equals"][0], ch["equals"][1]] if "equals" in ch else ch["value"]
+        assert abs(q[ch["node"], ch["comp"]] - ref) <= ch["tol"], ch
+    for f in STATE_FIELDS:
+        assert rel(g.get(f), o.get(f)) < 1e-12, f
+
+
+def run_pair(kyn, orc, spec, steps, mode, **kw):
+    g = ota.KynSystem(kyn, spec, exec_mode=mode, **kw)
+    o = OracleSystem(orc, spec)
+    hist = []
+    for _ in range(steps):
+        inputs = spec.step_inputs(g.step_index, o.get) if spec.step_inputs else None
+        sg = g.step(inputs)
+        so = o.step(inputs)
+        hist.append((sg, so, len(g.convergence()), len(o.convergence())))
+    return g, o, hist
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_config1_cantilever_follower_load(kyn, orc, mode):
+    spec = W.cantilever_config_spec(num_steps=40)
+    g, o, hist = run_pair(kyn, orc, spec, 40, mode)
+    for sg, so, ng, no in hist:
+        assert sg == 0 and so == 0
+        assert ng == no
+    for f in STATE_FIELDS:
+        assert rel(g.get(f), o.get(f)) < 1e-12, f
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_config2_rotor(kyn, orc, mode):
+    spec = W.rotor_config_spec(num_steps=30)
+    g, o, hist = run_pair(kyn, orc, spec, 30, mode)
+    for sg, so, ng, no in hist:
+        assert sg == so
+        assert ng == no
+    for f in STATE_FIELDS:
+        assert rel(g.get(f), o.get(f)) < 1e-12, f
+
+
+def test_small_ensemble_fused_fp64_and_fp32(kyn, orc):
+    specs = W.ensemble_config_specs(num_rotors=6, num_steps=5)
+    g64 = ota.KynSystem(kyn, specs, exec_mode=ota.EXEC_FUSED)
+    g32 = ota.KynSystem(kyn, specs, exec_mode=ota.EXEC_FUSED, precision=ota.TANGENT_FP32, refine_sweeps=2)
+    oracles = [OracleSystem(orc, sp) for sp in specs]
+    for _ in range(5):
+        assert g64.step() == 0
+        assert g32.step() == 0
+        for o in oracles:
+            assert o.step() == 0
+        _, it64 = g64.status()
+        for k, o in enumerate(oracles):
+            assert it64[k] == len(o.convergence())
+    q64, q32 = g64.get("q"), g32.get("q")
+    for k, o in enumerate(oracles):
+        sl = g64.model_slice(k)
+        assert rel(q64[sl], o.get("q")) < 1e-12
+        e32 = rel(q32[sl], o.get("q"))
+        print(f"rotor {k}: FP32-tangent final q rel error {e32:.2e}", flush=True)
+        assert e32 < 1e-9
