@@ -1,0 +1,215 @@
+"""Bench: batched FP32-tangent time steps/s vs plain FP64 on the 4096-rotor ensemble (1 GPU).
+
+Workload (one "step" = one generalized-alpha time step of every model): 4096 independent
+3-blade rotors, one beam element per blade with N uniform in {6, 11, 16, 21} GLL nodes and
+Q = 2N - 2 Gauss points, L in [30, 120] m, hub speed in [0.5, 1.5] rad/s, gravity, normal
+tip loads (sigma 1e4 N) every step, h = 0.01 s, rho_inf = 0.9, Newton with the reference's
+increment criterion (atol 1e-10, rtol 1e-6).  Inputs are synthetic (SplitMix64 seed
+20261202) and resident in HBM (device input schedule) before timing starts.
+
+Timed legs (all on the one GPU):
+  value            fused FP32-tangent form, one hipGraph-replayed kernel per step
+  fused_fp64       fused form, FP64 tangent
+  plain_fp64       the reference's launch structure (one kernel per stage, host reads the
+                   convergence error every Newton iteration), FP64
+Roofline: the fused FP32 step kernel, algorithmic flops per launch (element contraction
+144 N^2 Q + LU 2/3 n^3 + triangular solves 2 n^2 per block per Newton iteration, summed over
+the iterations actually taken) over its HIP-event-timed duration.
+cpu_baseline: the CPU oracle (same algorithm, dense pivoted LU instead of KLU) on a bounded
+sample of the same ensemble, OpenMP over rotors on the host cores.
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+import openturbine_amd as ota  # noqa: E402
+from openturbine_amd import workloads as W  # noqa: E402
+
+METRIC = "time steps/s + qp-kernel HBM GB/s: batched FP32-tangent vs plain FP64, 1 GPU"
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def block_flops(specs):
+    """Per model: flops of one Newton iteration summed over its blocks (blades)."""
+    out = []
+    for sp in specs:
+        f = 0.0
+        for b in sp.beams:
+            N = len(b.nodes)
+            Q = len(b.quad)
+            n = 6 * N + 6
+            f += 144.0 * N * N * Q + (2.0 / 3.0) * n**3 + 2.0 * n * n
+        out.append(f)
+    return np.array(out)
+
+
+def block_bytes(specs, s):
+    """Per model: algorithmic HBM bytes of one Newton iteration, 8(67N + 87Q) + 36 s N^2 per
+    element (element state/section inputs and its tangent at s bytes per entry)."""
+    out = []
+    for sp in specs:
+        b_ = 0.0
+        for b in sp.beams:
+            N = len(b.nodes)
+            Q = len(b.quad)
+            b_ += 8.0 * (67 * N + 87 * Q) + 36.0 * s * N * N
+        out.append(b_)
+    return np.array(out)
+
+
+def run_leg(lib, specs, sched, *, exec_mode, precision, warmup, steps, refine=0, use_graph=True):
+    t0 = time.time()
+    sysm = ota.KynSystem(lib, specs, exec_mode=exec_mode, precision=precision, refine_sweeps=refine)
+    log(f"  system built in {time.time() - t0:.1f}s ({sysm.n} nodes, {sysm.num_models} models)")
+    cdisp, tips, loads = sched
+    sysm.set_schedule(warmup + steps, cdisp[: warmup + steps], tips, loads[: warmup + steps])
+    if warmup:
+        sysm.step_many(warmup, use_graph=use_graph)
+    it0, _ = sysm.totals()
+    t0 = time.perf_counter()
+    rc = sysm.step_many(steps, use_graph=use_graph)
+    wall = time.perf_counter() - t0
+    it1, fl = sysm.totals()
+    dev_ms = sysm.last_step_ms()
+    kern_ms = sysm.last_kernel_ms()
+    return {"system": sysm, "rc": rc, "wall_s": wall, "device_ms": dev_ms, "kernel_ms": kern_ms,
+            "iters": (it1 - it0), "fails": int(fl.sum())}
+
+
+def cpu_baseline(specs_sample, steps):
+    sys.path.insert(0, str(ROOT / "tests"))
+    from oracle_lib import OracleLib, OracleSystem, step_many
+
+    lib = OracleLib()
+    systems = [OracleSystem(lib, sp) for sp in specs_sample]
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    t0 = time.perf_counter()
+    for k in range(steps):
+        inputs = [sp.step_inputs(k) for sp in specs_sample]
+        st = step_many(lib, systems, inputs)
+        assert all(s == 0 for s in st), "oracle did not converge"
+    wall = time.perf_counter() - t0
+    return wall, cores
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--rotors", type=int, default=4096)
+    ap.add_argument("--plain-steps", type=int, default=3)
+    ap.add_argument("--cpu-rotors", type=int, default=48)
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--skip-plain", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); the ensemble "
+              "shards by rotor with no per-iteration exchange, see DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+
+    lib = ota.KynLib()
+    name, arch = lib.device_info()
+    log(f"device {name} ({arch})")
+    t0 = time.time()
+    specs = W.ensemble_config_specs(num_rotors=args.rotors, num_steps=args.warmup + args.steps)
+    log(f"built {len(specs)} rotor specs in {time.time() - t0:.1f}s")
+    t0 = time.time()
+    nsched = args.warmup + max(args.steps, args.plain_steps)
+    sched = W.ensemble_schedule(specs, nsched)
+    log(f"built input schedule ({nsched} steps) in {time.time() - t0:.1f}s")
+
+    flops_iter = block_flops(specs)
+    bytes_iter32 = block_bytes(specs, 4)
+
+    log("leg: fused FP32 tangent (hipGraph)")
+    r32 = run_leg(lib, specs, sched, exec_mode=ota.EXEC_FUSED, precision=ota.TANGENT_FP32,
+                  warmup=args.warmup, steps=args.steps, refine=1)
+    log(f"  {args.steps} steps in {r32['wall_s']:.3f}s, fails {r32['fails']}, mean iters/model/step "
+        f"{r32['iters'].mean() / args.steps:.2f}")
+    # kernel timing without the graph (events bracket the step kernel alone)
+    sysm = r32["system"]
+    sysm.step_many(2, use_graph=False)
+    kern_ms = sysm.last_kernel_ms()
+    it_total, _ = sysm.totals()
+    del r32["system"]
+
+    log("leg: fused FP64")
+    r64 = run_leg(lib, specs, sched, exec_mode=ota.EXEC_FUSED, precision=ota.TANGENT_FP64,
+                  warmup=args.warmup, steps=args.steps)
+    log(f"  {args.steps} steps in {r64['wall_s']:.3f}s, fails {r64['fails']}")
+    del r64["system"]
+
+    rp = None
+    if not args.skip_plain:
+        log("leg: plain FP64 (reference launch structure)")
+        rp = run_leg(lib, specs, sched, exec_mode=ota.EXEC_PLAIN, precision=ota.TANGENT_FP64,
+                     warmup=1, steps=args.plain_steps, use_graph=False)
+        log(f"  {args.plain_steps} steps in {rp['wall_s']:.3f}s, fails {rp['fails']}")
+        del rp["system"]
+
+    value = args.steps / r32["wall_s"]
+    ms_per_step = 1000.0 * r32["wall_s"] / args.steps
+    # roofline of the fused FP32 step kernel: flops of the iterations taken per step
+    iters_per_step = r32["iters"] / args.steps
+    flops_launch = float(np.sum(iters_per_step * flops_iter))
+    bytes_launch = float(np.sum(iters_per_step * bytes_iter32))
+    kern_s = (kern_ms if kern_ms > 0 else ms_per_step) / 1000.0
+    achieved = flops_launch / kern_s / 1e12
+    result = {
+        "metric": METRIC,
+        "value": round(value, 4),
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32-tangent/f64-residual",
+        "data": "synthetic",
+        "config": {"workload": f"ensemble of {args.rotors} independent 3-blade rotors, N in {{6,11,16,21}}, "
+                               "Q=2N-2, h=0.01, fused one-kernel-per-step, hipGraph",
+                   "models": args.rotors, "blades": 3 * args.rotors},
+        "fused_fp64_steps_per_s": round(args.steps / r64["wall_s"], 4),
+        "plain_fp64_steps_per_s": round(args.plain_steps / rp["wall_s"], 4) if rp else None,
+        "speedup_fp32_fused_vs_plain_fp64": round(value / (args.plain_steps / rp["wall_s"]), 2) if rp else None,
+        "newton_iters_per_model_step": {"fp32": round(float(r32["iters"].mean()) / args.steps, 3),
+                                         "fp64": round(float(r64["iters"].mean()) / args.steps, 3)},
+        "failed_model_steps": {"fp32": r32["fails"], "fp64": r64["fails"]},
+        "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": 157.3, "unit": "TFLOP/s",
+                     "frac": round(achieved / 157.3, 5), "traffic": None,
+                     "kernel": "k_fused_step<float>", "kernel_ms": round(kern_s * 1000.0, 4),
+                     "algorithmic_flops_per_launch": flops_launch,
+                     "hbm_algorithmic_bytes_per_launch": bytes_launch,
+                     "hbm_achieved_GBps": round(bytes_launch / kern_s / 1e9, 2)},
+        "cpu_baseline": None,
+    }
+    if not args.skip_cpu:
+        sample = specs[: args.cpu_rotors]
+        log(f"cpu baseline: oracle on {len(sample)} rotors x {args.cpu_steps} steps")
+        wall, cores = cpu_baseline(sample, args.cpu_steps)
+        rotor_steps_per_s = len(sample) * args.cpu_steps / wall
+        result["cpu_baseline"] = {"value": round(rotor_steps_per_s / args.rotors, 6), "unit": "steps/s",
+                                  "cores": cores, "kind": "port",
+                                  "sample": f"{len(sample)} rotors x {args.cpu_steps} steps of the same ensemble "
+                                            f"({wall:.1f}s wall), scaled to {args.rotors} rotors"}
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

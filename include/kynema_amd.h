@@ -120,6 +120,13 @@ size_t kyn_system_constraint_offset(kyn_system s, size_t model);
  * Advances every model by one step.  Returns KYN_OK if every model converged,
  * KYN_NOT_CONVERGED if any did not (per-model status via kyn_get_step_status). */
 int kyn_step(kyn_system s);
+/* Advances num_steps steps back to back with one synchronisation at the end (the inputs come
+ * from kyn_set_input_schedule).  use_graph != 0 replays a captured hipGraph of one fused
+ * step.  Returns KYN_OK if every model converged in every step so far.  A loop of Step
+ * calls in the reference (test_rotor.cpp:112-135); no single counterpart. */
+int kyn_step_many(kyn_system s, size_t num_steps, int use_graph);
+/* per model: Newton iterations summed over all steps, and the number of failed steps */
+int kyn_get_totals(kyn_system s, int* iterations, int* failures);
 /* per model: 0 converged / 1 not converged, and Newton iteration count of the last step */
 int kyn_get_step_status(kyn_system s, int* status, int* iterations);
 /* Solver::convergence_err of one model for the last step; returns the count */

@@ -87,6 +87,8 @@ class KynLib:
             getattr(L, f).restype = C.c_size_t
         L.kyn_step.argtypes = [C.c_void_p]
         L.kyn_get_step_status.argtypes = [C.c_void_p, _ip, _ip]
+        L.kyn_step_many.argtypes = [C.c_void_p, C.c_size_t, C.c_int]
+        L.kyn_get_totals.argtypes = [C.c_void_p, _ip, _ip]
         L.kyn_get_convergence_history.argtypes = [C.c_void_p, C.c_size_t, _dp, C.c_size_t]
         L.kyn_get_convergence_history.restype = C.c_size_t
         L.kyn_last_step_ms.argtypes = [C.c_void_p]
@@ -265,6 +267,17 @@ class KynSystem:
         rc = self._check(self.lib.L.kyn_step(self.h))
         self.step_index += 1
         return rc
+
+    def step_many(self, num_steps, use_graph=True):
+        rc = self._check(self.lib.L.kyn_step_many(self.h, num_steps, int(use_graph)))
+        self.step_index += num_steps
+        return rc
+
+    def totals(self):
+        it = np.zeros(self.num_models, dtype=np.int32)
+        fl = np.zeros(self.num_models, dtype=np.int32)
+        self._check(self.lib.L.kyn_get_totals(self.h, it.ctypes.data_as(_ip), fl.ctypes.data_as(_ip)))
+        return it, fl
 
     def status(self):
         st = np.zeros(self.num_models, dtype=np.int32)

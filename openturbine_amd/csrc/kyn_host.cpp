@@ -816,6 +816,8 @@ kyn_system kyn_system_create(const kyn_model* models, size_t num_models, const k
         S.model_iters = Ar.zeros<int>(num_models);
         S.model_status = Ar.zeros<int>(num_models);
         S.model_active = Ar.zeros<int>(num_models);
+        S.model_iter_total = Ar.zeros<int>(num_models);
+        S.model_fail_total = Ar.zeros<int>(num_models);
         S.any_active = Ar.zeros<int>(1);
         S.sched_slot = Ar.zeros<int>(1);
         S.sched_steps = 0;
@@ -889,6 +891,81 @@ int kyn_step(kyn_system s) {
     return guarded([&] {
         if (!s) throw KynError(KYN_ERR_INVALID, "null system");
         return run_step(s);
+    });
+}
+
+int kyn_step_many(kyn_system s, size_t num_steps, int use_graph) {
+    return guarded([&] {
+        if (!s) throw KynError(KYN_ERR_INVALID, "null system");
+        if (num_steps == 0) return KYN_OK;
+        auto& S = s->S;
+        if (s->exec_mode != KYN_EXEC_FUSED) {
+            int rc = KYN_OK;
+            float kms = 0.f, tms = 0.f;
+            for (size_t k = 0; k < num_steps; ++k) {
+                if (run_step(s) != KYN_OK) rc = KYN_NOT_CONVERGED;
+                kms += s->last_kernel_ms;
+                tms += s->last_step_ms;
+            }
+            s->last_kernel_ms = kms / static_cast<float>(num_steps);
+            s->last_step_ms = tms;
+            return rc;
+        }
+        if (use_graph && !s->graph_ready) {
+            if (s->graph_exec) (void)hipGraphExecDestroy(s->graph_exec);
+            if (s->graph) (void)hipGraphDestroy(s->graph);
+            s->graph_exec = nullptr;
+            s->graph = nullptr;
+            hip_check(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+            kyn::launch_fused_step(S, s->Nm, s->Qm, s->nm, s->stream);
+            kyn::launch_advance_slot(S, s->stream);
+            hip_check(hipStreamEndCapture(s->stream, &s->graph), "hipStreamEndCapture");
+            hip_check(hipGraphInstantiate(&s->graph_exec, s->graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+            s->graph_ready = true;
+        }
+        std::vector<hipEvent_t> ev(2 * num_steps);
+        for (auto& e : ev) hip_check(hipEventCreate(&e), "event");
+        hip_check(hipEventRecord(s->ev0, s->stream), "event");
+        for (size_t k = 0; k < num_steps; ++k) {
+            hip_check(hipEventRecord(ev[2 * k], s->stream), "event");
+            if (use_graph) {
+                hip_check(hipGraphLaunch(s->graph_exec, s->stream), "hipGraphLaunch");
+                hip_check(hipEventRecord(ev[2 * k + 1], s->stream), "event");
+            } else {
+                kyn::launch_fused_step(S, s->Nm, s->Qm, s->nm, s->stream);
+                hip_check(hipGetLastError(), "k_fused_step launch");
+                hip_check(hipEventRecord(ev[2 * k + 1], s->stream), "event");
+                kyn::launch_advance_slot(S, s->stream);
+            }
+        }
+        hip_check(hipEventRecord(s->ev1, s->stream), "event");
+        hip_check(hipStreamSynchronize(s->stream), "sync");
+        float total = 0.f, kern = 0.f;
+        hip_check(hipEventElapsedTime(&total, s->ev0, s->ev1), "elapsed");
+        for (size_t k = 0; k < num_steps; ++k) {
+            float ms = 0.f;
+            hip_check(hipEventElapsedTime(&ms, ev[2 * k], ev[2 * k + 1]), "elapsed");
+            kern += ms;
+        }
+        for (auto& e : ev) (void)hipEventDestroy(e);
+        s->last_step_ms = total;
+        s->last_kernel_ms = kern / static_cast<float>(num_steps);
+        std::vector<int> fails(s->num_models);
+        hip_check(hipMemcpy(fails.data(), S.model_fail_total, fails.size() * sizeof(int), hipMemcpyDeviceToHost), "D2H");
+        for (int f : fails)
+            if (f != 0) return KYN_NOT_CONVERGED;
+        return KYN_OK;
+    });
+}
+
+int kyn_get_totals(kyn_system s, int* iterations, int* failures) {
+    return guarded([&] {
+        if (!s) throw KynError(KYN_ERR_INVALID, "null system");
+        if (iterations)
+            hip_check(hipMemcpy(iterations, s->S.model_iter_total, s->num_models * sizeof(int), hipMemcpyDeviceToHost), "D2H");
+        if (failures)
+            hip_check(hipMemcpy(failures, s->S.model_fail_total, s->num_models * sizeof(int), hipMemcpyDeviceToHost), "D2H");
+        return KYN_OK;
     });
 }
 

@@ -872,6 +872,8 @@ __global__ void __launch_bounds__(kThreads) k_fused_step(DevSys S, int Nm, int Q
         S.model_status[m] = converged ? 0 : 1;
         S.model_iters[m] = iter;
         S.model_active[m] = 0;
+        S.model_iter_total[m] += iter;
+        S.model_fail_total[m] += converged ? 0 : 1;
     }
 }
 
@@ -1120,6 +1122,10 @@ __global__ void __launch_bounds__(kThreads) k_plain_update(DevSys S, int iter) {
 
 __global__ void __launch_bounds__(kThreads) k_plain_epilogue(DevSys S) {
     const int m = blockIdx.x, t = threadIdx.x;
+    if (t == 0) {
+        S.model_iter_total[m] += S.model_iters[m];
+        S.model_fail_total[m] += S.model_status[m] == 0 ? 0 : 1;
+    }
     if (S.model_status[m] != 0) return;
     for (int node = S.model_node_begin[m] + t; node < S.model_node_end[m]; node += blockDim.x)
         epilogue_node(S, node);

@@ -92,6 +92,8 @@ struct DevSys {
     int* model_iters;
     int* model_status;    // 0 converged, 1 not converged, 2 iterating
     int* model_active;
+    int* model_iter_total;  // Newton iterations summed over steps
+    int* model_fail_total;  // steps that did not converge
     int* any_active;      // plain form: set by k_update when any model keeps iterating
 
     // ---- device-resident input schedule

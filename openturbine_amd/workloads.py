@@ -299,7 +299,7 @@ C1_MASS = np.array([10.0, 10.0, 10.0, 0.5, 0.25, 0.25])
 
 
 def cantilever_config_spec(num_nodes=8, num_qps=11, length=10.0, tip_force=1e5, ramp=0.1,
-                           h=0.005, num_steps=200, rho_inf=0.9, max_iter=50, atol=1e-10, rtol=0.0):
+                           h=0.005, num_steps=200, rho_inf=0.9, max_iter=50, atol=1e-10, rtol=1e-6):
     """Straight cantilever along x: one element, GLL nodes, Gauss quadrature, clamped root,
     tip follower force in local z ramped over `ramp` seconds (re-oriented once per step
     from the start-of-step tip rotation)."""
@@ -328,35 +328,35 @@ def cantilever_config_spec(num_nodes=8, num_qps=11, length=10.0, tip_force=1e5, 
 
 
 def _spd_project(A: np.ndarray) -> np.ndarray:
-    A = 0.5 * (A + A.T)
+    """Symmetric part, eigenvalues clipped to >= 1e-6 * lambda_max (stacked [..., 6, 6])."""
+    A = 0.5 * (A + np.swapaxes(A, -1, -2))
     w, V = np.linalg.eigh(A)
-    w = np.maximum(w, 1e-6 * w.max())
-    B = (V * w) @ V.T
-    return 0.5 * (B + B.T)
+    w = np.maximum(w, 1e-6 * w.max(axis=-1, keepdims=True))
+    B = (V * w[..., None, :]) @ np.swapaxes(V, -1, -2)
+    return 0.5 * (B + np.swapaxes(B, -1, -2))
 
 
 def section_properties(seed: int, blade_key: int, num_stations: int = 21):
     """Tapered, noisy, coupled SPD sections: diagonal = base * 20^-s * exp(U(-ln1.1, ln1.1)),
-    off-diagonal = 0.05 * sqrt(C_ii C_jj) * U(-1, 1), projected to SPD."""
+    off-diagonal = 0.05 * sqrt(C_ii C_jj) * U(-1, 1), projected to SPD.
+    Random key of station i, matrix m (0 mass, 1 stiffness), entry k:
+    (blade_key * num_stations * 128) + i * 128 + m * 64 + k (k < 6 diagonal, 6..20 upper)."""
     s = np.linspace(0.0, 1.0, num_stations)
-    base_key = np.uint64(blade_key) * np.uint64(num_stations * 128)
-    Ms, Cs = [], []
     iu = np.triu_indices(6, 1)
-    for i, si in enumerate(s):
-        key0 = base_key + np.uint64(i * 128)
-        out = []
-        for mat_id, base in enumerate((C1_MASS, C1_STIFF)):
-            k = key0 + np.uint64(mat_id * 64)
-            noise = np.exp((2.0 * uniform01(seed, k + np.arange(6, dtype=np.uint64)) - 1.0) * math.log(1.1))
-            d = base * 20.0 ** (-si) * noise
-            A = np.diag(d)
-            off = 0.05 * np.sqrt(d[iu[0]] * d[iu[1]]) * (2.0 * uniform01(seed, k + np.uint64(6) + np.arange(15, dtype=np.uint64)) - 1.0)
-            A[iu] = off
-            A[(iu[1], iu[0])] = off
-            out.append(_spd_project(A).reshape(36))
-        Ms.append(out[0])
-        Cs.append(out[1])
-    return s, np.array(Ms), np.array(Cs)
+    base_key = np.uint64(blade_key) * np.uint64(num_stations * 128)
+    st = np.arange(num_stations, dtype=np.uint64) * np.uint64(128)
+    out = []
+    for mat_id, base in enumerate((C1_MASS, C1_STIFF)):
+        keys = base_key + st[:, None] + np.uint64(mat_id * 64) + np.arange(21, dtype=np.uint64)[None, :]
+        u = uniform01(seed, keys)                                        # [S, 21]
+        d = base[None, :] * (20.0 ** (-s))[:, None] * np.exp((2.0 * u[:, :6] - 1.0) * math.log(1.1))
+        A = np.zeros((num_stations, 6, 6))
+        A[:, np.arange(6), np.arange(6)] = d
+        off = 0.05 * np.sqrt(d[:, iu[0]] * d[:, iu[1]]) * (2.0 * u[:, 6:21] - 1.0)
+        A[:, iu[0], iu[1]] = off
+        A[:, iu[1], iu[0]] = off
+        out.append(_spd_project(A).reshape(num_stations, 36))
+    return s, out[0], out[1]
 
 
 def _blade_frame(azimuth: float) -> np.ndarray:
@@ -388,7 +388,7 @@ def _add_rotor(nt, beams, cons, roots_info, *, num_nodes_per_blade, length_per_b
 
 
 def rotor_config_spec(num_nodes=11, num_qps=20, length=60.0, omega=1.0, hub_radius=2.0, h=0.01,
-                      num_steps=500, rho_inf=0.9, max_iter=50, atol=1e-10, rtol=0.0, seed=SEED,
+                      num_steps=500, rho_inf=0.9, max_iter=50, atol=1e-10, rtol=1e-6, seed=SEED,
                       gravity=(0.0, 0.0, -9.81)):
     """3 blades, one element each, prescribed root motion of a hub spinning about x."""
     assert num_qps == 2 * num_nodes - 2 or num_qps > 0
@@ -446,7 +446,7 @@ def _rotor_spec(name, x0, u, v, vd, s, beams, cons, roots, gravity, h, num_steps
 
 
 def ensemble_config_specs(num_rotors=4096, h=0.01, num_steps=100, rho_inf=0.9, max_iter=50,
-                          atol=1e-10, rtol=0.0, seed=SEED, tip_load_sigma=1e4, hub_radius=2.0,
+                          atol=1e-10, rtol=1e-6, seed=SEED, tip_load_sigma=1e4, hub_radius=2.0,
                           gravity=(0.0, 0.0, -9.81), first_rotor=0):
     """The batched ensemble: independent 3-blade rotors, per-blade N uniform in
     {6, 11, 16, 21}, Q = 2N-2, L uniform in [30, 120] m, omega uniform in [0.5, 1.5] rad/s,
@@ -470,3 +470,43 @@ def ensemble_config_specs(num_rotors=4096, h=0.01, num_steps=100, rho_inf=0.9, m
         spec.meta["blade_nodes"] = Ns
         specs.append(spec)
     return specs
+
+
+def ensemble_schedule(specs, num_steps, first_step=0):
+    """Device-schedule arrays for a list of rotor specs advanced as one system (models in
+    order): constraint displacements [steps, total_constraints, 7], the loaded (tip) node ids
+    [total_tips] and their loads [steps, total_tips, 6].  Values equal the specs' own
+    step_inputs (prescribed rigid rotation of each root about x, normal tip loads)."""
+    node_off = np.cumsum([0] + [sp.num_nodes for sp in specs[:-1]])
+    roots, omegas, tips, seeds = [], [], [], []
+    for sp, off in zip(specs, node_off):
+        for c in sp.constraints:
+            roots.append(sp.x0[c.target, :3])
+        omegas.extend(sp.meta["omegas"].tolist())
+        tips.extend((off + sp.meta["tips"]).tolist())
+    roots = np.array(roots)
+    omegas = np.array(omegas)
+    steps = np.arange(first_step, first_step + num_steps)
+    t = (steps[:, None] * 0 + steps[:, None] + 1) * specs[0].h                     # [S,1]
+    ang = omegas[None, :] * t                                                     # [S,C]
+    a = np.abs(ang)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        f = np.where(a < 1e-12, 0.0, np.sin(a / 2.0) / np.where(a < 1e-12, 1.0, a))
+    q0 = np.where(a < 1e-12, 1.0, np.cos(a / 2.0))
+    q1 = ang * f
+    c2 = q0 * q0 - q1 * q1
+    cs = 2.0 * (0.0 - q0 * q1)
+    py, pz = roots[None, :, 1], roots[None, :, 2]
+    ux = (q0 * q0 + q1 * q1) * roots[None, :, 0] - roots[None, :, 0]
+    uy = (c2 * py + cs * pz) - py
+    uz = (-cs * py + c2 * pz) - pz
+    cdisp = np.stack([ux, uy, uz, q0, q1, np.zeros_like(q0), np.zeros_like(q0)], axis=-1)
+    loads = np.zeros((num_steps, len(tips), 6))
+    k = 0
+    for sp in specs:
+        nb = len(sp.meta["tips"])
+        for si, st in enumerate(steps):
+            f = sp.meta["tip_loads"](int(st))
+            loads[si, k:k + nb] = f[sp.meta["tips"]]
+        k += nb
+    return np.ascontiguousarray(cdisp), np.array(tips, dtype=np.uint64), loads

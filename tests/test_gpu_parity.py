@@ -162,3 +162,46 @@ def test_small_ensemble_fused_fp64_and_fp32(kyn, orc):
         e32 = rel(q32[sl], o.get("q"))
         print(f"rotor {k}: FP32-tangent final q rel error {e32:.2e}", flush=True)
         assert e32 < 1e-9
+
+
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_schedule_and_graph_replay_match_host_driven_steps(kyn, use_graph):
+    specs = W.ensemble_config_specs(num_rotors=3, num_steps=4)
+    a = ota.KynSystem(kyn, specs, exec_mode=ota.EXEC_FUSED)
+    for _ in range(4):
+        assert a.step() == 0
+    b = ota.KynSystem(kyn, specs, exec_mode=ota.EXEC_FUSED)
+    cd, tips, loads = W.ensemble_schedule(specs, 4)
+    b.set_schedule(4, cd, tips, loads)
+    assert b.step_many(4, use_graph=use_graph) == 0
+    it, fl = b.totals()
+    assert fl.sum() == 0 and it.min() >= 4
+    for f in STATE_FIELDS:
+        assert rel(b.get(f), a.get(f)) < 1e-13, f
+
+
+def test_save_restore_and_restart_image(kyn, orc):
+    spec = W.rotor_config_spec(num_steps=4)
+    g = ota.KynSystem(kyn, spec, exec_mode=ota.EXEC_FUSED)
+    g.step()
+    g.save_state()
+    img = g.write_restart()
+    g.step()
+    q_after = g.get("q")
+    g.restore_state()
+    g.step_index -= 1
+    g.step()
+    assert rel(g.get("q"), q_after) == 0.0
+    # restart image layout of WriteStateToFile: size_t n, then x0, x, q_delta, q_prev, q (x7 or x6) ...
+    n = np.frombuffer(img[:8], dtype=np.uint64)[0]
+    assert n == g.n
+    assert len(img) == 8 + g.n * (7 * 4 + 6 * 5) * 8
+    o = OracleSystem(orc, spec)
+    o.step(spec.step_inputs(0))
+    import ctypes as C
+
+    ref = (C.c_ubyte * len(img))()
+    orc.L.orc_write_restart(o.h, ref)
+    ref_arr = np.frombuffer(bytes(ref)[8:], dtype=np.float64)
+    img_arr = np.frombuffer(img[8:], dtype=np.float64)
+    assert np.max(np.abs(ref_arr - img_arr)) <= 1e-12 * np.max(np.abs(ref_arr))
