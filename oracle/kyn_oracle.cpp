@@ -1119,10 +1119,10 @@ void LUSolve(size_t n, const double* A, const int* piv, double* b) {
         for (size_t j = 0; j < i; ++j) acc -= A[i * n + j] * b[j];
         b[i] = acc;
     }
-    for (size_t ii = n; ii-- > 0;) {
-        double acc = b[ii];
-        for (size_t j = ii + 1; j < n; ++j) acc -= A[ii * n + j] * b[j];
-        b[ii] = acc / A[ii * n + ii];
+    // column-oriented back substitution (x_k final, then eliminated from the rows above)
+    for (size_t k = n; k-- > 0;) {
+        b[k] = b[k] / A[k * n + k];
+        for (size_t i = 0; i < k; ++i) b[i] -= A[i * n + k] * b[k];
     }
 }
 
