@@ -32,7 +32,7 @@ constexpr int kContractThreads = 252;  // 7 row groups x 36 components
 // ---------------------------------------------------------------------------------------
 struct Lay {
     // persistent
-    int nq, nv, nvd, tr, loc, rhs, xs, xs2, piv, red, cres, misc;
+    int nq, nv, nvd, tr, loc, rhs, xs, xs2, piv, red, cres, misc, rowbuf, rowbuf2, colbuf, candv, candi;
     // overlay: element view
     int phi, dphi, w, J, qpF, qpM, R;
     // overlay: matrix view
@@ -63,6 +63,11 @@ __host__ __device__ inline Lay make_layout(int Nm, int Qm, int nm, int tt_size, 
     L.red = take(16 * 8);
     L.cres = take(8 * 24 * 8);  // up to 8 constraints per block: res, lres_b, lres_t, sys_res
     L.misc = take(16 * 4);
+    L.rowbuf = take((nm + 1) * 8);
+    L.rowbuf2 = take((nm + 1) * 8);
+    L.colbuf = take((nm + 1) * 8);
+    L.candv = take(16 * 8);
+    L.candi = take(16 * 4);
     const int overlay = o;
     L.phi = take(Nm * Qm * 8);
     L.dphi = take(Nm * Qm * 8);
@@ -104,7 +109,7 @@ __device__ __forceinline__ T tabs(T v) { return v < T(0) ? -v : v; }
 //           (times the conditioner, at block-local dofs); otherwise into elem_res/elem_mat.
 // ---------------------------------------------------------------------------------------
 template <typename TT, bool kFused>
-__device__ void element_phase(const DevSys& S, int e, char* sm, const Lay& L, int n) {
+__device__ __forceinline__ void element_phase(const DevSys& S, int e, char* sm, const Lay& L, int n) {
     const int tid = threadIdx.x;
     const int N = S.elem_N[e], Q = S.elem_Q[e];
     const int noff = S.elem_node_off[e];
@@ -375,7 +380,7 @@ __device__ void element_phase(const DevSys& S, int e, char* sm, const Lay& L, in
 // |a_ik|, lowest index on ties.
 // ---------------------------------------------------------------------------------------
 template <typename TT>
-__device__ void lu_factor(TT* A, int n, int lda, int* piv, int* misc) {
+__device__ __forceinline__ void lu_factor(TT* A, int n, int lda, int* piv, int* misc) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int nt = blockDim.x;
     for (int k = 0; k < n; ++k) {
@@ -428,12 +433,200 @@ __device__ void lu_factor(TT* A, int n, int lda, int* piv, int* misc) {
     }
 }
 
+// Register-resident LU with partial pivoting on an augmented matrix [A | b].
+// 256 threads form a 16 x 16 grid; thread (ty, tx) owns rows ty + 16a and columns tx + 16b
+// (a, b < 9): a block-cyclic 9 x 9 register tile, n + 1 <= 144.  The right-hand side b is
+// column n, so forward elimination applies the row swaps and L^-1 to it on the way (same
+// operation sequence per entry as pivoting b first and then solving L y = P b column by
+// column).  Pivot: largest |a_ik|, lowest row index on ties.  Three barriers per column.
+// On return the LDS matrix holds L (unit, below the diagonal), U, and y = L^-1 P b in column n.
+template <typename TT>
+__device__ __forceinline__ void lu_factor_augmented(TT* A, int n, int lda, int* piv, TT* rowbuf, TT* rowbuf2, TT* colbuf,
+                                    TT* candv, int* candi) {
+    const int tid = threadIdx.x;
+    const int ty = tid >> 4, tx = tid & 15;
+    const int nc = n + 1;  // columns including the right-hand side
+    TT R[9][9];
+#pragma unroll
+    for (int a = 0; a < 9; ++a)
+#pragma unroll
+        for (int b = 0; b < 9; ++b) {
+            const int r = ty + 16 * a, c = tx + 16 * b;
+            R[a][b] = (r < n && c < nc) ? A[r * lda + c] : TT(0);
+        }
+    for (int k = 0; k < n; ++k) {
+        const int kb = k >> 4, kt = k & 15;
+        // 1. pivot candidates from the owners of column k
+        if (tx == kt) {
+            TT best = TT(-1);
+            int bi = n;
+#pragma unroll
+            for (int a = 0; a < 9; ++a) {
+                const int r = ty + 16 * a;
+                TT ck = R[a][0];
+#pragma unroll
+                for (int bb = 1; bb < 9; ++bb) ck = (kb == bb) ? R[a][bb] : ck;
+                if (r >= k && r < n) {
+                    const TT v = tabs(ck);
+                    if (v > best) {
+                        best = v;
+                        bi = r;
+                    }
+                }
+            }
+            candv[ty] = best;
+            candi[ty] = bi;
+        }
+        __syncthreads();
+        TT best = candv[0];
+        int p = candi[0];
+#pragma unroll
+        for (int t = 1; t < 16; ++t) {
+            const TT v = candv[t];
+            const int i = candi[t];
+            if (v > best || (v == best && i < p)) {
+                best = v;
+                p = i;
+            }
+        }
+        if (p >= n) p = k;  // column of NaNs: keep the row
+        if (tid == 0) piv[k] = p;
+        // 2. publish the pivot row (old row p) and, if swapping, old row k
+        const int pt = p & 15, pb = p >> 4;
+        if (ty == pt) {
+#pragma unroll
+            for (int a = 0; a < 9; ++a)
+                if (a == pb)
+#pragma unroll
+                    for (int b = 0; b < 9; ++b) {
+                        const int c = tx + 16 * b;
+                        if (c < nc) rowbuf[c] = R[a][b];
+                    }
+        }
+        if (p != k && ty == kt) {
+#pragma unroll
+            for (int a = 0; a < 9; ++a)
+                if (a == kb)
+#pragma unroll
+                    for (int b = 0; b < 9; ++b) {
+                        const int c = tx + 16 * b;
+                        if (c < nc) rowbuf2[c] = R[a][b];
+                    }
+        }
+        __syncthreads();
+        if (p != k) {
+            if (ty == kt) {
+#pragma unroll
+                for (int a = 0; a < 9; ++a)
+                    if (a == kb)
+#pragma unroll
+                        for (int b = 0; b < 9; ++b) {
+                            const int c = tx + 16 * b;
+                            if (c < nc) R[a][b] = rowbuf[c];
+                        }
+            }
+            if (ty == pt) {
+#pragma unroll
+                for (int a = 0; a < 9; ++a)
+                    if (a == pb)
+#pragma unroll
+                        for (int b = 0; b < 9; ++b) {
+                            const int c = tx + 16 * b;
+                            if (c < nc) R[a][b] = rowbuf2[c];
+                        }
+            }
+        }
+        // 3. multipliers l_i = a_ik / a_kk below the diagonal (owners of column k)
+        const TT d = rowbuf[k];
+        const bool live = d != TT(0);
+        if (tx == kt && live) {
+            const TT inv = TT(1) / d;
+#pragma unroll
+            for (int a = 0; a < 9; ++a) {
+                const int r = ty + 16 * a;
+                TT ck = R[a][0];
+#pragma unroll
+                for (int bb = 1; bb < 9; ++bb) ck = (kb == bb) ? R[a][bb] : ck;
+                if (r > k && r < n) {
+                    const TT l = ck * inv;
+#pragma unroll
+                    for (int bb = 0; bb < 9; ++bb) R[a][bb] = (kb == bb) ? l : R[a][bb];
+                    colbuf[r] = l;
+                }
+            }
+        }
+        __syncthreads();
+        // 4. rank-1 update of the trailing block (and of the right-hand side column)
+        if (live) {
+            TT u[9];
+#pragma unroll
+            for (int b = 0; b < 9; ++b) {
+                const int c = tx + 16 * b;
+                u[b] = (c > k && c < nc) ? rowbuf[c] : TT(0);
+            }
+#pragma unroll
+            for (int a = 0; a < 9; ++a) {
+                const int r = ty + 16 * a;
+                if (r > k && r < n) {
+                    const TT l = colbuf[r];
+#pragma unroll
+                    for (int b = 0; b < 9; ++b) {
+                        const int c = tx + 16 * b;
+                        if (c > k && c < nc) R[a][b] -= l * u[b];
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int a = 0; a < 9; ++a)
+#pragma unroll
+        for (int b = 0; b < 9; ++b) {
+            const int r = ty + 16 * a, c = tx + 16 * b;
+            if (r < n && c < nc) A[r * lda + c] = R[a][b];
+        }
+    __syncthreads();
+}
+
+// Back substitution U x = y with y in column n of the factored augmented matrix, FP64
+// accumulation, by wave 0 (column-oriented, the oracle's order).
+template <typename TT>
+__device__ __forceinline__ void back_substitute(const TT* A, int n, int lda, double* xs) {
+    const int tid = threadIdx.x;
+    if (tid < 64) {
+        const int lane = tid;
+        double r[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) r[a] = (lane + 64 * a) < n ? (double)A[(lane + 64 * a) * lda + n] : 0.;
+        for (int k = n - 1; k >= 0; --k) {
+            const int ka = k >> 6;
+            if (lane == (k & 63)) {
+                const double d = (double)A[k * lda + k];
+                if (ka == 0) r[0] = r[0] / d;
+                else if (ka == 1) r[1] = r[1] / d;
+                else r[2] = r[2] / d;
+            }
+            const double xk = __shfl(ka == 0 ? r[0] : (ka == 1 ? r[1] : r[2]), k & 63, 64);
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                const int i = lane + 64 * a;
+                if (i < k) r[a] -= (double)A[i * lda + k] * xk;
+            }
+        }
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+            if (lane + 64 * a < n) xs[lane + 64 * a] = r[a];
+    }
+    __syncthreads();
+}
+
 __device__ __forceinline__ double sel3(const double* r, int a) { return a == 0 ? r[0] : (a == 1 ? r[1] : r[2]); }
 
 // Triangular solves with the LDS factors, FP64 accumulation, executed by wave 0
 // (lane owns rows lane, lane+64, lane+128; n <= 192).  b: LDS, pivoted here; x -> xs.
 template <typename TT>
-__device__ void lu_solve(const TT* A, int n, int lda, const int* piv, double* b, double* xs) {
+__device__ __forceinline__ void lu_solve(const TT* A, int n, int lda, const int* piv, double* b, double* xs) {
     const int tid = threadIdx.x;
     if (tid == 0)
         for (int k = 0; k < n; ++k) {
@@ -603,7 +796,7 @@ __global__ void k_advance_slot(DevSys S) {
 // Block phase helpers (fused form): constraints into the LDS KKT block
 // ---------------------------------------------------------------------------------------
 template <typename TT>
-__device__ void block_constraints(const DevSys& S, int b, char* sm, const Lay& L) {
+__device__ __forceinline__ void block_constraints(const DevSys& S, int b, char* sm, const Lay& L) {
     TT* A = reinterpret_cast<TT*>(sm + L.A);
     double* cres = reinterpret_cast<double*>(sm + L.cres);
     const int c0 = S.block_con_begin[b], c1 = S.block_con_end[b];
@@ -711,7 +904,7 @@ __device__ void block_constraints(const DevSys& S, int b, char* sm, const Lay& L
 
 // Convergence-error terms and state update of one block after its solve
 // (calculate_convergence_error.hpp, update_state_prediction.hpp, update_lambda_prediction.hpp)
-__device__ double block_error_and_update(const DevSys& S, int b, const double* xs, double* red) {
+__device__ __forceinline__ double block_error_and_update(const DevSys& S, int b, const double* xs, double* red) {
     const StepConsts& p = S.p;
     const int n = S.block_n[b], nsys = S.block_nsys[b];
     const int nb0 = S.block_node_begin[b];
@@ -776,27 +969,39 @@ __device__ __forceinline__ void condition_rhs(const DevSys& S, int b, double* rh
     __syncthreads();
 }
 
-// Solve + optional FP64 iterative refinement against the unfactored matrix (FP32 tangent),
-// then UnconditionSolution.  A: LDS factors; Ac: unfactored copy in HBM (row-major, lda).
+// Factor + solve of one block (replaces dss::numeric_factorization / dss::solve,
+// src/step/solve_system.hpp:27-31): the conditioned rhs rides along as column n of the
+// register LU, then back substitution; optional FP64 iterative refinement against the
+// unfactored matrix (FP32 tangent; copy in HBM); then UnconditionSolution
+// (condition_system.hpp:24-31).  A: LDS matrix with lda >= n + 1.
 template <typename TT>
-__device__ void solve_refine_uncondition(const DevSys& S, int b, TT* A, const TT* Ac, int lda,
-                                         int* piv, double* rhs, double* xs, double* xs2) {
+__device__ __forceinline__ void factor_solve_uncondition(const DevSys& S, int b, TT* A, TT* Ac, int lda, int* piv,
+                                         double* rhs, double* xs, double* xs2, char* sm, const Lay& L) {
     const int n = S.block_n[b], nsys = S.block_nsys[b], off = S.block_vec_off[b];
-    lu_solve<TT>(A, n, lda, piv, rhs, xs);
+    const int t = threadIdx.x;
+    if (Ac != nullptr) {
+        for (int k = t; k < n * n; k += blockDim.x) Ac[k] = A[(k / n) * lda + (k % n)];
+    }
+    for (int i = t; i < n; i += blockDim.x) A[i * lda + n] = (TT)rhs[i];
+    __syncthreads();
+    lu_factor_augmented<TT>(A, n, lda, piv, reinterpret_cast<TT*>(sm + L.rowbuf), reinterpret_cast<TT*>(sm + L.rowbuf2),
+                            reinterpret_cast<TT*>(sm + L.colbuf), reinterpret_cast<TT*>(sm + L.candv),
+                            reinterpret_cast<int*>(sm + L.candi));
+    back_substitute<TT>(A, n, lda, xs);
     for (int sweep = 0; Ac != nullptr && sweep < S.refine_sweeps; ++sweep) {
-        for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        for (int i = t; i < n; i += blockDim.x) {
             double r = S.rhs[off + i];
-            const TT* row = Ac + (size_t)i * lda;
+            const TT* row = Ac + (size_t)i * n;
             for (int j = 0; j < n; ++j) r -= (double)row[j] * xs[j];
             rhs[i] = r;
         }
         __syncthreads();
         lu_solve<TT>(A, n, lda, piv, rhs, xs2);
-        for (int i = threadIdx.x; i < n; i += blockDim.x) xs[i] += xs2[i];
+        for (int i = t; i < n; i += blockDim.x) xs[i] += xs2[i];
         __syncthreads();
     }
     const double cond = S.p.conditioner;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    for (int i = t; i < n; i += blockDim.x) {
         if (i >= nsys) xs[i] /= cond;
         S.xsol[off + i] = xs[i];
     }
@@ -845,14 +1050,10 @@ __global__ void __launch_bounds__(kThreads) k_fused_step(DevSys S, int Nm, int Q
             element_phase<TT, true>(S, e, sm, L, n);
             block_constraints<TT>(S, b, sm, L);
             condition_rhs(S, b, rhs);
-            TT* Ac = nullptr;
-            if (S.Acopy != nullptr && S.refine_sweeps > 0) {
-                Ac = reinterpret_cast<TT*>(S.Acopy) + S.block_A_off[b];
-                for (int k = t; k < n * L.lda; k += blockDim.x) Ac[k] = A[k];
-                __syncthreads();
-            }
-            lu_factor<TT>(A, n, L.lda, piv, misc);
-            solve_refine_uncondition<TT>(S, b, A, Ac, L.lda, piv, rhs, xs, xs2);
+            // unfactored copy for refinement: compact rows (stride n) at the block's own offset
+            TT* Ac = (S.Acopy != nullptr && S.refine_sweeps > 0) ? reinterpret_cast<TT*>(S.Acopy) + S.block_A_off[b]
+                                                                 : nullptr;
+            factor_solve_uncondition<TT>(S, b, A, Ac, L.lda, piv, rhs, xs, xs2, sm, L);
             err_sum += block_error_and_update(S, b, xs, red);
         }
         const double err = sqrt(err_sum / ndofs);
@@ -1078,22 +1279,21 @@ __global__ void __launch_bounds__(kThreads) k_plain_assemble(DevSys S) {
 // SolveSystem (src/step/solve_system.hpp:14-39): LU + solve per block in LDS
 __global__ void __launch_bounds__(kThreads) k_plain_solve(DevSys S, int nm) {
     extern __shared__ __align__(16) char sm[];
+    const Lay L = make_layout(1, 1, nm, 8, true);
     const int b = blockIdx.x, t = threadIdx.x;
     const int m = S.block_model[b];
     if (!S.model_active[m]) return;
     const int n = S.block_n[b];
-    const int lda = nm + 1;
-    double* A = reinterpret_cast<double*>(sm);
-    double* rhs = A + (size_t)nm * lda;
-    double* xs = rhs + nm;
-    int* piv = reinterpret_cast<int*>(xs + nm);
-    int* misc = piv + nm;
+    double* A = reinterpret_cast<double*>(sm + L.A);
+    double* rhs = reinterpret_cast<double*>(sm + L.rhs);
+    double* xs = reinterpret_cast<double*>(sm + L.xs);
+    double* xs2 = reinterpret_cast<double*>(sm + L.xs2);
+    int* piv = reinterpret_cast<int*>(sm + L.piv);
     const double* Ag = reinterpret_cast<const double*>(S.A) + S.block_A_off[b];
-    for (int k = t; k < n * n; k += blockDim.x) A[(k / n) * lda + (k % n)] = Ag[k];
+    for (int k = t; k < n * n; k += blockDim.x) A[(k / n) * L.lda + (k % n)] = Ag[k];
     for (int i = t; i < n; i += blockDim.x) rhs[i] = S.rhs[S.block_vec_off[b] + i];
     __syncthreads();
-    lu_factor<double>(A, n, lda, piv, misc);
-    solve_refine_uncondition<double>(S, b, A, nullptr, lda, piv, rhs, xs, xs);
+    factor_solve_uncondition<double>(S, b, A, nullptr, L.lda, piv, rhs, xs, xs2, sm, L);
 }
 
 // CalculateConvergenceError + UpdateStatePrediction + UpdateConstraintPrediction
@@ -1139,7 +1339,7 @@ size_t fused_lds_bytes(int Nm, int Qm, int nm, int precision) {
     return (size_t)make_layout(Nm, Qm, nm, precision ? 4 : 8, true).total;
 }
 size_t element_lds_bytes(int Nm, int Qm) { return (size_t)make_layout(Nm, Qm, 1, 8, false).total; }
-size_t solve_lds_bytes(int nm) { return (size_t)nm * (nm + 1) * 8 + (size_t)nm * 16 + (size_t)nm * 4 + 64; }
+size_t solve_lds_bytes(int nm) { return (size_t)make_layout(1, 1, nm, 8, true).total; }
 
 hipError_t set_lds_limits(size_t bytes) {
     hipError_t e;
