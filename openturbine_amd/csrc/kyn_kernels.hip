@@ -1058,8 +1058,12 @@ __global__ void __launch_bounds__(kThreads) k_fused_step(DevSys S, int Nm, int Q
         }
         const double err = sqrt(err_sum / ndofs);
         if (t == 0) S.err_hist[(size_t)m * p.max_iter + iter] = err;
-        if (!(err > 1.)) {
+        if (err <= 1.) {
             converged = 1;
+            ++iter;
+            break;
+        }
+        if (isnan(err)) {  // the reference's `err > 1` loop test would end here as converged
             ++iter;
             break;
         }
@@ -1308,10 +1312,10 @@ __global__ void __launch_bounds__(kThreads) k_plain_update(DevSys S, int iter) {
     if (t == 0) {
         S.err_hist[(size_t)m * S.p.max_iter + iter] = err;
         S.model_iters[m] = iter + 1;
-        if (!(err > 1.)) {
+        if (err <= 1.) {
             S.model_status[m] = 0;
             S.model_active[m] = 0;
-        } else if (iter + 1 >= S.p.max_iter) {
+        } else if (iter + 1 >= S.p.max_iter || isnan(err)) {
             S.model_status[m] = 1;
             S.model_active[m] = 0;
         } else {

@@ -336,17 +336,22 @@ def _spd_project(A: np.ndarray) -> np.ndarray:
     return 0.5 * (B + np.swapaxes(B, -1, -2))
 
 
-def section_properties(seed: int, blade_key: int, num_stations: int = 21):
+SCALE_EXPONENTS = np.array([2.0, 2.0, 2.0, 4.0, 4.0, 4.0])
+
+
+def section_properties(seed: int, blade_key: int, num_stations: int = 21, size_factor: float = 1.0):
     """Tapered, noisy, coupled SPD sections: diagonal = base * 20^-s * exp(U(-ln1.1, ln1.1)),
     off-diagonal = 0.05 * sqrt(C_ii C_jj) * U(-1, 1), projected to SPD.
     Random key of station i, matrix m (0 mass, 1 stiffness), entry k:
-    (blade_key * num_stations * 128) + i * 128 + m * 64 + k (k < 6 diagonal, 6..20 upper)."""
+    (blade_key * num_stations * 128) + i * 128 + m * 64 + k (k < 6 diagonal, 6..20 upper).
+    size_factor s scales the base section geometrically: mass / axial / shear entries by s^2,
+    rotary inertias / torsion / bending by s^4."""
     s = np.linspace(0.0, 1.0, num_stations)
     iu = np.triu_indices(6, 1)
     base_key = np.uint64(blade_key) * np.uint64(num_stations * 128)
     st = np.arange(num_stations, dtype=np.uint64) * np.uint64(128)
     out = []
-    for mat_id, base in enumerate((C1_MASS, C1_STIFF)):
+    for mat_id, base in enumerate((C1_MASS * size_factor**SCALE_EXPONENTS, C1_STIFF * size_factor**SCALE_EXPONENTS)):
         keys = base_key + st[:, None] + np.uint64(mat_id * 64) + np.arange(21, dtype=np.uint64)[None, :]
         u = uniform01(seed, keys)                                        # [S, 21]
         d = base[None, :] * (20.0 ** (-s))[:, None] * np.exp((2.0 * u[:, :6] - 1.0) * math.log(1.1))
@@ -365,7 +370,7 @@ def _blade_frame(azimuth: float) -> np.ndarray:
 
 
 def _add_rotor(nt, beams, cons, roots_info, *, num_nodes_per_blade, length_per_blade, omega,
-               hub_radius, sec_seed, blade_keys):
+               hub_radius, sec_seed, blade_keys, length_scaled_stiffness=False):
     """Append one 3-blade rotor (hub axis x, blades in the y-z plane)."""
     w = np.array([omega, 0.0, 0.0])
     for b in range(3):
@@ -381,7 +386,9 @@ def _add_rotor(nt, beams, cons, roots_info, *, num_nodes_per_blade, length_per_b
             p = (hub_radius + sl * L) * d
             ids.append(nt.add(np.concatenate([p, qb]), v=np.concatenate([np.cross(w, p), w]), s=sl))
         gx, gw = gauss_legendre(2 * N - 2)
-        ss, Ms, Cs = section_properties(sec_seed, blade_keys[b])
+        # optional geometric scaling of the 10 m base section to the blade length
+        ss, Ms, Cs = section_properties(sec_seed, blade_keys[b],
+                                        size_factor=(L / 10.0) if length_scaled_stiffness else 1.0)
         beams.append(BeamSpec(np.array(ids), ss, Ms, Cs, np.stack([gx, gw], axis=1)))
         cons.append(ConstraintSpec(PRESCRIBED_BC, 0, ids[0]))
         roots_info.append((len(cons) - 1, ids[0], ids[-1], omega))
@@ -450,7 +457,11 @@ def ensemble_config_specs(num_rotors=4096, h=0.01, num_steps=100, rho_inf=0.9, m
                           gravity=(0.0, 0.0, -9.81), first_rotor=0):
     """The batched ensemble: independent 3-blade rotors, per-blade N uniform in
     {6, 11, 16, 21}, Q = 2N-2, L uniform in [30, 120] m, omega uniform in [0.5, 1.5] rad/s,
-    per-step tip loads normal(0, tip_load_sigma) N.  Returns one ModelSpec per rotor."""
+    per-step tip loads normal(0, tip_load_sigma) N.  Sections are the config-1 base section
+    scaled geometrically to the blade (s = L / 10 m: mass, EA, GA x s^2; rotary inertia, GJ,
+    EI x s^4): with unscaled base sections a 120 m blade sags ~127 m under gravity and the
+    reference's Newton iteration diverges, and the random tip loads drive the light tips
+    unstable.  Returns one ModelSpec per rotor."""
     specs = []
     for r in range(first_rotor, first_rotor + num_rotors):
         keys = np.uint64(r) * np.uint64(8) + np.arange(8, dtype=np.uint64)
@@ -461,7 +472,8 @@ def ensemble_config_specs(num_rotors=4096, h=0.01, num_steps=100, rho_inf=0.9, m
         nt = _NodeTable()
         beams, cons, roots = [], [], []
         _add_rotor(nt, beams, cons, roots, num_nodes_per_blade=Ns, length_per_blade=Ls, omega=om,
-                   hub_radius=hub_radius, sec_seed=seed, blade_keys=[3 * r, 3 * r + 1, 3 * r + 2])
+                   hub_radius=hub_radius, sec_seed=seed, blade_keys=[3 * r, 3 * r + 1, 3 * r + 2],
+                   length_scaled_stiffness=True)
         x0, uu, v, vd, s = nt.arrays()
         spec = _rotor_spec(f"ensemble_rotor_{r}", x0, uu, v, vd, s, beams, cons, roots,
                            np.array(gravity), h, num_steps, rho_inf, max_iter, atol, rtol,
