@@ -88,6 +88,21 @@ def run_leg(lib, specs, sched, *, exec_mode, precision, warmup, steps, refine=0,
             "iters": (it1 - it0), "fails": int(fl.sum())}
 
 
+def copy_calibration(nbytes):
+    """One device-to-device copy of nbytes (torch's elementwise copy kernel): the reference
+    reading for the FETCH_SIZE / WRITE_SIZE counters (KiB per dispatch) that scale the step
+    kernel's HBM traffic."""
+    import torch
+
+    a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": nbytes, "read_KiB": nbytes / 1024, "write_KiB": nbytes / 1024}),
+          flush=True)
+
+
 def cpu_baseline(specs_sample, steps):
     sys.path.insert(0, str(ROOT / "tests"))
     from oracle_lib import OracleLib, OracleSystem, step_many
@@ -115,7 +130,13 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-plain", action="store_true")
+    ap.add_argument("--skip-fp64", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
+                    help="only run one device copy of BYTES (counter calibration) and exit")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        copy_calibration(args.calibrate_copy)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); the ensemble "
               "shards by rotor with no per-iteration exchange, see DESIGN.md", file=sys.stderr)
@@ -147,11 +168,13 @@ def main():
     it_total, _ = sysm.totals()
     del r32["system"]
 
-    log("leg: fused FP64")
-    r64 = run_leg(lib, specs, sched, exec_mode=ota.EXEC_FUSED, precision=ota.TANGENT_FP64,
-                  warmup=args.warmup, steps=args.steps)
-    log(f"  {args.steps} steps in {r64['wall_s']:.3f}s, fails {r64['fails']}")
-    del r64["system"]
+    r64 = None
+    if not args.skip_fp64:
+        log("leg: fused FP64")
+        r64 = run_leg(lib, specs, sched, exec_mode=ota.EXEC_FUSED, precision=ota.TANGENT_FP64,
+                      warmup=args.warmup, steps=args.steps)
+        log(f"  {args.steps} steps in {r64['wall_s']:.3f}s, fails {r64['fails']}")
+        del r64["system"]
 
     rp = None
     if not args.skip_plain:
@@ -185,12 +208,12 @@ def main():
         "config": {"workload": f"ensemble of {args.rotors} independent 3-blade rotors, N in {{6,11,16,21}}, "
                                "Q=2N-2, h=0.01, fused one-kernel-per-step, hipGraph",
                    "models": args.rotors, "blades": 3 * args.rotors},
-        "fused_fp64_steps_per_s": round(args.steps / r64["wall_s"], 4),
+        "fused_fp64_steps_per_s": round(args.steps / r64["wall_s"], 4) if r64 else None,
         "plain_fp64_steps_per_s": round(args.plain_steps / rp["wall_s"], 4) if rp else None,
         "speedup_fp32_fused_vs_plain_fp64": round(value / (args.plain_steps / rp["wall_s"]), 2) if rp else None,
         "newton_iters_per_model_step": {"fp32": round(float(r32["iters"].mean()) / args.steps, 3),
-                                         "fp64": round(float(r64["iters"].mean()) / args.steps, 3)},
-        "failed_model_steps": {"fp32": r32["fails"], "fp64": r64["fails"]},
+                                         "fp64": round(float(r64["iters"].mean()) / args.steps, 3) if r64 else None},
+        "failed_model_steps": {"fp32": r32["fails"], "fp64": r64["fails"] if r64 else None},
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": 157.3, "unit": "TFLOP/s",
                      "frac": round(achieved / 157.3, 5), "traffic": None,
                      "kernel": "k_fused_step<float>", "kernel_ms": round(kern_s * 1000.0, 4),

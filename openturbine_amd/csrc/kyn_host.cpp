@@ -695,7 +695,9 @@ kyn_system kyn_system_create(const kyn_model* models, size_t num_models, const k
         for (int N : B.elem_N) Nm = std::max(Nm, N);
         for (int Q : B.elem_Q) Qm = std::max(Qm, Q);
         for (int n : B.block_n) nm = std::max(nm, n);
-        if (nm > 192) throw KynError(KYN_ERR_UNSUPPORTED, "independent KKT block larger than 192 dofs");
+        if (nm > kyn::kMaxBlockDofs)
+            throw KynError(KYN_ERR_UNSUPPORTED, "independent KKT block larger than " + std::to_string(kyn::kMaxBlockDofs) +
+                                                    " dofs");
         sys->Nm = Nm;
         sys->Qm = Qm;
         sys->nm = nm;

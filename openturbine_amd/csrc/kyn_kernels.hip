@@ -24,7 +24,14 @@
 
 namespace kyn {
 
+// Optional per-phase timestamps for profiling builds; expands to nothing in the library.
+#ifndef KYN_PHASE_MARK
+#define KYN_PHASE_MARK(i)
+#endif
+
 constexpr int kQpChunk = 2;
+constexpr int kLuTile = 144;  // row bookkeeping of the block LU (n <= kMaxBlockDofs < kLuTile)
+static_assert(kMaxBlockDofs < kLuTile && kMaxBlockDofs < 192, "block LU: 3 rows per panel lane, 8-bit row keys");
 constexpr int kContractThreads = 252;  // 7 row groups x 36 components
 
 // ---------------------------------------------------------------------------------------
@@ -32,7 +39,7 @@ constexpr int kContractThreads = 252;  // 7 row groups x 36 components
 // ---------------------------------------------------------------------------------------
 struct Lay {
     // persistent
-    int nq, nv, nvd, tr, loc, rhs, xs, xs2, piv, red, cres, misc, rowbuf, rowbuf2, colbuf, candv, candi;
+    int nq, nv, nvd, tr, loc, rhs, xs, xs2, piv, red, cres, misc, lg_of, phys_of;
     // overlay: element view
     int phi, dphi, w, J, qpF, qpM, R;
     // overlay: matrix view
@@ -63,11 +70,8 @@ __host__ __device__ inline Lay make_layout(int Nm, int Qm, int nm, int tt_size, 
     L.red = take(16 * 8);
     L.cres = take(8 * 24 * 8);  // up to 8 constraints per block: res, lres_b, lres_t, sys_res
     L.misc = take(16 * 4);
-    L.rowbuf = take((nm + 1) * 8);
-    L.rowbuf2 = take((nm + 1) * 8);
-    L.colbuf = take((nm + 1) * 8);
-    L.candv = take(16 * 8);
-    L.candi = take(16 * 4);
+    L.lg_of = take(kLuTile * 4);
+    L.phys_of = take(kLuTile * 4);
     const int overlay = o;
     L.phi = take(Nm * Qm * 8);
     L.dphi = take(Nm * Qm * 8);
@@ -75,9 +79,9 @@ __host__ __device__ inline Lay make_layout(int Nm, int Qm, int nm, int tt_size, 
     L.J = take(Qm * 8);
     L.qpF = take(Qm * 24 * 8);
     L.qpM = take(Qm * 6 * 36 * tt_size);
-    L.R = take(kQpChunk * 2 * Nm * 36 * tt_size);
+    L.R = take(2 * kQpChunk * 2 * Nm * 36 * tt_size);  // two chunk buffers
     const int elem_end = o;
-    L.lda = nm + 1;
+    L.lda = (nm + 1) | 1;  // [A | b]; odd row stride: row-per-lane accesses hit distinct banks
     L.A = overlay;
     const int mat_end = with_matrix ? align16(overlay + nm * L.lda * tt_size) : overlay;
     L.total = elem_end > mat_end ? elem_end : mat_end;
@@ -162,6 +166,7 @@ __device__ __forceinline__ void element_phase(const DevSys& S, int e, char* sm, 
         Jq[q] = S.qp_jac[qoff + q];
     }
     __syncthreads();
+    KYN_PHASE_MARK(1);
 
     // ---- QP values: two lanes per QP (inertial | stiffness)
     for (int t = tid; t < 2 * Q; t += blockDim.x) {
@@ -238,6 +243,7 @@ __device__ __forceinline__ void element_phase(const DevSys& S, int e, char* sm, 
         }
     }
     __syncthreads();
+    KYN_PHASE_MARK(2);
 
     // ---- residual (integrate_residual_vector.hpp:32-50)
     for (int t = tid; t < 6 * N; t += blockDim.x) {
@@ -259,7 +265,51 @@ __device__ __forceinline__ void element_phase(const DevSys& S, int e, char* sm, 
         }
     }
 
+    KYN_PHASE_MARK(3);
     // ---- contraction over QP chunks
+    // R rows of one QP chunk are built into one of two LDS buffers while the chunk before is
+    // contracted from the other, so each chunk costs one barrier.
+    // Build roles: thread t < 216 owns (k, r, s) = (t % 72) / 36, (t % 36) / 6, t % 6 and nodes
+    // j = t / 72 + 3 jj.  Contraction roles: thread t < 252 owns component cc = t % 36 and rows
+    // i = t / 36 + 7 a for all j (a 3 x 21 register tile).
+    const int bkc = tid % 72;
+    const int bk = bkc / 36;
+    const int bc = bkc - 36 * bk;
+    const int br = bc / 6, bs = bc - 6 * (bc / 6);
+    const int bj0 = tid / 72;
+    const bool builder = tid < 216;
+    const bool rot = bs >= 3;
+    const int bcol = rot ? 3 : 0;  // first column of the 3 combined with T_j's column (s - 3)
+    // M6 layout per QP: A1 = wJ Kuu, A2 = wJ Quu, Mi = wJ(beta' Muu + gamma' Guu), P, O, C
+    const int offa = bk ? 144 : 0;    // A1 | O
+    const int offp = bk ? 180 : 108;  // P  | C
+    const TT a2on = bk ? TT(0) : TT(1);
+    const TT mon = bk ? TT(0) : TT(1);
+    auto build_chunk = [&](int q0, int qn, TT* Rc) {
+        if (!builder) return;
+        for (int qq = 0; qq < qn; ++qq) {
+            const int q = q0 + qq;
+            const TT* M6 = qpM + (size_t)q * 6 * 36;
+            TT ra[3], rp[3];
+#pragma unroll
+            for (int u = 0; u < 3; ++u) {
+                ra[u] = M6[offa + br * 6 + bcol + u] + a2on * M6[36 + br * 6 + bcol + u];
+                rp[u] = M6[offp + br * 6 + bcol + u];
+            }
+            const TT rm = mon * M6[72 + br * 6 + bs];
+            TT* out = Rc + (size_t)(qq * 2 + bk) * N * 36 + bc;
+            for (int j = bj0; j < N; j += 3) {
+                const TT pj = (TT)phi[j * Q + q], dj = (TT)dphi[j * Q + q];
+                TT tc[3];
+#pragma unroll
+                for (int u = 0; u < 3; ++u) tc[u] = rot ? (TT)tr[j * 9 + u * 3 + (bs - 3)] : (u == bs ? TT(1) : TT(0));
+                TT val = pj * rm;
+#pragma unroll
+                for (int u = 0; u < 3; ++u) val += (pj * ra[u] + dj * rp[u]) * tc[u];
+                out[j * 36] = val;
+            }
+        }
+    };
     const int ig = tid / 36;
     const int cc = tid - 36 * (tid / 36);
     const bool active = tid < kContractThreads;
@@ -269,50 +319,14 @@ __device__ __forceinline__ void element_phase(const DevSys& S, int e, char* sm, 
     for (int a = 0; a < 3; ++a)
 #pragma unroll
         for (int j = 0; j < kMaxElemNodes; ++j) acc[a][j] = TT(0);
-
-    for (int q0 = 0; q0 < Q; q0 += kQpChunk) {
+    const size_t rbuf = (size_t)kQpChunk * 2 * N * 36;
+    build_chunk(0, Q < kQpChunk ? Q : kQpChunk, Rb);
+    __syncthreads();
+    for (int q0 = 0, ch = 0; q0 < Q; q0 += kQpChunk, ++ch) {
         const int qn = (Q - q0) < kQpChunk ? (Q - q0) : kQpChunk;
-        // build R chunk: items (qq, k, j, c)
-        const int items = qn * 2 * N * 36;
-        for (int t = tid; t < items; t += blockDim.x) {
-            const int c = t % 36;
-            const int j = (t / 36) % N;
-            const int k = (t / (36 * N)) & 1;
-            const int qq = t / (72 * N);
-            const int q = q0 + qq;
-            const int r = c / 6, s = c - 6 * (c / 6);
-            const TT* M6 = qpM + (size_t)q * 6 * 36;
-            const TT pj = (TT)phi[j * Q + q], dj = (TT)dphi[j * Q + q];
-            TT val;
-            if (k == 0) {
-                // (phi_j (A1+A2) + phi'_j P) T_j + phi_j Mi
-                if (s < 3) {
-                    val = pj * (M6[r * 6 + s] + M6[36 + r * 6 + s]) + dj * M6[108 + r * 6 + s];
-                } else {
-                    val = TT(0);
-#pragma unroll
-                    for (int u = 0; u < 3; ++u) {
-                        const TT x = pj * (M6[r * 6 + 3 + u] + M6[36 + r * 6 + 3 + u]) + dj * M6[108 + r * 6 + 3 + u];
-                        val += x * (TT)tr[j * 9 + u * 3 + (s - 3)];
-                    }
-                }
-                val += pj * M6[72 + r * 6 + s];
-            } else {
-                // (phi_j O + phi'_j C) T_j
-                if (s < 3) {
-                    val = pj * M6[144 + r * 6 + s] + dj * M6[180 + r * 6 + s];
-                } else {
-                    val = TT(0);
-#pragma unroll
-                    for (int u = 0; u < 3; ++u) {
-                        const TT x = pj * M6[144 + r * 6 + 3 + u] + dj * M6[180 + r * 6 + 3 + u];
-                        val += x * (TT)tr[j * 9 + u * 3 + (s - 3)];
-                    }
-                }
-            }
-            Rb[((qq * 2 + k) * N + j) * 36 + c] = val;
-        }
-        __syncthreads();
+        const int q1 = q0 + kQpChunk;
+        if (q1 < Q) build_chunk(q1, (Q - q1) < kQpChunk ? (Q - q1) : kQpChunk, Rb + ((ch + 1) & 1) * rbuf);
+        const TT* Rc = Rb + (ch & 1) * rbuf;
         if (active) {
             for (int qq = 0; qq < qn; ++qq) {
                 const int q = q0 + qq;
@@ -323,8 +337,8 @@ __device__ __forceinline__ void element_phase(const DevSys& S, int e, char* sm, 
                     pi[a] = i < N ? (TT)phi[i * Q + q] : TT(0);
                     di[a] = i < N ? (TT)dphi[i * Q + q] : TT(0);
                 }
-                const TT* R0 = Rb + (size_t)(qq * 2 + 0) * N * 36 + cc;
-                const TT* R1 = Rb + (size_t)(qq * 2 + 1) * N * 36 + cc;
+                const TT* R0 = Rc + (size_t)(qq * 2 + 0) * N * 36 + cc;
+                const TT* R1 = Rc + (size_t)(qq * 2 + 1) * N * 36 + cc;
 #pragma unroll
                 for (int j = 0; j < kMaxElemNodes; ++j) {
                     if (j < N) {
@@ -338,6 +352,7 @@ __device__ __forceinline__ void element_phase(const DevSys& S, int e, char* sm, 
         __syncthreads();
     }
 
+    KYN_PHASE_MARK(4);
     // ---- write out
     if (kFused) {
         // the overlay is free after the barrier above: zero the matrix view, then write
@@ -375,245 +390,419 @@ __device__ __forceinline__ void element_phase(const DevSys& S, int e, char* sm, 
 }
 
 // ---------------------------------------------------------------------------------------
-// LU with partial pivoting on an LDS matrix (replaces dss::numeric_factorization/solve,
-// src/step/solve_system.hpp:27-31).  Row-major, leading dimension lda.  Pivot: largest
-// |a_ik|, lowest index on ties.
+// Blocked LU with partial pivoting on an augmented LDS matrix [A | b] (replaces
+// dss::numeric_factorization / dss::solve, src/step/solve_system.hpp:27-31).
+//
+// Right-looking, panels of 16 columns.  Wave 0 factors a panel in registers (lane l holds
+// physical rows l, l + 64, l + 128): per column a branch-free candidate per lane, a 64-lane
+// argmax by DPP row exchanges and permlane swaps, the pivot row broadcast by readlane, then
+// the in-panel rank-1 update; no barrier inside a panel.  All waves then form the panel's U
+// rows over the trailing columns (forward substitution with the panel's unit L) and update the
+// remaining rows, one column per lane pair.  Every entry sees the same FMA sequence as in the
+// oracle's unblocked loop (a_ij -= l_ik u_kj for k ascending), so results are bit-identical.
+//
+// Pivoting is implicit: rows never move; lg_of/phys_of map physical <-> logical rows (the
+// position the oracle's dense LU holds a row at after its swaps).  Pivot: largest |a_ik| over
+// logical rows i >= k, lowest logical index on ties, row k kept when a_kk is NaN.  The right-
+// hand side b is column n, so elimination applies P and L^-1 to it on the way.
+// On return: L (unit) and U at physical rows, y = L^-1 P b in column n, piv as the oracle's.
 // ---------------------------------------------------------------------------------------
+constexpr int kPanel = 16;
+
+struct LuScratch {
+    int* lg_of;    // logical row of each physical row (kLuTile)
+    int* phys_of;  // physical row of each logical row (kLuTile)
+};
+
+constexpr int kDppRowMirror = 0x140, kDppRowHalfMirror = 0x141, kDppQuad3210 = 0x1B, kDppQuad1032 = 0xB1;
+
+template <int C>
+__device__ __forceinline__ int dpp_x(int v) {
+    return __builtin_amdgcn_update_dpp(0, v, C, 0xF, 0xF, false);
+}
+// lane ^ 16 / lane ^ 32 (gfx950 permlane swaps: rows 2i+1 of the first operand trade with
+// rows 2i of the second, resp. rows 2,3 with rows 0,1)
+__device__ __forceinline__ int xor16_x(int v) {
+    const auto r = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+    return ((threadIdx.x >> 4) & 1) ? (int)r[0] : (int)r[1];
+}
+__device__ __forceinline__ int xor32_x(int v) {
+    const auto r = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+    return ((threadIdx.x >> 5) & 1) ? (int)r[0] : (int)r[1];
+}
+template <int C>
+__device__ __forceinline__ int lane_x(int v) {
+    if constexpr (C == 16) return xor16_x(v);
+    else if constexpr (C == 32) return xor32_x(v);
+    else return dpp_x<C>(v);
+}
+template <int C>
+__device__ __forceinline__ float lane_x(float v) {
+    return __int_as_float(lane_x<C>(__float_as_int(v)));
+}
+template <int C>
+__device__ __forceinline__ double lane_x(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = lane_x<C>((int)(b & 0xffffffffll)), hi = lane_x<C>((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
 template <typename TT>
-__device__ __forceinline__ void lu_factor(TT* A, int n, int lda, int* piv, int* misc) {
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int nt = blockDim.x;
-    for (int k = 0; k < n; ++k) {
-        if (wave == 0) {
-            TT best = TT(-1);
-            int bi = n;
-            for (int i = k + lane; i < n; i += 64) {
-                const TT v = tabs(A[i * lda + k]);
-                if (v > best) {
-                    best = v;
-                    bi = i;
-                }
-            }
+__device__ __forceinline__ TT wave_max(TT v) {
+    v = fmax(v, lane_x<kDppRowMirror>(v));
+    v = fmax(v, lane_x<kDppRowHalfMirror>(v));
+    v = fmax(v, lane_x<kDppQuad3210>(v));
+    v = fmax(v, lane_x<kDppQuad1032>(v));
+    v = fmax(v, lane_x<16>(v));
+    return fmax(v, lane_x<32>(v));
+}
+__device__ __forceinline__ int imin(int a, int b) { return a < b ? a : b; }
+__device__ __forceinline__ int wave_min(int v) {
+    v = imin(v, lane_x<kDppRowMirror>(v));
+    v = imin(v, lane_x<kDppRowHalfMirror>(v));
+    v = imin(v, lane_x<kDppQuad3210>(v));
+    v = imin(v, lane_x<kDppQuad1032>(v));
+    v = imin(v, lane_x<16>(v));
+    return imin(v, lane_x<32>(v));
+}
+
+__device__ __forceinline__ float rdlane(float v, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ __forceinline__ double rdlane(double v, int l) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+// Empty asm use: the value must exist here (keeps a load unconditional instead of being sunk
+// into a branch by the select that follows it).
+template <typename TT>
+__device__ __forceinline__ void pin(TT& v) {
+    asm volatile("" : "+v"(v));
+}
+// Empty asm use tagged with the register slot: keeps per-slot branches from being merged into
+// a dynamically indexed (scratch) register read.
+template <int S, typename TT>
+__device__ __forceinline__ TT slot_tag(TT v) {
+    asm volatile("; lu slot %1" : "+v"(v) : "n"(S));
+    return v;
+}
+
+// Factor panel columns [c0, c0 + w) in wave 0.  Row bookkeeping stays in registers (lg per
+// lane row) and is written back once at the end of the panel.
+template <typename TT>
+__device__ __forceinline__ void lu_panel(TT* A, int n, int lda, int c0, int w, int* piv, const LuScratch& W) {
+    const int lane = threadIdx.x & 63;
+    TT P[3][kPanel];
+    int lg[3];
+#pragma unroll
+    for (int s = 0; s < 3; ++s) {
+        const int r = lane + 64 * s;
+        const bool ok = r < n;
+        const int rc = ok ? r : 0;  // clamped: loads are unconditional, values selected after
+        lg[s] = W.lg_of[rc];
+        lg[s] = ok ? lg[s] : -1;
+#pragma unroll
+        for (int jj = 0; jj < kPanel; ++jj) {
+            P[s][jj] = A[rc * lda + c0 + (jj < w ? jj : 0)];
+            pin(P[s][jj]);
+            P[s][jj] = (ok && jj < w) ? P[s][jj] : TT(0);
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < kPanel; ++j) {
+        if (j >= w) continue;
+        const int k = c0 + j;
+        // largest |a_rk| over eligible rows (NaN never wins), then the lowest key holding it;
+        // both reduced over the 64 lanes, so every lane sees the same pivot
+        TT vs[3];
+        TT bv = TT(-1);
+        bool dnan = false;
+#pragma unroll
+        for (int s = 0; s < 3; ++s) {
+            const TT v = P[s][j];
+            vs[s] = lg[s] >= k ? tabs(v) : TT(-1);
+            bv = fmax(bv, vs[s]);
+            dnan |= (lg[s] == k) & (v != v);
+        }
+        bv = wave_max(bv);
+        int bk = 0x7fffffff;
+#pragma unroll
+        for (int s = 0; s < 3; ++s) {
+            const int key = (lg[s] << 8) | (lane + 64 * s);
+            bk = ((lg[s] >= k) & (vs[s] == bv)) ? (key < bk ? key : bk) : bk;
+        }
+        bk = wave_min(bk);
+        int key = __builtin_amdgcn_readfirstlane(bk);
+        if (__any(dnan)) {  // the oracle keeps row k when a_kk is NaN
+            int kp = -1;
+#pragma unroll
+            for (int s = 0; s < 3; ++s) kp = lg[s] == k ? lane + 64 * s : kp;
             for (int off = 32; off > 0; off >>= 1) {
-                const TT ob = __shfl_xor(best, off, 64);
-                const int oi = __shfl_xor(bi, off, 64);
-                if (ob > best || (ob == best && oi < bi)) {
-                    best = ob;
-                    bi = oi;
-                }
+                const int o = __shfl_xor(kp, off, 64);
+                kp = o > kp ? o : kp;
             }
-            if (lane == 0) {
-                if (bi >= n) bi = k;  // column of NaNs: keep the row (no out-of-range swap)
-                piv[k] = bi;
-                misc[0] = bi;
-            }
+            key = (k << 8) | __builtin_amdgcn_readfirstlane(kp);
         }
-        __syncthreads();
-        const int p = misc[0];
-        if (p != k)
-            for (int j = tid; j < n; j += nt) {
-                const TT t = A[k * lda + j];
-                A[k * lda + j] = A[p * lda + j];
-                A[p * lda + j] = t;
-            }
-        __syncthreads();
-        const TT d = A[k * lda + k];
-        const int m = n - k - 1;
-        if (d != TT(0) && m > 0) {
+        const int plog = key >> 8, pphys = key & 255;
+        const int ol = pphys & 63, slot = pphys >> 6;
+        // pivot row (panel part) to every lane
+        TT prow[kPanel];
+#pragma unroll
+        for (int jj = 0; jj < kPanel; ++jj) prow[jj] = TT(0);
+        if (slot == 0) {
+#pragma unroll
+            for (int jj = j; jj < kPanel; ++jj) prow[jj] = rdlane(slot_tag<0>(P[0][jj]), ol);
+        } else if (slot == 1) {
+#pragma unroll
+            for (int jj = j; jj < kPanel; ++jj) prow[jj] = rdlane(slot_tag<1>(P[1][jj]), ol);
+        } else {
+#pragma unroll
+            for (int jj = j; jj < kPanel; ++jj) prow[jj] = rdlane(slot_tag<2>(P[2][jj]), ol);
+        }
+        const TT d = prow[j];
+        if (lane == 0) piv[k] = plog;
+#pragma unroll
+        for (int s = 0; s < 3; ++s) lg[s] = lg[s] == k ? plog : (lg[s] == plog ? k : lg[s]);
+        if (d != TT(0)) {
             const TT inv = TT(1) / d;
-            for (int i = k + 1 + tid; i < n; i += nt) A[i * lda + k] = A[i * lda + k] * inv;
-            __syncthreads();
-            for (int idx = tid; idx < m * m; idx += nt) {
-                const int ii = idx / m, jj = idx - ii * m;
-                const int i = k + 1 + ii, j = k + 1 + jj;
-                A[i * lda + j] -= A[i * lda + k] * A[k * lda + j];
+#pragma unroll
+            for (int s = 0; s < 3; ++s) {
+                const bool below = lg[s] > k;
+                const TT l = below ? P[s][j] * inv : TT(0);
+                P[s][j] = below ? l : P[s][j];
+#pragma unroll
+                for (int jj = j + 1; jj < kPanel; ++jj) P[s][jj] -= l * prow[jj];
             }
         }
-        __syncthreads();
+    }
+#pragma unroll
+    for (int s = 0; s < 3; ++s) {
+        const int r = lane + 64 * s;
+        if (r < n) {
+#pragma unroll
+            for (int jj = 0; jj < kPanel; ++jj)
+                if (jj < w) A[r * lda + c0 + jj] = P[s][jj];
+            W.lg_of[r] = lg[s];
+            W.phys_of[lg[s]] = r;
+        }
     }
 }
 
-// Register-resident LU with partial pivoting on an augmented matrix [A | b].
-// 256 threads form a 16 x 16 grid; thread (ty, tx) owns rows ty + 16a and columns tx + 16b
-// (a, b < 9): a block-cyclic 9 x 9 register tile, n + 1 <= 144.  The right-hand side b is
-// column n, so forward elimination applies the row swaps and L^-1 to it on the way (same
-// operation sequence per entry as pivoting b first and then solving L y = P b column by
-// column).  Pivot: largest |a_ik|, lowest row index on ties.  Three barriers per column.
-// On return the LDS matrix holds L (unit, below the diagonal), U, and y = L^-1 P b in column n.
+// U rows of the panel over the trailing columns: column j of U12 = L11^-1 A12[:, j], one
+// column per thread, written in place into the pivot rows.
 template <typename TT>
-__device__ __forceinline__ void lu_factor_augmented(TT* A, int n, int lda, int* piv, TT* rowbuf, TT* rowbuf2, TT* colbuf,
-                                    TT* candv, int* candi) {
+__device__ __forceinline__ void lu_panel_u(TT* A, int n, int lda, int c0, int w, const LuScratch& W) {
+    const int j0 = c0 + w, nc = n + 1;
+    int pr[kPanel];
+#pragma unroll
+    for (int i = 0; i < kPanel; ++i) pr[i] = __builtin_amdgcn_readfirstlane(W.phys_of[c0 + (i < w ? i : 0)]);
+    for (int j = j0 + (int)threadIdx.x; j < nc; j += blockDim.x) {
+        TT u[kPanel];
+#pragma unroll
+        for (int i = 0; i < kPanel; ++i) {
+            u[i] = A[pr[i] * lda + j];
+            pin(u[i]);
+        }
+#pragma unroll
+        for (int i = 1; i < kPanel; ++i) {
+            const TT* Ar = A + pr[i] * lda + c0;
+            TT acc = u[i];
+#pragma unroll
+            for (int i2 = 0; i2 < i; ++i2) acc -= Ar[i2] * u[i2];
+            u[i] = i < w ? acc : TT(0);
+        }
+#pragma unroll
+        for (int i = 1; i < kPanel; ++i)
+            if (i < w) A[pr[i] * lda + j] = u[i];
+    }
+}
+
+// Trailing rows: a_rj -= sum_k l_rk u_kj (k ascending).  Lane <-> rows (lane, lane + 64,
+// lane + 128) with their panel multipliers in registers (zero for rows already pivoted, whose
+// entries are then rewritten unchanged); wave v takes columns j0 + v, +4, ... two at a time
+// (six independent FMA chains per lane).
+template <typename TT, bool kFull>
+__device__ __forceinline__ void lu_trailing(TT* A, int n, int lda, int c0, int w, const LuScratch& W) {
+    if (kFull) w = kPanel;
+    const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+    const int j0 = c0 + w, nc = n + 1;
+    const int nw = blockDim.x >> 6;
+    TT L[3][kPanel];
+    bool ok[3];
+#pragma unroll
+    for (int s = 0; s < 3; ++s) {
+        const int r = lane + 64 * s;
+        ok[s] = r < n;
+        const int rc = ok[s] ? r : 0;
+        const bool act = ok[s] && W.lg_of[rc] >= j0;
+#pragma unroll
+        for (int i = 0; i < kPanel; ++i) {
+            L[s][i] = A[rc * lda + c0 + (i < w ? i : 0)];
+            pin(L[s][i]);
+            L[s][i] = (act && i < w) ? L[s][i] : TT(0);  // zero multipliers leave a row unchanged
+        }
+    }
+    int pr[kPanel];
+#pragma unroll
+    for (int i = 0; i < kPanel; ++i) pr[i] = __builtin_amdgcn_readfirstlane(W.phys_of[c0 + (i < w ? i : 0)]);
+    for (int j = j0 + wv; j < nc; j += 2 * nw) {
+        const int j2 = j + nw;
+        const bool two = j2 < nc;
+        const int jb = two ? j2 : j;
+        TT u0[kPanel], u1[kPanel];
+#pragma unroll
+        for (int i = 0; i < kPanel; ++i) {
+            u0[i] = A[pr[i] * lda + j];
+            u1[i] = A[pr[i] * lda + jb];
+            if (!kFull) {
+                pin(u0[i]);
+                pin(u1[i]);
+                u0[i] = i < w ? u0[i] : TT(0);
+                u1[i] = i < w ? u1[i] : TT(0);
+            }
+        }
+#pragma unroll
+        for (int s = 0; s < 3; ++s) {
+            if (ok[s]) {
+                TT* Ar = A + (lane + 64 * s) * lda;
+                TT a0 = Ar[j];
+                TT a1 = Ar[jb];
+#pragma unroll
+                for (int i = 0; i < kPanel; ++i) {
+                    a0 -= L[s][i] * u0[i];
+                    a1 -= L[s][i] * u1[i];
+                }
+                Ar[j] = a0;
+                if (two) Ar[jb] = a1;
+            }
+        }
+    }
+}
+
+template <typename TT>
+__device__ __forceinline__ void lu_factor_augmented(TT* A, int n, int lda, int* piv, const LuScratch& W) {
     const int tid = threadIdx.x;
-    const int ty = tid >> 4, tx = tid & 15;
-    const int nc = n + 1;  // columns including the right-hand side
-    TT R[9][9];
-#pragma unroll
-    for (int a = 0; a < 9; ++a)
-#pragma unroll
-        for (int b = 0; b < 9; ++b) {
-            const int r = ty + 16 * a, c = tx + 16 * b;
-            R[a][b] = (r < n && c < nc) ? A[r * lda + c] : TT(0);
-        }
-    for (int k = 0; k < n; ++k) {
-        const int kb = k >> 4, kt = k & 15;
-        // 1. pivot candidates from the owners of column k
-        if (tx == kt) {
-            TT best = TT(-1);
-            int bi = n;
-#pragma unroll
-            for (int a = 0; a < 9; ++a) {
-                const int r = ty + 16 * a;
-                TT ck = R[a][0];
-#pragma unroll
-                for (int bb = 1; bb < 9; ++bb) ck = (kb == bb) ? R[a][bb] : ck;
-                if (r >= k && r < n) {
-                    const TT v = tabs(ck);
-                    if (v > best) {
-                        best = v;
-                        bi = r;
-                    }
-                }
-            }
-            candv[ty] = best;
-            candi[ty] = bi;
-        }
+    for (int r = tid; r < kLuTile; r += blockDim.x) {
+        W.lg_of[r] = r;
+        W.phys_of[r] = r;
+    }
+    __syncthreads();
+    for (int c0 = 0; c0 < n; c0 += kPanel) {
+        const int w = (n - c0) < kPanel ? (n - c0) : kPanel;
+        if (tid < 64) lu_panel<TT>(A, n, lda, c0, w, piv, W);
         __syncthreads();
-        TT best = candv[0];
-        int p = candi[0];
-#pragma unroll
-        for (int t = 1; t < 16; ++t) {
-            const TT v = candv[t];
-            const int i = candi[t];
-            if (v > best || (v == best && i < p)) {
-                best = v;
-                p = i;
-            }
+        if (c0 + w <= n) {
+            lu_panel_u<TT>(A, n, lda, c0, w, W);
+            __syncthreads();
+            if (w == kPanel)
+                lu_trailing<TT, true>(A, n, lda, c0, w, W);
+            else
+                lu_trailing<TT, false>(A, n, lda, c0, w, W);
+            __syncthreads();
         }
-        if (p >= n) p = k;  // column of NaNs: keep the row
-        if (tid == 0) piv[k] = p;
-        // 2. publish the pivot row (old row p) and, if swapping, old row k
-        const int pt = p & 15, pb = p >> 4;
-        if (ty == pt) {
+    }
+}
+
+// x = y / d with y_inv = 1/d correctly rounded: q0 = y y_inv, e = y - d q0, q = q0 + e y_inv
+// (FMA remainder correction; correctly rounded while no intermediate leaves the normal range),
+// i.e. the same value as the division y / d.
+__device__ __forceinline__ double div_by(double y, double d, double d_inv) {
+    const double q0 = y * d_inv;
+    const double e = fma(-d, q0, y);
+    return fma(e, d_inv, q0);
+}
+__device__ __forceinline__ double sel3(const double* r, int a) { return a == 0 ? r[0] : (a == 1 ? r[1] : r[2]); }
+
+// Column-oriented back substitution U x = r by wave 0 (lane owns logical rows lane, +64, +128;
+// n <= 192), FP64, the oracle's operation order: x_k = r_k / u_kk, then r_i -= u_ik x_k for
+// i < k, k descending.  Columns go in blocks of 8 whose entries are loaded together; x_k goes
+// to every lane by readlane; the diagonal reciprocals are formed up front.  Row i of the
+// factors is at physical row pr[a] of A.  On return r holds x.
+constexpr int kSweepBlock = 8;
+
+template <typename TT>
+__device__ __forceinline__ void back_sweep(const TT* A, int n, int lda, const int* pr, const double* dg,
+                                           const double* dinv, double* r) {
+    const int lane = threadIdx.x & 63;
+    for (int kb = ((n - 1) / kSweepBlock) * kSweepBlock; kb >= 0; kb -= kSweepBlock) {
+        TT U[3][kSweepBlock];
 #pragma unroll
-            for (int a = 0; a < 9; ++a)
-                if (a == pb)
+        for (int a = 0; a < 3; ++a)
 #pragma unroll
-                    for (int b = 0; b < 9; ++b) {
-                        const int c = tx + 16 * b;
-                        if (c < nc) rowbuf[c] = R[a][b];
-                    }
-        }
-        if (p != k && ty == kt) {
+            for (int c = 0; c < kSweepBlock; ++c) U[a][c] = A[pr[a] * lda + (kb + c < n ? kb + c : n - 1)];
 #pragma unroll
-            for (int a = 0; a < 9; ++a)
-                if (a == kb)
+        for (int c = kSweepBlock - 1; c >= 0; --c) {
+            const int k = kb + c;
+            if (k < n) {
+                const int ka = k >> 6, kl = k & 63;
+                const double xk = rdlane(div_by(sel3(r, ka), sel3(dg, ka), sel3(dinv, ka)), kl);
 #pragma unroll
-                    for (int b = 0; b < 9; ++b) {
-                        const int c = tx + 16 * b;
-                        if (c < nc) rowbuf2[c] = R[a][b];
-                    }
-        }
-        __syncthreads();
-        if (p != k) {
-            if (ty == kt) {
-#pragma unroll
-                for (int a = 0; a < 9; ++a)
-                    if (a == kb)
-#pragma unroll
-                        for (int b = 0; b < 9; ++b) {
-                            const int c = tx + 16 * b;
-                            if (c < nc) R[a][b] = rowbuf[c];
-                        }
-            }
-            if (ty == pt) {
-#pragma unroll
-                for (int a = 0; a < 9; ++a)
-                    if (a == pb)
-#pragma unroll
-                        for (int b = 0; b < 9; ++b) {
-                            const int c = tx + 16 * b;
-                            if (c < nc) R[a][b] = rowbuf2[c];
-                        }
-            }
-        }
-        // 3. multipliers l_i = a_ik / a_kk below the diagonal (owners of column k)
-        const TT d = rowbuf[k];
-        const bool live = d != TT(0);
-        if (tx == kt && live) {
-            const TT inv = TT(1) / d;
-#pragma unroll
-            for (int a = 0; a < 9; ++a) {
-                const int r = ty + 16 * a;
-                TT ck = R[a][0];
-#pragma unroll
-                for (int bb = 1; bb < 9; ++bb) ck = (kb == bb) ? R[a][bb] : ck;
-                if (r > k && r < n) {
-                    const TT l = ck * inv;
-#pragma unroll
-                    for (int bb = 0; bb < 9; ++bb) R[a][bb] = (kb == bb) ? l : R[a][bb];
-                    colbuf[r] = l;
-                }
-            }
-        }
-        __syncthreads();
-        // 4. rank-1 update of the trailing block (and of the right-hand side column)
-        if (live) {
-            TT u[9];
-#pragma unroll
-            for (int b = 0; b < 9; ++b) {
-                const int c = tx + 16 * b;
-                u[b] = (c > k && c < nc) ? rowbuf[c] : TT(0);
-            }
-#pragma unroll
-            for (int a = 0; a < 9; ++a) {
-                const int r = ty + 16 * a;
-                if (r > k && r < n) {
-                    const TT l = colbuf[r];
-#pragma unroll
-                    for (int b = 0; b < 9; ++b) {
-                        const int c = tx + 16 * b;
-                        if (c > k && c < nc) R[a][b] -= l * u[b];
-                    }
+                for (int a = 0; a < 3; ++a) {
+                    const int i = lane + 64 * a;
+                    const double upd = fma(-(double)U[a][c], xk, r[a]);
+                    r[a] = i < k ? upd : (i == k ? xk : r[a]);
                 }
             }
         }
     }
-    __syncthreads();
-#pragma unroll
-    for (int a = 0; a < 9; ++a)
-#pragma unroll
-        for (int b = 0; b < 9; ++b) {
-            const int r = ty + 16 * a, c = tx + 16 * b;
-            if (r < n && c < nc) A[r * lda + c] = R[a][b];
-        }
-    __syncthreads();
 }
 
-// Back substitution U x = y with y in column n of the factored augmented matrix, FP64
-// accumulation, by wave 0 (column-oriented, the oracle's order).
+// Forward substitution L y = r (unit L) by wave 0, the oracle's order: r_i -= l_ik y_k for
+// i > k, k ascending, in blocks of 8 columns.
 template <typename TT>
-__device__ __forceinline__ void back_substitute(const TT* A, int n, int lda, double* xs) {
+__device__ __forceinline__ void forward_sweep(const TT* A, int n, int lda, const int* pr, double* r) {
+    const int lane = threadIdx.x & 63;
+    for (int kb = 0; kb < n; kb += kSweepBlock) {
+        TT Lc[3][kSweepBlock];
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int c = 0; c < kSweepBlock; ++c) Lc[a][c] = A[pr[a] * lda + (kb + c < n ? kb + c : n - 1)];
+#pragma unroll
+        for (int c = 0; c < kSweepBlock; ++c) {
+            const int k = kb + c;
+            if (k < n) {
+                const double yk = rdlane(sel3(r, k >> 6), k & 63);
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    const int i = lane + 64 * a;
+                    const double upd = fma(-(double)Lc[a][c], yk, r[a]);
+                    r[a] = (i > k) & (i < n) ? upd : r[a];
+                }
+            }
+        }
+    }
+}
+
+template <typename TT>
+__device__ __forceinline__ void diag_of(const TT* A, int n, int lda, const int* pr, double* dg, double* dinv) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const int i = lane + 64 * a;
+        dg[a] = i < n ? (double)A[pr[a] * lda + i] : 1.;
+        dinv[a] = 1. / dg[a];
+    }
+}
+
+// Back substitution U x = y with y in column n of the factored augmented matrix (logical row
+// i at physical row phys_of[i]).
+template <typename TT>
+__device__ __forceinline__ void back_substitute(const TT* A, int n, int lda, const int* phys_of, double* xs) {
     const int tid = threadIdx.x;
     if (tid < 64) {
         const int lane = tid;
-        double r[3];
+        double r[3], dg[3], dinv[3];
+        int pr[3];
 #pragma unroll
-        for (int a = 0; a < 3; ++a) r[a] = (lane + 64 * a) < n ? (double)A[(lane + 64 * a) * lda + n] : 0.;
-        for (int k = n - 1; k >= 0; --k) {
-            const int ka = k >> 6;
-            if (lane == (k & 63)) {
-                const double d = (double)A[k * lda + k];
-                if (ka == 0) r[0] = r[0] / d;
-                else if (ka == 1) r[1] = r[1] / d;
-                else r[2] = r[2] / d;
-            }
-            const double xk = __shfl(ka == 0 ? r[0] : (ka == 1 ? r[1] : r[2]), k & 63, 64);
-#pragma unroll
-            for (int a = 0; a < 3; ++a) {
-                const int i = lane + 64 * a;
-                if (i < k) r[a] -= (double)A[i * lda + k] * xk;
-            }
+        for (int a = 0; a < 3; ++a) {
+            const int i = lane + 64 * a;
+            pr[a] = i < n ? phys_of[i] : 0;
+            r[a] = i < n ? (double)A[pr[a] * lda + n] : 0.;
         }
+        diag_of<TT>(A, n, lda, pr, dg, dinv);
+        back_sweep<TT>(A, n, lda, pr, dg, dinv, r);
 #pragma unroll
         for (int a = 0; a < 3; ++a)
             if (lane + 64 * a < n) xs[lane + 64 * a] = r[a];
@@ -621,12 +810,11 @@ __device__ __forceinline__ void back_substitute(const TT* A, int n, int lda, dou
     __syncthreads();
 }
 
-__device__ __forceinline__ double sel3(const double* r, int a) { return a == 0 ? r[0] : (a == 1 ? r[1] : r[2]); }
-
-// Triangular solves with the LDS factors, FP64 accumulation, executed by wave 0
-// (lane owns rows lane, lane+64, lane+128; n <= 192).  b: LDS, pivoted here; x -> xs.
+// Triangular solves with the LDS factors (logical row i at physical row phys_of[i]), FP64
+// accumulation, by wave 0.  b: LDS, pivoted here; x -> xs.
 template <typename TT>
-__device__ __forceinline__ void lu_solve(const TT* A, int n, int lda, const int* piv, double* b, double* xs) {
+__device__ __forceinline__ void lu_solve(const TT* A, int n, int lda, const int* piv, const int* phys_of, double* b,
+                                         double* xs) {
     const int tid = threadIdx.x;
     if (tid == 0)
         for (int k = 0; k < n; ++k) {
@@ -640,32 +828,17 @@ __device__ __forceinline__ void lu_solve(const TT* A, int n, int lda, const int*
     __syncthreads();
     if (tid < 64) {
         const int lane = tid;
-        double r[3];
+        double r[3], dg[3], dinv[3];
+        int pr[3];
 #pragma unroll
-        for (int a = 0; a < 3; ++a) r[a] = (lane + 64 * a) < n ? b[lane + 64 * a] : 0.;
-        for (int k = 0; k < n; ++k) {
-            const double bk = __shfl(sel3(r, k >> 6), k & 63, 64);
-#pragma unroll
-            for (int a = 0; a < 3; ++a) {
-                const int i = lane + 64 * a;
-                if (i > k && i < n) r[a] -= (double)A[i * lda + k] * bk;
-            }
+        for (int a = 0; a < 3; ++a) {
+            const int i = lane + 64 * a;
+            pr[a] = i < n ? phys_of[i] : 0;
+            r[a] = i < n ? b[i] : 0.;
         }
-        for (int k = n - 1; k >= 0; --k) {
-            const int ka = k >> 6;
-            if (lane == (k & 63)) {
-                const double d = (double)A[k * lda + k];
-                if (ka == 0) r[0] = r[0] / d;
-                else if (ka == 1) r[1] = r[1] / d;
-                else r[2] = r[2] / d;
-            }
-            const double xk = __shfl(sel3(r, ka), k & 63, 64);
-#pragma unroll
-            for (int a = 0; a < 3; ++a) {
-                const int i = lane + 64 * a;
-                if (i < k) r[a] -= (double)A[i * lda + k] * xk;
-            }
-        }
+        diag_of<TT>(A, n, lda, pr, dg, dinv);
+        forward_sweep<TT>(A, n, lda, pr, r);
+        back_sweep<TT>(A, n, lda, pr, dg, dinv, r);
 #pragma unroll
         for (int a = 0; a < 3; ++a)
             if (lane + 64 * a < n) xs[lane + 64 * a] = r[a];
@@ -984,10 +1157,12 @@ __device__ __forceinline__ void factor_solve_uncondition(const DevSys& S, int b,
     }
     for (int i = t; i < n; i += blockDim.x) A[i * lda + n] = (TT)rhs[i];
     __syncthreads();
-    lu_factor_augmented<TT>(A, n, lda, piv, reinterpret_cast<TT*>(sm + L.rowbuf), reinterpret_cast<TT*>(sm + L.rowbuf2),
-                            reinterpret_cast<TT*>(sm + L.colbuf), reinterpret_cast<TT*>(sm + L.candv),
-                            reinterpret_cast<int*>(sm + L.candi));
-    back_substitute<TT>(A, n, lda, xs);
+    const LuScratch W{reinterpret_cast<int*>(sm + L.lg_of), reinterpret_cast<int*>(sm + L.phys_of)};
+    KYN_PHASE_MARK(7);
+    lu_factor_augmented<TT>(A, n, lda, piv, W);
+    KYN_PHASE_MARK(8);
+    back_substitute<TT>(A, n, lda, W.phys_of, xs);
+    KYN_PHASE_MARK(9);
     for (int sweep = 0; Ac != nullptr && sweep < S.refine_sweeps; ++sweep) {
         for (int i = t; i < n; i += blockDim.x) {
             double r = S.rhs[off + i];
@@ -996,7 +1171,7 @@ __device__ __forceinline__ void factor_solve_uncondition(const DevSys& S, int b,
             rhs[i] = r;
         }
         __syncthreads();
-        lu_solve<TT>(A, n, lda, piv, rhs, xs2);
+        lu_solve<TT>(A, n, lda, piv, W.phys_of, rhs, xs2);
         for (int i = t; i < n; i += blockDim.x) xs[i] += xs2[i];
         __syncthreads();
     }
@@ -1047,14 +1222,19 @@ __global__ void __launch_bounds__(kThreads) k_fused_step(DevSys S, int Nm, int Q
         for (int b = S.model_block_begin[m]; b < S.model_block_end[m]; ++b) {
             const int n = S.block_n[b];
             const int e = S.block_elems[S.block_elem_begin[b]];
+            KYN_PHASE_MARK(0);
             element_phase<TT, true>(S, e, sm, L, n);
+            KYN_PHASE_MARK(5);
             block_constraints<TT>(S, b, sm, L);
             condition_rhs(S, b, rhs);
+            KYN_PHASE_MARK(6);
             // unfactored copy for refinement: compact rows (stride n) at the block's own offset
             TT* Ac = (S.Acopy != nullptr && S.refine_sweeps > 0) ? reinterpret_cast<TT*>(S.Acopy) + S.block_A_off[b]
                                                                  : nullptr;
             factor_solve_uncondition<TT>(S, b, A, Ac, L.lda, piv, rhs, xs, xs2, sm, L);
+            KYN_PHASE_MARK(10);
             err_sum += block_error_and_update(S, b, xs, red);
+            KYN_PHASE_MARK(11);
         }
         const double err = sqrt(err_sum / ndofs);
         if (t == 0) S.err_hist[(size_t)m * p.max_iter + iter] = err;

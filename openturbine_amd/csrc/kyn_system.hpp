@@ -17,6 +17,7 @@ namespace kyn {
 constexpr int kMaxElemNodes = 21;  // register tile of the element contraction (3 x 21)
 constexpr int kMaxElemQps = 64;
 constexpr int kThreads = 256;      // workgroup size of every per-element / per-block kernel
+constexpr int kMaxBlockDofs = 143; // register LU tile holds n + 1 <= 144 columns ([A | b])
 
 struct StepConsts {
     double h, alpha_m, alpha_f, gamma, beta, gamma_prime, beta_prime, conditioner, atol, rtol;
