@@ -131,6 +131,7 @@ def main():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-plain", action="store_true")
     ap.add_argument("--skip-fp64", action="store_true")
+    ap.add_argument("--refine", type=int, default=0, help="FP64 refinement sweeps of the FP32-tangent solve")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run one device copy of BYTES (counter calibration) and exit")
     args = ap.parse_args()
@@ -158,7 +159,7 @@ def main():
 
     log("leg: fused FP32 tangent (hipGraph)")
     r32 = run_leg(lib, specs, sched, exec_mode=ota.EXEC_FUSED, precision=ota.TANGENT_FP32,
-                  warmup=args.warmup, steps=args.steps, refine=1)
+                  warmup=args.warmup, steps=args.steps, refine=args.refine)
     log(f"  {args.steps} steps in {r32['wall_s']:.3f}s, fails {r32['fails']}, mean iters/model/step "
         f"{r32['iters'].mean() / args.steps:.2f}")
     # kernel timing without the graph (events bracket the step kernel alone)
@@ -216,7 +217,7 @@ def main():
         "failed_model_steps": {"fp32": r32["fails"], "fp64": r64["fails"] if r64 else None},
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": 157.3, "unit": "TFLOP/s",
                      "frac": round(achieved / 157.3, 5), "traffic": None,
-                     "kernel": "k_fused_step<float>", "kernel_ms": round(kern_s * 1000.0, 4),
+                     "kernel": "k_fused_step_f32", "kernel_ms": round(kern_s * 1000.0, 4),
                      "algorithmic_flops_per_launch": flops_launch,
                      "hbm_algorithmic_bytes_per_launch": bytes_launch,
                      "hbm_achieved_GBps": round(bytes_launch / kern_s / 1e9, 2)},

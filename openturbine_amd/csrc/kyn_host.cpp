@@ -22,7 +22,7 @@
 #include "kyn_system.hpp"
 
 namespace kyn {
-size_t fused_lds_bytes(int Nm, int Qm, int nm, int precision);
+size_t fused_lds_bytes(int Nm, int Qm, int nm, int cm, int refine, int precision);
 size_t element_lds_bytes(int Nm, int Qm);
 size_t solve_lds_bytes(int nm);
 hipError_t set_lds_limits(size_t bytes);
@@ -712,8 +712,9 @@ kyn_system kyn_system_create(const kyn_model* models, size_t num_models, const k
                     throw KynError(KYN_ERR_UNSUPPORTED, "the fused form needs exactly one beam element per independent block");
                 if (B.block_con_end[b] - B.block_con_begin[b] > 8)
                     throw KynError(KYN_ERR_UNSUPPORTED, "the fused form allows at most 8 constraints per block");
+                S.max_block_cons = std::max(S.max_block_cons, B.block_con_end[b] - B.block_con_begin[b]);
             }
-            need = kyn::fused_lds_bytes(Nm, Qm, nm, S.precision);
+            need = kyn::fused_lds_bytes(Nm, Qm, nm, S.max_block_cons, S.refine_sweeps, S.precision);
         } else {
             need = std::max(kyn::element_lds_bytes(Nm, Qm), kyn::solve_lds_bytes(nm));
         }

@@ -12,9 +12,9 @@
 // calculate_system_matrix.hpp): S_ij = sum_q phi_i R0_q[j] + phi'_i R1_q[j] where
 //   R0_q[j] = (phi_j wJ(Kuu+Quu) + phi'_j w Puu) T_j + phi_j wJ(beta' Muu + gamma' Guu)
 //   R1_q[j] = (phi_j w Ouu + phi'_j (w/J) Cuu) T_j
-// i.e. an (N x 2Q) x (2Q x 36N) product.  R is built per QP chunk in LDS; each of 252
-// threads owns one 6x6 component c and the rows i = ig, ig+7, ig+14 for all j (a 3 x 21
-// register tile).
+// i.e. an (N x 2Q) x (2Q x 36N) product.  Each of 252 threads owns one 6x6 component c and
+// the nodes j = jg, jg+7, jg+14, forms its own R entries per QP in registers and accumulates
+// all rows i (a 21 x 3 register tile); no LDS staging of R, no barrier inside the QP loop.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -29,19 +29,18 @@ namespace kyn {
 #define KYN_PHASE_MARK(i)
 #endif
 
-constexpr int kQpChunk = 2;
 constexpr int kLuTile = 144;  // row bookkeeping of the block LU (n <= kMaxBlockDofs < kLuTile)
 static_assert(kMaxBlockDofs < kLuTile && kMaxBlockDofs < 192, "block LU: 3 rows per panel lane, 8-bit row keys");
-constexpr int kContractThreads = 252;  // 7 row groups x 36 components
+constexpr int kContractThreads = 252;  // 36 components x 7 node groups
 
 // ---------------------------------------------------------------------------------------
 // LDS layout (identical on host and device)
 // ---------------------------------------------------------------------------------------
 struct Lay {
     // persistent
-    int nq, nv, nvd, tr, loc, rhs, xs, xs2, piv, red, cres, misc, lg_of, phys_of;
+    int loc, rhs, xs, xs2, piv, red, cres, misc, lg_of, phys_of;
     // overlay: element view
-    int phi, dphi, w, J, qpF, qpM, R;
+    int nq, nv, nvd, tr, phi, dphi, w, J, qpF, qpM, phiT;
     // overlay: matrix view
     int A;
     int lda;
@@ -50,7 +49,8 @@ struct Lay {
 
 __host__ __device__ inline int align16(int x) { return (x + 15) & ~15; }
 
-__host__ __device__ inline Lay make_layout(int Nm, int Qm, int nm, int tt_size, bool with_matrix) {
+// cm: most constraints in one block; refine: FP64 refinement scratch needed.
+__host__ __device__ inline Lay make_layout(int Nm, int Qm, int nm, int cm, bool refine, int tt_size, bool with_matrix) {
     Lay L{};
     int o = 0;
     auto take = [&](int bytes) {
@@ -58,28 +58,28 @@ __host__ __device__ inline Lay make_layout(int Nm, int Qm, int nm, int tt_size, 
         o = align16(o + bytes);
         return r;
     };
-    L.nq = take(Nm * 7 * 8);
-    L.nv = take(Nm * 6 * 8);
-    L.nvd = take(Nm * 6 * 8);
-    L.tr = take(Nm * 9 * 8);
     L.loc = take(Nm * 4);
     L.rhs = take(nm * 8);
     L.xs = take(nm * 8);
-    L.xs2 = take(nm * 8);
+    L.xs2 = take(refine ? nm * 8 : 0);
     L.piv = take(nm * 4);
     L.red = take(16 * 8);
-    L.cres = take(8 * 24 * 8);  // up to 8 constraints per block: res, lres_b, lres_t, sys_res
+    L.cres = take(cm * 24 * 8);  // per constraint: res, lres_b, lres_t, sys_res
     L.misc = take(16 * 4);
     L.lg_of = take(kLuTile * 4);
     L.phys_of = take(kLuTile * 4);
     const int overlay = o;
+    L.nq = take(Nm * 7 * 8);
+    L.nv = take(Nm * 6 * 8);
+    L.nvd = take(Nm * 6 * 8);
+    L.tr = take(Nm * 9 * 8);
     L.phi = take(Nm * Qm * 8);
     L.dphi = take(Nm * Qm * 8);
     L.w = take(Qm * 8);
     L.J = take(Qm * 8);
     L.qpF = take(Qm * 24 * 8);
     L.qpM = take(Qm * 6 * 36 * tt_size);
-    L.R = take(2 * kQpChunk * 2 * Nm * 36 * tt_size);  // two chunk buffers
+    L.phiT = take(Qm * Nm * 2 * tt_size);  // {phi_i(q), phi'_i(q)} pairs, QP-major, tangent precision
     const int elem_end = o;
     L.lda = (nm + 1) | 1;  // [A | b]; odd row stride: row-per-lane accesses hit distinct banks
     L.A = overlay;
@@ -131,7 +131,7 @@ __device__ __forceinline__ void element_phase(const DevSys& S, int e, char* sm, 
     double* Jq = reinterpret_cast<double*>(sm + L.J);
     double* qpF = reinterpret_cast<double*>(sm + L.qpF);
     TT* qpM = reinterpret_cast<TT*>(sm + L.qpM);
-    TT* Rb = reinterpret_cast<TT*>(sm + L.R);
+    TT* phiT = reinterpret_cast<TT*>(sm + L.phiT);
     double* rhs = reinterpret_cast<double*>(sm + L.rhs);
     const double h = S.p.h;
 
@@ -158,8 +158,12 @@ __device__ __forceinline__ void element_phase(const DevSys& S, int e, char* sm, 
         loc[k] = kFused ? S.node_local[node] : 0;
     }
     for (int k = tid; k < N * Q; k += blockDim.x) {
-        phi[k] = S.shape_interp[soff + k];
-        dphi[k] = S.shape_deriv[soff + k];
+        const double p = S.shape_interp[soff + k], dp = S.shape_deriv[soff + k];
+        phi[k] = p;
+        dphi[k] = dp;
+        const int i = k / Q, q = k - Q * (k / Q);
+        phiT[(q * N + i) * 2] = (TT)p;
+        phiT[(q * N + i) * 2 + 1] = (TT)dp;
     }
     for (int q = tid; q < Q; q += blockDim.x) {
         wq[q] = S.qp_weight[qoff + q];
@@ -266,91 +270,67 @@ __device__ __forceinline__ void element_phase(const DevSys& S, int e, char* sm, 
     }
 
     KYN_PHASE_MARK(3);
-    // ---- contraction over QP chunks
-    // R rows of one QP chunk are built into one of two LDS buffers while the chunk before is
-    // contracted from the other, so each chunk costs one barrier.
-    // Build roles: thread t < 216 owns (k, r, s) = (t % 72) / 36, (t % 36) / 6, t % 6 and nodes
-    // j = t / 72 + 3 jj.  Contraction roles: thread t < 252 owns component cc = t % 36 and rows
-    // i = t / 36 + 7 a for all j (a 3 x 21 register tile).
-    const int bkc = tid % 72;
-    const int bk = bkc / 36;
-    const int bc = bkc - 36 * bk;
-    const int br = bc / 6, bs = bc - 6 * (bc / 6);
-    const int bj0 = tid / 72;
-    const bool builder = tid < 216;
-    const bool rot = bs >= 3;
-    const int bcol = rot ? 3 : 0;  // first column of the 3 combined with T_j's column (s - 3)
-    // M6 layout per QP: A1 = wJ Kuu, A2 = wJ Quu, Mi = wJ(beta' Muu + gamma' Guu), P, O, C
-    const int offa = bk ? 144 : 0;    // A1 | O
-    const int offp = bk ? 180 : 108;  // P  | C
-    const TT a2on = bk ? TT(0) : TT(1);
-    const TT mon = bk ? TT(0) : TT(1);
-    auto build_chunk = [&](int q0, int qn, TT* Rc) {
-        if (!builder) return;
-        for (int qq = 0; qq < qn; ++qq) {
-            const int q = q0 + qq;
-            const TT* M6 = qpM + (size_t)q * 6 * 36;
-            TT ra[3], rp[3];
-#pragma unroll
-            for (int u = 0; u < 3; ++u) {
-                ra[u] = M6[offa + br * 6 + bcol + u] + a2on * M6[36 + br * 6 + bcol + u];
-                rp[u] = M6[offp + br * 6 + bcol + u];
-            }
-            const TT rm = mon * M6[72 + br * 6 + bs];
-            TT* out = Rc + (size_t)(qq * 2 + bk) * N * 36 + bc;
-            for (int j = bj0; j < N; j += 3) {
-                const TT pj = (TT)phi[j * Q + q], dj = (TT)dphi[j * Q + q];
-                TT tc[3];
-#pragma unroll
-                for (int u = 0; u < 3; ++u) tc[u] = rot ? (TT)tr[j * 9 + u * 3 + (bs - 3)] : (u == bs ? TT(1) : TT(0));
-                TT val = pj * rm;
-#pragma unroll
-                for (int u = 0; u < 3; ++u) val += (pj * ra[u] + dj * rp[u]) * tc[u];
-                out[j * 36] = val;
-            }
-        }
-    };
-    const int ig = tid / 36;
-    const int cc = tid - 36 * (tid / 36);
+    // ---- contraction, one pass over the QPs, no barrier
+    // Thread t < 252 owns component cc = t % 36 = (r, s) and nodes j = t / 36 + 7 b (b < 3);
+    // per QP it forms its own R0_q[j][cc], R1_q[j][cc] from row r of the QP matrices and
+    // column s of T_j, then accumulates S_ij[cc] += phi_i R0 + phi'_i R1 for every row i
+    // (phi pairs are LDS broadcasts).  Each R entry is formed by exactly one thread.
+    const int cc = tid % 36, jg = tid / 36;
     const bool active = tid < kContractThreads;
     const int r_ = cc / 6, s_ = cc - 6 * (cc / 6);
-    TT acc[3][kMaxElemNodes];
+    const bool rot = s_ >= 3;
+    const int col0 = rot ? 3 : 0;  // columns of A/P/O/C combined with T_j's column (s - 3)
+    TT tc[3][3];
+    int jc[3];
 #pragma unroll
-    for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b) {
+        const int j = jg + 7 * b;
+        const bool ok = active && j < N;
+        jc[b] = ok ? j : 0;
 #pragma unroll
-        for (int j = 0; j < kMaxElemNodes; ++j) acc[a][j] = TT(0);
-    const size_t rbuf = (size_t)kQpChunk * 2 * N * 36;
-    build_chunk(0, Q < kQpChunk ? Q : kQpChunk, Rb);
-    __syncthreads();
-    for (int q0 = 0, ch = 0; q0 < Q; q0 += kQpChunk, ++ch) {
-        const int qn = (Q - q0) < kQpChunk ? (Q - q0) : kQpChunk;
-        const int q1 = q0 + kQpChunk;
-        if (q1 < Q) build_chunk(q1, (Q - q1) < kQpChunk ? (Q - q1) : kQpChunk, Rb + ((ch + 1) & 1) * rbuf);
-        const TT* Rc = Rb + (ch & 1) * rbuf;
-        if (active) {
-            for (int qq = 0; qq < qn; ++qq) {
-                const int q = q0 + qq;
-                TT pi[3], di[3];
+        for (int u = 0; u < 3; ++u)
+            tc[b][u] = !ok ? TT(0) : (rot ? (TT)tr[j * 9 + u * 3 + (s_ - 3)] : (u == s_ ? TT(1) : TT(0)));
+    }
+    TT acc[kMaxElemNodes][3];
 #pragma unroll
-                for (int a = 0; a < 3; ++a) {
-                    const int i = ig + 7 * a;
-                    pi[a] = i < N ? (TT)phi[i * Q + q] : TT(0);
-                    di[a] = i < N ? (TT)dphi[i * Q + q] : TT(0);
-                }
-                const TT* R0 = Rc + (size_t)(qq * 2 + 0) * N * 36 + cc;
-                const TT* R1 = Rc + (size_t)(qq * 2 + 1) * N * 36 + cc;
+    for (int i = 0; i < kMaxElemNodes; ++i)
 #pragma unroll
-                for (int j = 0; j < kMaxElemNodes; ++j) {
-                    if (j < N) {
-                        const TT x0 = R0[j * 36], x1 = R1[j * 36];
+        for (int b = 0; b < 3; ++b) acc[i][b] = TT(0);
+    for (int q = 0; q < Q; ++q) {
+        const TT* M6 = qpM + (size_t)q * 6 * 36 + r_ * 6;
+        TT ra[3], rp[3], ro[3], rc[3];
 #pragma unroll
-                        for (int a = 0; a < 3; ++a) acc[a][j] += pi[a] * x0 + di[a] * x1;
-                    }
-                }
+        for (int u = 0; u < 3; ++u) {
+            ra[u] = M6[col0 + u] + M6[36 + col0 + u];
+            rp[u] = M6[108 + col0 + u];
+            ro[u] = M6[144 + col0 + u];
+            rc[u] = M6[180 + col0 + u];
+        }
+        const TT rm = M6[72 + s_];
+        const TT* ph = phiT + (size_t)q * N * 2;
+        TT R0[3], R1[3];
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+            const TT pj = ph[jc[b] * 2], dj = ph[jc[b] * 2 + 1];
+            TT v0 = pj * rm, v1 = TT(0);
+#pragma unroll
+            for (int u = 0; u < 3; ++u) {
+                v0 += (pj * ra[u] + dj * rp[u]) * tc[b][u];
+                v1 += (pj * ro[u] + dj * rc[u]) * tc[b][u];
+            }
+            R0[b] = v0;
+            R1[b] = v1;
+        }
+#pragma unroll
+        for (int i = 0; i < kMaxElemNodes; ++i) {
+            if (i < N) {
+                const TT pi = ph[i * 2], di = ph[i * 2 + 1];
+#pragma unroll
+                for (int b = 0; b < 3; ++b) acc[i][b] += pi * R0[b] + di * R1[b];
             }
         }
-        __syncthreads();
     }
+    __syncthreads();
 
     KYN_PHASE_MARK(4);
     // ---- write out
@@ -362,13 +342,14 @@ __device__ __forceinline__ void element_phase(const DevSys& S, int e, char* sm, 
         const TT cond = (TT)S.p.conditioner;
         if (active) {
 #pragma unroll
-            for (int a = 0; a < 3; ++a) {
-                const int i = ig + 7 * a;
+            for (int i = 0; i < kMaxElemNodes; ++i) {
                 if (i < N) {
                     const int row = loc[i] + r_;
 #pragma unroll
-                    for (int j = 0; j < kMaxElemNodes; ++j)
-                        if (j < N) A[row * L.lda + loc[j] + s_] = acc[a][j] * cond;
+                    for (int b = 0; b < 3; ++b) {
+                        const int j = jg + 7 * b;
+                        if (j < N) A[row * L.lda + loc[j] + s_] = acc[i][b] * cond;
+                    }
                 }
             }
         }
@@ -377,12 +358,13 @@ __device__ __forceinline__ void element_phase(const DevSys& S, int e, char* sm, 
         if (active) {
             double* out = S.elem_mat + S.elem_mat_off[e];
 #pragma unroll
-            for (int a = 0; a < 3; ++a) {
-                const int i = ig + 7 * a;
+            for (int i = 0; i < kMaxElemNodes; ++i) {
                 if (i < N) {
 #pragma unroll
-                    for (int j = 0; j < kMaxElemNodes; ++j)
-                        if (j < N) out[(size_t)(i * N + j) * 36 + cc] = (double)acc[a][j];
+                    for (int b = 0; b < 3; ++b) {
+                        const int j = jg + 7 * b;
+                        if (j < N) out[(size_t)(i * N + j) * 36 + cc] = (double)acc[i][b];
+                    }
                 }
             }
         }
@@ -1187,9 +1169,9 @@ __device__ __forceinline__ void factor_solve_uncondition(const DevSys& S, int b,
 // Fused form: one workgroup per model, one launch per time step
 // ---------------------------------------------------------------------------------------
 template <typename TT>
-__global__ void __launch_bounds__(kThreads) k_fused_step(DevSys S, int Nm, int Qm, int nm) {
+__device__ __forceinline__ void fused_step_body(const DevSys& S, int Nm, int Qm, int nm) {
     extern __shared__ __align__(16) char sm[];
-    const Lay L = make_layout(Nm, Qm, nm, (int)sizeof(TT), true);
+    const Lay L = make_layout(Nm, Qm, nm, S.max_block_cons, S.refine_sweeps > 0, (int)sizeof(TT), true);
     const int m = blockIdx.x;
     const int t = threadIdx.x;
     double* red = reinterpret_cast<double*>(sm + L.red);
@@ -1262,6 +1244,16 @@ __global__ void __launch_bounds__(kThreads) k_fused_step(DevSys S, int Nm, int Q
     }
 }
 
+// FP64 tangent: the KKT block alone is up to 141 KB of LDS, one workgroup per CU.
+__global__ void __launch_bounds__(kThreads) k_fused_step_f64(DevSys S, int Nm, int Qm, int nm) {
+    fused_step_body<double>(S, Nm, Qm, nm);
+}
+// FP32 tangent: blocks up to 143 dofs fit in 80 KB, so two workgroups share a CU (registers
+// capped at 256 per lane for that).
+__global__ void __launch_bounds__(kThreads, 2) k_fused_step_f32(DevSys S, int Nm, int Qm, int nm) {
+    fused_step_body<float>(S, Nm, Qm, nm);
+}
+
 // ---------------------------------------------------------------------------------------
 // Plain form: the reference's kernel sequence, one launch per stage
 // ---------------------------------------------------------------------------------------
@@ -1309,7 +1301,7 @@ __global__ void k_plain_tangent(DevSys S) {
 // UpdateSystemVariablesBeams (src/step/update_system_variables_beams.hpp:13-69)
 __global__ void __launch_bounds__(kThreads) k_plain_element(DevSys S, int Nm, int Qm) {
     extern __shared__ __align__(16) char sm[];
-    const Lay L = make_layout(Nm, Qm, 1, 8, false);
+    const Lay L = make_layout(Nm, Qm, 1, 0, false, 8, false);
     const int e = blockIdx.x;
     const int m = S.block_model[S.node_block[S.elem_nodes[S.elem_node_off[e]]]];
     if (!S.model_active[m]) return;
@@ -1463,7 +1455,7 @@ __global__ void __launch_bounds__(kThreads) k_plain_assemble(DevSys S) {
 // SolveSystem (src/step/solve_system.hpp:14-39): LU + solve per block in LDS
 __global__ void __launch_bounds__(kThreads) k_plain_solve(DevSys S, int nm) {
     extern __shared__ __align__(16) char sm[];
-    const Lay L = make_layout(1, 1, nm, 8, true);
+    const Lay L = make_layout(1, 1, nm, 0, false, 8, true);
     const int b = blockIdx.x, t = threadIdx.x;
     const int m = S.block_model[b];
     if (!S.model_active[m]) return;
@@ -1519,17 +1511,17 @@ __global__ void __launch_bounds__(kThreads) k_plain_epilogue(DevSys S) {
 // ---------------------------------------------------------------------------------------
 // Host-side launchers
 // ---------------------------------------------------------------------------------------
-size_t fused_lds_bytes(int Nm, int Qm, int nm, int precision) {
-    return (size_t)make_layout(Nm, Qm, nm, precision ? 4 : 8, true).total;
+size_t fused_lds_bytes(int Nm, int Qm, int nm, int cm, int refine, int precision) {
+    return (size_t)make_layout(Nm, Qm, nm, cm, refine > 0, precision ? 4 : 8, true).total;
 }
-size_t element_lds_bytes(int Nm, int Qm) { return (size_t)make_layout(Nm, Qm, 1, 8, false).total; }
-size_t solve_lds_bytes(int nm) { return (size_t)make_layout(1, 1, nm, 8, true).total; }
+size_t element_lds_bytes(int Nm, int Qm) { return (size_t)make_layout(Nm, Qm, 1, 0, false, 8, false).total; }
+size_t solve_lds_bytes(int nm) { return (size_t)make_layout(1, 1, nm, 0, false, 8, true).total; }
 
 hipError_t set_lds_limits(size_t bytes) {
     hipError_t e;
-    e = hipFuncSetAttribute((const void*)k_fused_step<double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    e = hipFuncSetAttribute((const void*)k_fused_step_f64, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
     if (e != hipSuccess) return e;
-    e = hipFuncSetAttribute((const void*)k_fused_step<float>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    e = hipFuncSetAttribute((const void*)k_fused_step_f32, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
     if (e != hipSuccess) return e;
     e = hipFuncSetAttribute((const void*)k_plain_element, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
     if (e != hipSuccess) return e;
@@ -1537,11 +1529,11 @@ hipError_t set_lds_limits(size_t bytes) {
 }
 
 void launch_fused_step(const DevSys& S, int Nm, int Qm, int nm, hipStream_t st) {
-    const size_t lds = fused_lds_bytes(Nm, Qm, nm, S.precision);
+    const size_t lds = fused_lds_bytes(Nm, Qm, nm, S.max_block_cons, S.refine_sweeps, S.precision);
     if (S.precision)
-        hipLaunchKernelGGL(k_fused_step<float>, dim3(S.num_models), dim3(kThreads), lds, st, S, Nm, Qm, nm);
+        hipLaunchKernelGGL(k_fused_step_f32, dim3(S.num_models), dim3(kThreads), lds, st, S, Nm, Qm, nm);
     else
-        hipLaunchKernelGGL(k_fused_step<double>, dim3(S.num_models), dim3(kThreads), lds, st, S, Nm, Qm, nm);
+        hipLaunchKernelGGL(k_fused_step_f64, dim3(S.num_models), dim3(kThreads), lds, st, S, Nm, Qm, nm);
 }
 void launch_advance_slot(const DevSys& S, hipStream_t st) {
     hipLaunchKernelGGL(k_advance_slot, dim3(1), dim3(64), 0, st, S);

@@ -30,6 +30,7 @@ struct DevSys {
     int num_models, num_blocks, num_elems, num_nodes, num_cons;
     int precision;      // 0 FP64 tangent, 1 FP32 tangent
     int refine_sweeps;  // FP64 iterative refinement sweeps (FP32 tangent)
+    int max_block_cons; // most constraints in one block (fused LDS layout)
 
     // ---- state (src/state/state.hpp), [n][7] / [n][6]
     double *x0, *x, *q_prev, *q, *q_delta, *v, *vd, *a, *f;

@@ -146,10 +146,11 @@ def test_config2_rotor(kyn, orc, mode):
         assert rel(g.get(f), o.get(f)) < STATE_TOL[f], f
 
 
-def test_small_ensemble_fused_fp64_and_fp32(kyn, orc):
+@pytest.mark.parametrize("refine", [0, 2])
+def test_small_ensemble_fused_fp64_and_fp32(kyn, orc, refine):
     specs = W.ensemble_config_specs(num_rotors=6, num_steps=5)
     g64 = ota.KynSystem(kyn, specs, exec_mode=ota.EXEC_FUSED)
-    g32 = ota.KynSystem(kyn, specs, exec_mode=ota.EXEC_FUSED, precision=ota.TANGENT_FP32, refine_sweeps=2)
+    g32 = ota.KynSystem(kyn, specs, exec_mode=ota.EXEC_FUSED, precision=ota.TANGENT_FP32, refine_sweeps=refine)
     oracles = [OracleSystem(orc, sp) for sp in specs]
     for _ in range(5):
         assert g64.step() == 0
@@ -164,7 +165,7 @@ def test_small_ensemble_fused_fp64_and_fp32(kyn, orc):
         sl = g64.model_slice(k)
         assert rel(q64[sl], o.get("q")) < 1e-12
         e32 = rel(q32[sl], o.get("q"))
-        print(f"rotor {k}: FP32-tangent final q rel error {e32:.2e}", flush=True)
+        print(f"rotor {k}: FP32-tangent (refine {refine}) final q rel error {e32:.2e}", flush=True)
         assert e32 < 1e-9
 
 
