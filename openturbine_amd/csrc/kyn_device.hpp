@@ -150,10 +150,13 @@ KYN_DEV void calc_displacement(double h, const double* qd, const double* qp, dou
 // calculate_gravity_force, calculate_gyroscopic_matrix, calculate_inertia_stiffness_matrix}.
 // Outputs Muu, Guu, Kuu (36 each) and Fi, Fg (6 each).
 // ---------------------------------------------------------------------------------------
+// Results go out as soon as they are final (keeps few 6x6 blocks live): F[12..18) = Fi,
+// F[18..24) = Fg, M6[0..36) = wJ Kuu, M6[72..108) = wJ (beta' Muu + gamma' Guu).
+template <typename TT>
 KYN_DEV void qp_inertial(const double* r_in /*4, unnormalized*/, const double* ud,
                          const double* w, const double* wd, const double* r0, const double* Ms,
-                         const double* g, double* Muu, double* Guu, double* Kuu, double* Fi,
-                         double* Fg) {
+                         const double* g, double wJ, double bp, double gp, TT* M6, double* F) {
+    double Muu[36], Fi[6], Fg[6];
     double r[4];
     const double rl = sqrt(r_in[0] * r_in[0] + r_in[1] * r_in[1] + r_in[2] * r_in[2] + r_in[3] * r_in[3]);
 #pragma unroll
@@ -200,12 +203,15 @@ KYN_DEV void qp_inertial(const double* r_in /*4, unnormalized*/, const double* u
         Fg[3 + i] = tg[i];
     }
 #pragma unroll
-    for (int i = 0; i < 36; ++i) {
-        Guu[i] = 0.;
-        Kuu[i] = 0.;
+    for (int i = 0; i < 6; ++i) {
+        F[12 + i] = Fi[i];
+        F[18 + i] = Fg[i];
     }
-    // gyroscopic (calculate_gyroscopic_matrix.hpp:20-60)
+    // gyroscopic (calculate_gyroscopic_matrix.hpp:20-60); only the right 6x3 column is nonzero
     {
+        double Guu[36];
+#pragma unroll
+        for (int i = 0; i < 36; ++i) Guu[i] = 0.;
         const double me[3] = {m * eta[0], m * eta[1], m * eta[2]};
         double v2[3], A[9], B[9], tm[9];
         mv3(wt, me, v2);
@@ -223,6 +229,8 @@ KYN_DEV void qp_inertial(const double* r_in /*4, unnormalized*/, const double* u
         for (int i = 0; i < 3; ++i)
 #pragma unroll
             for (int j = 0; j < 3; ++j) Guu[(3 + i) * 6 + 3 + j] = wr[i * 3 + j] - C[i * 3 + j];
+#pragma unroll
+        for (int c = 0; c < 36; ++c) M6[72 + c] = (TT)(wJ * (bp * Muu[c] + gp * Guu[c]));
     }
     // inertia stiffness (calculate_inertia_stiffness_matrix.hpp:19-60)
     {
@@ -251,6 +259,9 @@ KYN_DEV void qp_inertial(const double* r_in /*4, unnormalized*/, const double* u
         mm3(wt, m1, t5);
 #pragma unroll
         for (int i = 0; i < 9; ++i) k22[i] = t5[i] + k22[i];
+        double Kuu[36];
+#pragma unroll
+        for (int i = 0; i < 36; ++i) Kuu[i] = 0.;
 #pragma unroll
         for (int i = 0; i < 3; ++i)
 #pragma unroll
@@ -258,15 +269,21 @@ KYN_DEV void qp_inertial(const double* r_in /*4, unnormalized*/, const double* u
                 Kuu[i * 6 + 3 + j] = K12[i * 3 + j];
                 Kuu[(3 + i) * 6 + 3 + j] = k22[i * 3 + j];
             }
+#pragma unroll
+        for (int c = 0; c < 36; ++c) M6[c] = (TT)(wJ * Kuu[c]);
     }
 }
 
 // Stiffness half: calculate_stiffness_quadrature_point_values.hpp:47-128 with
 // beams/{calculate_strain, calculate_temporary_variables, calculate_force_FC,
 // calculate_force_FD, calculate_Ouu, calculate_Puu, calculate_Quu}.
+// Results go out as soon as they are final: F[0..6) = Fc, F[6..12) = Fd,
+// M6[180..216) = (w/J) Cuu, M6[144..180) = w Ouu, M6[108..144) = w Puu, M6[36..72) = wJ Quu.
+template <typename TT>
 KYN_DEV void qp_stiffness(const double* r_in, const double* rp, const double* up,
-                          const double* r0, const double* x0p, const double* Cs, double* Cuu,
-                          double* Ouu, double* Puu, double* Quu, double* Fc, double* Fd) {
+                          const double* r0, const double* x0p, const double* Cs, double wJ, double w,
+                          double wdJ, TT* M6, double* F) {
+    double Cuu[36], Fc[6], Fd[6];
     double r[4];
     const double rl = sqrt(r_in[0] * r_in[0] + r_in[1] * r_in[1] + r_in[2] * r_in[2] + r_in[3] * r_in[3]);
 #pragma unroll
@@ -304,6 +321,11 @@ KYN_DEV void qp_stiffness(const double* r_in, const double* rp, const double* up
     Fd[2] = 0.;
 #pragma unroll
     for (int i = 0; i < 3; ++i) Fd[3 + i] = xs[i] * Fc[0] + xs[3 + i] * Fc[1] + xs[6 + i] * Fc[2];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+        F[i] = Fc[i];
+        F[6 + i] = Fd[i];
+    }
     double Nt[9], Mt[9];
     tilde(Fc, Nt);
     tilde(Fc + 3, Mt);
@@ -316,6 +338,9 @@ KYN_DEV void qp_stiffness(const double* r_in, const double* rp, const double* up
             C12[i * 3 + j] = Cuu[i * 6 + 3 + j];
             C21[i * 3 + j] = Cuu[(3 + i) * 6 + j];
         }
+#pragma unroll
+    for (int c = 0; c < 36; ++c) M6[180 + c] = (TT)(wdJ * Cuu[c]);
+    double Ouu[36], Puu[36], Quu[36];
 #pragma unroll
     for (int i = 0; i < 36; ++i) {
         Ouu[i] = 0.;
@@ -332,6 +357,8 @@ KYN_DEV void qp_stiffness(const double* r_in, const double* rp, const double* up
             Ouu[i * 6 + 3 + j] = a[i * 3 + j] - Nt[i * 3 + j];
             Ouu[(3 + i) * 6 + 3 + j] = b[i * 3 + j] - Mt[i * 3 + j];
         }
+#pragma unroll
+    for (int c = 0; c < 36; ++c) M6[144 + c] = (TT)(w * Ouu[c]);
     double c[9], d[9];
     mtm3(xs, C11, c);
     mtm3(xs, C12, d);
@@ -342,6 +369,8 @@ KYN_DEV void qp_stiffness(const double* r_in, const double* rp, const double* up
             Puu[(3 + i) * 6 + j] = c[i * 3 + j] + Nt[i * 3 + j];
             Puu[(3 + i) * 6 + 3 + j] = d[i * 3 + j];
         }
+#pragma unroll
+    for (int k = 0; k < 36; ++k) M6[108 + k] = (TT)(w * Puu[k]);
     double m1[9], q22[9];
 #pragma unroll
     for (int i = 0; i < 9; ++i) m1[i] = a[i] - Nt[i];
@@ -350,6 +379,8 @@ KYN_DEV void qp_stiffness(const double* r_in, const double* rp, const double* up
     for (int i = 0; i < 3; ++i)
 #pragma unroll
         for (int j = 0; j < 3; ++j) Quu[(3 + i) * 6 + 3 + j] = q22[i * 3 + j];
+#pragma unroll
+    for (int k = 0; k < 36; ++k) M6[36 + k] = (TT)(wJ * Quu[k]);
 }
 
 // AX_Matrix / AxialVectorOfMatrix (src/math/matrix_operations.hpp:18-49)

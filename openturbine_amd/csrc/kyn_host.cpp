@@ -748,6 +748,18 @@ kyn_system kyn_system_create(const kyn_model* models, size_t num_models, const k
         S.model_ndofs = Ar.up(B.model_ndofs);
         S.model_gravity = Ar.up(B.model_gravity);
         S.model_load_begin = Ar.zeros<int>(num_models + 1);
+        {
+            // dispatch order of the fused launch: longest expected step first (sum over the model's
+            // blocks of n^2, between the panel's O(n) and the trailing update's O(n^3)), so the
+            // short models fill the tail of the grid
+            std::vector<double> cost(num_models, 0.);
+            for (size_t b = 0; b < B.block_n.size(); ++b)
+                cost[B.block_model[b]] += static_cast<double>(B.block_n[b]) * B.block_n[b];
+            std::vector<int> order(num_models);
+            for (size_t m = 0; m < num_models; ++m) order[m] = static_cast<int>(m);
+            std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cost[x] > cost[y]; });
+            S.model_order = Ar.up(order);
+        }
         S.block_model = Ar.up(B.block_model);
         S.block_n = Ar.up(B.block_n);
         S.block_nsys = Ar.up(B.block_nsys);

@@ -195,22 +195,9 @@ __device__ __forceinline__ void element_phase(const DevSys& S, int e, char* sm, 
 #pragma unroll
                 for (int c = 0; c < 4; ++c) r[c] += nq[k * 7 + 3 + c] * p;
             }
-            double Muu[36], Guu[36], Kuu[36], Fi[6], Fg[6];
             double r0l[4] = {r0[0], r0[1], r0[2], r0[3]};
-            qp_inertial(r, ud, om, omd, r0l, S.qp_Mstar + (size_t)(qoff + q) * 36, grav, Muu, Guu,
-                        Kuu, Fi, Fg);
-            const double wJ = w * J;
-            const double bp = S.p.beta_prime, gp = S.p.gamma_prime;
-#pragma unroll
-            for (int c = 0; c < 36; ++c) {
-                M6[c] = (TT)(wJ * Kuu[c]);
-                M6[72 + c] = (TT)(wJ * (bp * Muu[c] + gp * Guu[c]));
-            }
-#pragma unroll
-            for (int c = 0; c < 6; ++c) {
-                F[12 + c] = Fi[c];
-                F[18 + c] = Fg[c];
-            }
+            qp_inertial<TT>(r, ud, om, omd, r0l, S.qp_Mstar + (size_t)(qoff + q) * 36, grav, w * J, S.p.beta_prime,
+                            S.p.gamma_prime, M6, F);
         } else {
             // interpolate_to_quadrature_point_for_stiffness.hpp:19-41
             double r[4] = {0., 0., 0., 0.}, rp[4] = {0., 0., 0., 0.}, up[3] = {0., 0., 0.};
@@ -225,25 +212,10 @@ __device__ __forceinline__ void element_phase(const DevSys& S, int e, char* sm, 
                     rp[c] += nq[k * 7 + 3 + c] * dp;
                 }
             }
-            double Cuu[36], Ouu[36], Puu[36], Quu[36], Fc[6], Fd[6];
             const double r0l[4] = {r0[0], r0[1], r0[2], r0[3]};
             const double* xp = S.qp_x0p + (size_t)(qoff + q) * 3;
             const double x0p[3] = {xp[0], xp[1], xp[2]};
-            qp_stiffness(r, rp, up, r0l, x0p, S.qp_Cstar + (size_t)(qoff + q) * 36, Cuu, Ouu, Puu, Quu,
-                         Fc, Fd);
-            const double wJ = w * J, wdJ = w / J;
-#pragma unroll
-            for (int c = 0; c < 36; ++c) {
-                M6[36 + c] = (TT)(wJ * Quu[c]);
-                M6[108 + c] = (TT)(w * Puu[c]);
-                M6[144 + c] = (TT)(w * Ouu[c]);
-                M6[180 + c] = (TT)(wdJ * Cuu[c]);
-            }
-#pragma unroll
-            for (int c = 0; c < 6; ++c) {
-                F[c] = Fc[c];
-                F[6 + c] = Fd[c];
-            }
+            qp_stiffness<TT>(r, rp, up, r0l, x0p, S.qp_Cstar + (size_t)(qoff + q) * 36, w * J, w, w / J, M6, F);
         }
     }
     __syncthreads();
@@ -1126,16 +1098,16 @@ __device__ __forceinline__ void condition_rhs(const DevSys& S, int b, double* rh
 
 // Factor + solve of one block (replaces dss::numeric_factorization / dss::solve,
 // src/step/solve_system.hpp:27-31): the conditioned rhs rides along as column n of the
-// register LU, then back substitution; optional FP64 iterative refinement against the
-// unfactored matrix (FP32 tangent; copy in HBM); then UnconditionSolution
+// blocked LU, then back substitution; optional FP64 iterative refinement against the
+// unfactored matrix (FP32 tangent; column-major copy in HBM); then UnconditionSolution
 // (condition_system.hpp:24-31).  A: LDS matrix with lda >= n + 1.
 template <typename TT>
 __device__ __forceinline__ void factor_solve_uncondition(const DevSys& S, int b, TT* A, TT* Ac, int lda, int* piv,
                                          double* rhs, double* xs, double* xs2, char* sm, const Lay& L) {
     const int n = S.block_n[b], nsys = S.block_nsys[b], off = S.block_vec_off[b];
     const int t = threadIdx.x;
-    if (Ac != nullptr) {
-        for (int k = t; k < n * n; k += blockDim.x) Ac[k] = A[(k / n) * lda + (k % n)];
+    if (Ac != nullptr) {  // column-major copy: the refinement residual reads it coalesced
+        for (int k = t; k < n * n; k += blockDim.x) Ac[k] = A[(k % n) * lda + (k / n)];
     }
     for (int i = t; i < n; i += blockDim.x) A[i * lda + n] = (TT)rhs[i];
     __syncthreads();
@@ -1148,8 +1120,9 @@ __device__ __forceinline__ void factor_solve_uncondition(const DevSys& S, int b,
     for (int sweep = 0; Ac != nullptr && sweep < S.refine_sweeps; ++sweep) {
         for (int i = t; i < n; i += blockDim.x) {
             double r = S.rhs[off + i];
-            const TT* row = Ac + (size_t)i * n;
-            for (int j = 0; j < n; ++j) r -= (double)row[j] * xs[j];
+            const TT* col = Ac + i;
+#pragma unroll 4
+            for (int j = 0; j < n; ++j) r -= (double)col[(size_t)j * n] * xs[j];
             rhs[i] = r;
         }
         __syncthreads();
@@ -1172,7 +1145,7 @@ template <typename TT>
 __device__ __forceinline__ void fused_step_body(const DevSys& S, int Nm, int Qm, int nm) {
     extern __shared__ __align__(16) char sm[];
     const Lay L = make_layout(Nm, Qm, nm, S.max_block_cons, S.refine_sweeps > 0, (int)sizeof(TT), true);
-    const int m = blockIdx.x;
+    const int m = S.model_order[blockIdx.x];
     const int t = threadIdx.x;
     double* red = reinterpret_cast<double*>(sm + L.red);
     int* misc = reinterpret_cast<int*>(sm + L.misc);

@@ -43,6 +43,7 @@ struct DevSys {
     const int* model_ndofs;
     const double* model_gravity;  // [m][3]
     const int* model_load_begin;  // into sched_load_nodes (sorted by node), [m+1]
+    const int* model_order;       // fused launch: workgroup g runs model model_order[g] (costliest first)
 
     // ---- blocks
     const int* block_model;
