@@ -12,11 +12,12 @@ Timed legs (all on the one GPU):
   fused_fp64       fused form, FP64 tangent
   plain_fp64       the reference's launch structure (one kernel per stage, host reads the
                    convergence error every Newton iteration), FP64
-Roofline: the fused FP32 step kernel, algorithmic flops per launch (element contraction
-144 N^2 Q + LU 2/3 n^3 + triangular solves 2 n^2 per block per Newton iteration, summed over
-the iterations actually taken) over its HIP-event-timed duration.
-cpu_baseline: the CPU oracle (same algorithm, dense pivoted LU instead of KLU) on a bounded
-sample of the same ensemble, OpenMP over rotors on the host cores.
+Roofline: the fused FP32 step kernel; algorithmic HBM bytes per launch, 8(67N + 87Q) + 36 s N^2
+per element and Newton iteration (s = 4), summed over the iterations actually taken, over its
+HIP-event-timed duration; flop rates (contraction 144 N^2 Q; with LU 2/3 n^3 and solves 2 n^2)
+as extra fields; measured traffic from the committed counter summary.
+cpu_baseline: the CPU oracle (same algorithm, dense pivoted LU instead of KLU) on the same
+ensemble (all rotors, 4 steps by default), OpenMP over rotors on the host cores.
 """
 
 import argparse
@@ -67,6 +68,23 @@ def block_bytes(specs, s):
             b_ += 8.0 * (67 * N + 87 * Q) + 36.0 * s * N * N
         out.append(b_)
     return np.array(out)
+
+
+def contraction_flops(specs):
+    """Per model: node-pair contraction flops of one Newton iteration, 144 N^2 Q per element."""
+    return np.array([sum(144.0 * len(b.nodes) ** 2 * len(b.quad) for b in sp.beams) for sp in specs])
+
+
+def measured_traffic():
+    """HBM bytes per launch of the FP32 step kernel from the committed counter summary
+    (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs, copy-calibrated), if present."""
+    path = ROOT / "profiles" / "k_fused_step_f32_ensemble4096_traffic.txt"
+    if not path.exists():
+        return None
+    for line in path.read_text().splitlines():
+        if line.startswith("traffic_bytes_per_launch:"):
+            return {"bytes_per_launch": float(line.split(":", 1)[1]), "source": str(path.relative_to(ROOT))}
+    return None
 
 
 def run_leg(lib, specs, sched, *, exec_mode, precision, warmup, steps, refine=0, use_graph=True):
@@ -126,8 +144,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--rotors", type=int, default=4096)
     ap.add_argument("--plain-steps", type=int, default=3)
-    ap.add_argument("--cpu-rotors", type=int, default=48)
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-rotors", type=int, default=4096)
+    ap.add_argument("--cpu-steps", type=int, default=4)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-plain", action="store_true")
     ap.add_argument("--skip-fp64", action="store_true")
@@ -161,12 +179,7 @@ def main():
     r32 = run_leg(lib, specs, sched, exec_mode=ota.EXEC_FUSED, precision=ota.TANGENT_FP32,
                   warmup=args.warmup, steps=args.steps, refine=args.refine)
     log(f"  {args.steps} steps in {r32['wall_s']:.3f}s, fails {r32['fails']}, mean iters/model/step "
-        f"{r32['iters'].mean() / args.steps:.2f}")
-    # kernel timing without the graph (events bracket the step kernel alone)
-    sysm = r32["system"]
-    sysm.step_many(2, use_graph=False)
-    kern_ms = sysm.last_kernel_ms()
-    it_total, _ = sysm.totals()
+        f"{r32['iters'].mean() / args.steps:.2f}, step kernel {r32['kernel_ms']:.3f} ms")
     del r32["system"]
 
     r64 = None
@@ -187,12 +200,16 @@ def main():
 
     value = args.steps / r32["wall_s"]
     ms_per_step = 1000.0 * r32["wall_s"] / args.steps
-    # roofline of the fused FP32 step kernel: flops of the iterations taken per step
+    # roofline of the fused FP32 step kernel (HIP events around each graph launch of the timed
+    # loop, on the library's stream): algorithmic HBM bytes 8(67N + 87Q) + 36 s N^2 per element
+    # and Newton iteration (s = 4), summed over the iterations each model actually took
     iters_per_step = r32["iters"] / args.steps
-    flops_launch = float(np.sum(iters_per_step * flops_iter))
     bytes_launch = float(np.sum(iters_per_step * bytes_iter32))
-    kern_s = (kern_ms if kern_ms > 0 else ms_per_step) / 1000.0
-    achieved = flops_launch / kern_s / 1e12
+    flops_launch = float(np.sum(iters_per_step * flops_iter))
+    mfma_flops_launch = float(np.sum(iters_per_step * contraction_flops(specs)))
+    kern_s = (r32["kernel_ms"] if r32["kernel_ms"] > 0 else ms_per_step) / 1000.0
+    achieved = bytes_launch / kern_s / 1e9
+    traffic = measured_traffic()
     result = {
         "metric": METRIC,
         "value": round(value, 4),
@@ -208,19 +225,23 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"ensemble of {args.rotors} independent 3-blade rotors, N in {{6,11,16,21}}, "
                                "Q=2N-2, h=0.01, fused one-kernel-per-step, hipGraph",
-                   "models": args.rotors, "blades": 3 * args.rotors},
+                   "models": args.rotors, "blades": 3 * args.rotors, "refine_sweeps": args.refine},
         "fused_fp64_steps_per_s": round(args.steps / r64["wall_s"], 4) if r64 else None,
         "plain_fp64_steps_per_s": round(args.plain_steps / rp["wall_s"], 4) if rp else None,
         "speedup_fp32_fused_vs_plain_fp64": round(value / (args.plain_steps / rp["wall_s"]), 2) if rp else None,
         "newton_iters_per_model_step": {"fp32": round(float(r32["iters"].mean()) / args.steps, 3),
                                          "fp64": round(float(r64["iters"].mean()) / args.steps, 3) if r64 else None},
         "failed_model_steps": {"fp32": r32["fails"], "fp64": r64["fails"] if r64 else None},
-        "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": 157.3, "unit": "TFLOP/s",
-                     "frac": round(achieved / 157.3, 5), "traffic": None,
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": 8000.0, "unit": "GB/s",
+                     "frac": round(achieved / 8000.0, 5),
+                     "traffic": traffic["bytes_per_launch"] if traffic else None,
+                     "traffic_source": traffic["source"] if traffic else None,
                      "kernel": "k_fused_step_f32", "kernel_ms": round(kern_s * 1000.0, 4),
-                     "algorithmic_flops_per_launch": flops_launch,
-                     "hbm_algorithmic_bytes_per_launch": bytes_launch,
-                     "hbm_achieved_GBps": round(bytes_launch / kern_s / 1e9, 2)},
+                     "algorithmic_bytes_per_launch": bytes_launch,
+                     "contraction_flops_per_launch": mfma_flops_launch,
+                     "contraction_TFLOPs": round(mfma_flops_launch / kern_s / 1e12, 3),
+                     "all_flops_per_launch": flops_launch,
+                     "all_TFLOPs": round(flops_launch / kern_s / 1e12, 3)},
         "cpu_baseline": None,
     }
     if not args.skip_cpu:
