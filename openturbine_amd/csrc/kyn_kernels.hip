@@ -543,15 +543,26 @@ __device__ __forceinline__ void lu_panel(TT* A, int n, int lda, int c0, int w, i
     }
 }
 
-// U rows of the panel over the trailing columns: column j of U12 = L11^-1 A12[:, j], one
-// column per thread, written in place into the pivot rows.
-template <typename TT>
-__device__ __forceinline__ void lu_panel_u(TT* A, int n, int lda, int c0, int w, const LuScratch& W) {
-    const int j0 = c0 + w, nc = n + 1;
+// Applies panel [c0, c0 + w) to this wave's share of the columns [ja, jb): columns
+// ja + g, ja + g + gs, ... (wave g of gs).  First the panel's U rows (U12 = L11^-1 A12, lane <->
+// column, written in place into the pivot rows), then the trailing rows a_rj -= sum_k l_rk u_kj
+// (k ascending; lane <-> rows lane, lane + 64, lane + 128 with their multipliers in registers,
+// zero for rows already pivoted, which are then rewritten unchanged; two columns at a time).
+// Runs within one wave: its LDS writes are seen by its own later reads without a barrier.
+template <typename TT, bool kFull>
+__device__ __forceinline__ void lu_apply_panel(TT* A, int n, int lda, int c0, int w, int ja, int jb, int g, int gs,
+                                               const LuScratch& W) {
+    if (kFull) w = kPanel;
+    const int lane = threadIdx.x & 63;
+    const int j0 = c0 + w;  // rows with logical index >= j0 are trailing rows
+    const int m = ja + g < jb ? (jb - ja - g + gs - 1) / gs : 0;
+    if (m == 0) return;
     int pr[kPanel];
 #pragma unroll
     for (int i = 0; i < kPanel; ++i) pr[i] = __builtin_amdgcn_readfirstlane(W.phys_of[c0 + (i < w ? i : 0)]);
-    for (int j = j0 + (int)threadIdx.x; j < nc; j += blockDim.x) {
+    // U rows
+    for (int t = lane; t < m; t += 64) {
+        const int j = ja + g + gs * t;
         TT u[kPanel];
 #pragma unroll
         for (int i = 0; i < kPanel; ++i) {
@@ -564,24 +575,16 @@ __device__ __forceinline__ void lu_panel_u(TT* A, int n, int lda, int c0, int w,
             TT acc = u[i];
 #pragma unroll
             for (int i2 = 0; i2 < i; ++i2) acc -= Ar[i2] * u[i2];
-            u[i] = i < w ? acc : TT(0);
+            u[i] = (kFull || i < w) ? acc : TT(0);
         }
 #pragma unroll
         for (int i = 1; i < kPanel; ++i)
-            if (i < w) A[pr[i] * lda + j] = u[i];
+            if (kFull || i < w) A[pr[i] * lda + j] = u[i];
     }
-}
-
-// Trailing rows: a_rj -= sum_k l_rk u_kj (k ascending).  Lane <-> rows (lane, lane + 64,
-// lane + 128) with their panel multipliers in registers (zero for rows already pivoted, whose
-// entries are then rewritten unchanged); wave v takes columns j0 + v, +4, ... two at a time
-// (six independent FMA chains per lane).
-template <typename TT, bool kFull>
-__device__ __forceinline__ void lu_trailing(TT* A, int n, int lda, int c0, int w, const LuScratch& W) {
-    if (kFull) w = kPanel;
-    const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
-    const int j0 = c0 + w, nc = n + 1;
-    const int nw = blockDim.x >> 6;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    // trailing rows
     TT L[3][kPanel];
     bool ok[3];
 #pragma unroll
@@ -594,21 +597,18 @@ __device__ __forceinline__ void lu_trailing(TT* A, int n, int lda, int c0, int w
         for (int i = 0; i < kPanel; ++i) {
             L[s][i] = A[rc * lda + c0 + (i < w ? i : 0)];
             pin(L[s][i]);
-            L[s][i] = (act && i < w) ? L[s][i] : TT(0);  // zero multipliers leave a row unchanged
+            L[s][i] = (act && (kFull || i < w)) ? L[s][i] : TT(0);
         }
     }
-    int pr[kPanel];
-#pragma unroll
-    for (int i = 0; i < kPanel; ++i) pr[i] = __builtin_amdgcn_readfirstlane(W.phys_of[c0 + (i < w ? i : 0)]);
-    for (int j = j0 + wv; j < nc; j += 2 * nw) {
-        const int j2 = j + nw;
-        const bool two = j2 < nc;
-        const int jb = two ? j2 : j;
+    for (int t = 0; t < m; t += 2) {
+        const int j = ja + g + gs * t;
+        const bool two = t + 1 < m;
+        const int jq = two ? j + gs : j;
         TT u0[kPanel], u1[kPanel];
 #pragma unroll
         for (int i = 0; i < kPanel; ++i) {
             u0[i] = A[pr[i] * lda + j];
-            u1[i] = A[pr[i] * lda + jb];
+            u1[i] = A[pr[i] * lda + jq];
             if (!kFull) {
                 pin(u0[i]);
                 pin(u1[i]);
@@ -621,40 +621,59 @@ __device__ __forceinline__ void lu_trailing(TT* A, int n, int lda, int c0, int w
             if (ok[s]) {
                 TT* Ar = A + (lane + 64 * s) * lda;
                 TT a0 = Ar[j];
-                TT a1 = Ar[jb];
+                TT a1 = Ar[jq];
 #pragma unroll
                 for (int i = 0; i < kPanel; ++i) {
                     a0 -= L[s][i] * u0[i];
                     a1 -= L[s][i] * u1[i];
                 }
                 Ar[j] = a0;
-                if (two) Ar[jb] = a1;
+                if (two) Ar[jq] = a1;
             }
         }
     }
 }
 
 template <typename TT>
+__device__ __forceinline__ void lu_apply(TT* A, int n, int lda, int c0, int w, int ja, int jb, int g, int gs,
+                                         const LuScratch& W) {
+    if (w == kPanel)
+        lu_apply_panel<TT, true>(A, n, lda, c0, w, ja, jb, g, gs, W);
+    else
+        lu_apply_panel<TT, false>(A, n, lda, c0, w, ja, jb, g, gs, W);
+}
+
+// Panels with look-ahead: all waves first apply panel p to the next panel's columns; then wave
+// 0 factors the next panel while the other waves apply panel p to the remaining columns.  The
+// other waves read the row maps while wave 0 rewrites them at the end of a panel; the entries
+// they use (physical rows of panel p's pivots, whether a row is still trailing) do not change
+// there.
+template <typename TT>
 __device__ __forceinline__ void lu_factor_augmented(TT* A, int n, int lda, int* piv, const LuScratch& W) {
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, wv = tid >> 6;
+    const int nw = blockDim.x >> 6;
+    const int nc = n + 1;
     for (int r = tid; r < kLuTile; r += blockDim.x) {
         W.lg_of[r] = r;
         W.phys_of[r] = r;
     }
     __syncthreads();
+    if (wv == 0) lu_panel<TT>(A, n, lda, 0, n < kPanel ? n : kPanel, piv, W);
+    __syncthreads();
     for (int c0 = 0; c0 < n; c0 += kPanel) {
         const int w = (n - c0) < kPanel ? (n - c0) : kPanel;
-        if (tid < 64) lu_panel<TT>(A, n, lda, c0, w, piv, W);
-        __syncthreads();
-        if (c0 + w <= n) {
-            lu_panel_u<TT>(A, n, lda, c0, w, W);
-            __syncthreads();
-            if (w == kPanel)
-                lu_trailing<TT, true>(A, n, lda, c0, w, W);
-            else
-                lu_trailing<TT, false>(A, n, lda, c0, w, W);
+        const int j0 = c0 + w;
+        const int wn = j0 < n ? ((n - j0) < kPanel ? (n - j0) : kPanel) : 0;  // next panel's width
+        if (wn > 0) {
+            lu_apply<TT>(A, n, lda, c0, w, j0, j0 + wn, wv, nw, W);
             __syncthreads();
         }
+        if (wv == 0) {
+            if (wn > 0) lu_panel<TT>(A, n, lda, j0, wn, piv, W);
+        } else {
+            lu_apply<TT>(A, n, lda, c0, w, j0 + wn, nc, wv - 1, nw - 1, W);
+        }
+        __syncthreads();
     }
 }
 
