@@ -133,6 +133,20 @@ class ConstraintSpec:
 
 
 @dataclass
+class MassSpec:
+    node: int
+    M: np.ndarray              # [36] mass matrix in the material frame
+
+
+@dataclass
+class SpringSpec:
+    n1: int
+    n2: int
+    k: float
+    l0: float
+
+
+@dataclass
 class ModelSpec:
     name: str
     x0: np.ndarray             # [n, 7]
@@ -155,6 +169,8 @@ class ModelSpec:
     #   "element_node_loads": {elem: [N, 6]}
     step_inputs: Optional[Callable] = None
     meta: dict = field(default_factory=dict)
+    masses: list = field(default_factory=list)
+    springs: list = field(default_factory=list)
 
     @property
     def num_nodes(self) -> int:
@@ -186,9 +202,61 @@ def build_model(api, spec: ModelSpec):
         api.model_add_node(m, spec.x0[i], spec.u[i], spec.v[i], spec.vd[i], spec.s[i])
     for b in spec.beams:
         api.model_add_beam_element(m, b.nodes, b.section_s, b.section_M, b.section_C, b.quad)
+    for e in spec.masses:
+        api.model_add_mass_element(m, e.node, e.M)
+    for e in spec.springs:
+        api.model_add_spring_element(m, e.n1, e.n2, e.k, e.l0)
     for c in spec.constraints:
         api.model_add_constraint(m, c.type, c.base, c.target, c.axis, c.init)
     return m
+
+
+# --------------------------------------------------------------------------------------
+# reference point-mass / spring regression tests (inputs in tests/golden/masses_springs.json)
+# --------------------------------------------------------------------------------------
+def masses_external_force_spec(fx: dict) -> ModelSpec:
+    """tests/regression_tests/regression/test_masses.cpp (MassesTest.ExternalForce): one free
+    point mass, constant force and torque, rho_inf = 0."""
+    nt = _NodeTable()
+    nt.add([0, 0, 0, 1, 0, 0, 0])
+    x0, u, v, vd, s = nt.arrays()
+    f = np.zeros((1, 6))
+    f[0, 0] = fx["force_x"]
+    f[0, 4] = fx["torque_y"]
+    max_iter, rho = fx["step_parameters"]
+
+    def step_inputs(step, _state=None):
+        return {"nodal_loads": f}
+
+    return ModelSpec("masses_external_force", x0, u, v, vd, s, [], [], np.zeros(3), max_iter=int(max_iter),
+                     h=fx["time_step"], rho_inf=rho, num_steps=int(fx["n_steps"]), step_inputs=step_inputs,
+                     masses=[MassSpec(0, np.array(fx["mass_matrix"]))])
+
+
+def precession_spec(fx: dict) -> ModelSpec:
+    """tests/regression_tests/regression/test_precession.cpp: a free spinning point mass."""
+    nt = _NodeTable()
+    nt.add([0, 0, 0, 1, 0, 0, 0], v=fx["velocity"])
+    x0, u, v, vd, s = nt.arrays()
+    return ModelSpec("precession", x0, u, v, vd, s, [], [], np.zeros(3), max_iter=int(fx["max_iter"]),
+                     h=fx["step_size"], rho_inf=fx["rho_inf"], num_steps=int(fx["num_steps"]),
+                     masses=[MassSpec(0, np.array(fx["mass_matrix"]))])
+
+
+def spring_mass_spec(fx: dict) -> ModelSpec:
+    """tests/regression_tests/regression/test_spring_mass_system.cpp (SpringMassSystemTest): a
+    point mass on a spring to a clamped node, released 2 m from equilibrium; h = T / steps."""
+    nt = _NodeTable()
+    for p in fx["positions"]:
+        nt.add(p)
+    x0, u, v, vd, s = nt.arrays()
+    m, j, k = fx["m"], fx["j"], fx["k"]
+    period = 2.0 * np.pi * np.sqrt(m / k)
+    M = _diag6([m, m, m, j, j, j])
+    return ModelSpec("spring_mass", x0, u, v, vd, s, [], [ConstraintSpec(FIXED_BC, 0, 0)], np.zeros(3),
+                     max_iter=int(fx["max_iter"]), h=period / fx["num_steps"], rho_inf=fx["rho_inf"],
+                     num_steps=int(fx["num_steps"]), masses=[MassSpec(1, M)],
+                     springs=[SpringSpec(0, 1, k, fx["l0"])])
 
 
 def _diag6(vals):

@@ -73,6 +73,7 @@ int count_active_dofs(int sig) {  // src/dof_management/freedom_signature.hpp:38
     return c;
 }
 constexpr int kAll = 0b111111;
+constexpr int kJustPosition = 0b111000;  // freedom_signature.hpp:20
 constexpr int kNone = 0;
 }  // namespace
 
@@ -611,6 +612,25 @@ System CreateSystem(const ModelIn& model) {
         s.beams.push_back(std::move(e));
     }
 
+    // Masses: src/elements/masses/create_masses.hpp:24-60
+    for (const auto& mi : model.masses) {
+        MassElem e;
+        e.node = mi.node;
+        e.Mstar = mi.M;
+        for (int c = 0; c < 7; ++c) e.x0[c] = model.nodes[mi.node].x0[c];
+        s.masses.push_back(e);
+    }
+    // Springs: src/elements/springs/create_springs.hpp:24-58
+    for (const auto& si : model.springs) {
+        SpringElem e;
+        e.n1 = si.n1;
+        e.n2 = si.n2;
+        for (int c = 0; c < 3; ++c) e.x0[c] = model.nodes[si.n2].x0[c] - model.nodes[si.n1].x0[c];
+        e.l_ref = si.l0;
+        e.k = si.k;
+        s.springs.push_back(e);
+    }
+
     // Constraints: src/constraints/constraints.hpp:76-266
     size_t start_row = 0;
     for (const auto& ci : model.constraints) {
@@ -673,6 +693,11 @@ System CreateSystem(const ModelIn& model) {
     s.sig.assign(n, kNone);
     for (const auto& e : s.beams)
         for (auto nd : e.node_idx) s.sig[nd] |= kAll;
+    for (const auto& e : s.masses) s.sig[e.node] |= kAll;
+    for (const auto& e : s.springs) {
+        s.sig[e.n1] |= kJustPosition;
+        s.sig[e.n2] |= kJustPosition;
+    }
     for (const auto& c : s.cons) {
         s.sig[c.target] |= c.target_sig;
         if (c.type == CType::RigidJoint || c.type == CType::RevoluteJoint) s.sig[c.base] |= c.base_sig;
@@ -706,6 +731,7 @@ System CreateSystem(const ModelIn& model) {
     };
     for (const auto& e : s.beams)
         for (size_t k = 1; k < e.N; ++k) unite(e.node_idx[0], e.node_idx[k]);
+    for (const auto& e : s.springs) unite(e.n1, e.n2);
     for (const auto& c : s.cons)
         if (c.type == CType::RigidJoint || c.type == CType::RevoluteJoint) unite(c.base, c.target);
     std::vector<long> root_to_block(n, -1);
@@ -930,7 +956,67 @@ void UpdateSystemVariablesBeams(const StepParameters& p, System& s) {
     for (long e = 0; e < nb; ++e)
         CalculateQuadraturePointValues(p.beta_prime, p.gamma_prime, s, s.beams[static_cast<size_t>(e)]);
 }
-// src/step/assemble_system_residual.hpp:17-64 (forces, then beams)
+// src/system/masses/calculate_quadrature_point_values.hpp:41-152
+void CalculateMassValues(double beta_prime, double gamma_prime, const System& s, MassElem& e) {
+    const size_t i = e.node;
+    const Vec3 ud{s.vd[i * 6 + 0], s.vd[i * 6 + 1], s.vd[i * 6 + 2]};
+    const Vec3 wd{s.vd[i * 6 + 3], s.vd[i * 6 + 4], s.vd[i * 6 + 5]};
+    const Vec3 w{s.v[i * 6 + 3], s.v[i * 6 + 4], s.v[i * 6 + 5]};
+    const Quat r0{e.x0[3], e.x0[4], e.x0[5], e.x0[6]};
+    const Quat r{s.q[i * 7 + 3], s.q[i * 7 + 4], s.q[i * 7 + 5], s.q[i * 7 + 6]};
+    const Quat xr = QuaternionCompose(r, r0);
+    const Mat3 wt = VecTilde(w), wdt = VecTilde(wd);
+    const Mat6 Muu = RotateSectionMatrix(xr, e.Mstar);
+    const Vec3 eta = CalculateEta(Muu);
+    const Mat3 rho = CalculateRho(Muu);
+    const double m = Muu[0];
+    const Mat3 et = VecTilde(eta);
+    const auto Fg = CalculateGravityForce(m, s.gravity, et);
+    const auto Fi = CalculateInertialForce(m, ud, w, wd, eta, et, rho, wt, wdt);
+    for (int c = 0; c < 6; ++c) e.residual[c] = Fi[c] - Fg[c];
+    const Mat6 Guu = CalculateGyroscopicMatrix(m, w, eta, rho, wt);
+    const Mat6 Kuu = CalculateInertiaStiffnessMatrix(m, ud, w, wd, eta, rho, wt, wdt);
+    const double* T = &s.tangent[i * 36];
+    for (int a = 0; a < 6; ++a)
+        for (int b = 0; b < 6; ++b) {
+            const double v = beta_prime * Muu[a * 6 + b] + gamma_prime * Guu[a * 6 + b];
+            // Gemm(1, Kuu, T, 1, STpI): inner product first, then added
+            double kt = 0.;
+            for (int k = 0; k < 6; ++k) kt += Kuu[a * 6 + k] * T[k * 6 + b];
+            e.sysmat[a * 6 + b] = v + kt;
+        }
+}
+// src/system/springs/calculate_quadrature_point_values.hpp:28-90 and the springs/*.hpp helpers
+void CalculateSpringValues(const System& s, SpringElem& e) {
+    Vec3 r{};
+    for (int c = 0; c < 3; ++c) r[c] = e.x0[c] + s.q[e.n2 * 7 + c] - s.q[e.n1 * 7 + c];
+    const double l = std::sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+    const double c1 = e.k * (e.l_ref / l - 1.);
+    const double c2 = e.k * e.l_ref / (l * l * l);
+    const Mat3 rt = VecTilde(r);
+    Mat3 a{};
+    const double diag = c1 - c2 * l * l;
+    const Mat3 rr = mm(rt, rt);
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) a[i * 3 + j] = -c2 * rr[i * 3 + j] + (i == j ? diag : 0.);
+    for (int c = 0; c < 3; ++c) {
+        e.residual[c] = c1 * r[c];
+        e.residual[3 + c] = -(c1 * r[c]);
+    }
+    for (int n1 = 0; n1 < 2; ++n1)
+        for (int n2 = 0; n2 < 2; ++n2) {
+            const double sg = (n1 == n2) ? 1. : -1.;
+            for (int k = 0; k < 9; ++k) e.stiff[(n1 * 2 + n2) * 9 + k] = sg * a[k];
+        }
+}
+void UpdateSystemVariablesMasses(const StepParameters& p, System& s) {
+    for (auto& e : s.masses) CalculateMassValues(p.beta_prime, p.gamma_prime, s, e);
+}
+void UpdateSystemVariablesSprings(System& s) {
+    for (auto& e : s.springs) CalculateSpringValues(s, e);
+}
+
+// src/step/assemble_system_residual.hpp:17-64 (forces, then beams, masses, springs)
 void AssembleSystemResidual(System& s) {
     // contribute_forces_to_vector.hpp:27-48
     for (size_t i = 0; i < s.num_nodes; ++i) {
@@ -944,6 +1030,19 @@ void AssembleSystemResidual(System& s) {
             const size_t d = s.dof_map[e.node_idx[k]];
             for (int c = 0; c < 6; ++c) s.b[d + c] += e.residual[k * 6 + c];
         }
+    // contribute_masses_to_vector.hpp
+    for (const auto& e : s.masses) {
+        const size_t d = s.dof_map[e.node];
+        for (int c = 0; c < 6; ++c) s.b[d + c] += e.residual[c];
+    }
+    // contribute_springs_to_vector.hpp (translational dofs of both nodes)
+    for (const auto& e : s.springs) {
+        const size_t d1 = s.dof_map[e.n1], d2 = s.dof_map[e.n2];
+        for (int c = 0; c < 3; ++c) {
+            s.b[d1 + c] += e.residual[c];
+            s.b[d2 + c] += e.residual[3 + c];
+        }
+    }
 }
 // src/step/assemble_system_matrix.hpp:16-55, contribute_beams_to_sparse_matrix.hpp:34-73
 void AssembleSystemMatrix(const StepParameters& p, System& s) {
@@ -955,6 +1054,21 @@ void AssembleSystemMatrix(const StepParameters& p, System& s) {
                 for (int r = 0; r < 6; ++r)
                     for (int c = 0; c < 6; ++c) Aentry(s, ri + r, cj + c) += S[r * 6 + c] * p.conditioner;
             }
+    // contribute_masses_to_sparse_matrix.hpp
+    for (const auto& e : s.masses) {
+        const size_t d = s.dof_map[e.node];
+        for (int r = 0; r < 6; ++r)
+            for (int c = 0; c < 6; ++c) Aentry(s, d + r, d + c) += e.sysmat[r * 6 + c] * p.conditioner;
+    }
+    // contribute_springs_to_sparse_matrix.hpp:34-80
+    for (const auto& e : s.springs) {
+        const size_t d[2] = {s.dof_map[e.n1], s.dof_map[e.n2]};
+        for (int n1 = 0; n1 < 2; ++n1)
+            for (int n2 = 0; n2 < 2; ++n2)
+                for (int r = 0; r < 3; ++r)
+                    for (int c = 0; c < 3; ++c)
+                        Aentry(s, d[n1] + r, d[n2] + c) += e.stiff[(n1 * 2 + n2) * 9 + r * 3 + c] * p.conditioner;
+    }
 }
 // src/step/update_constraint_variables.hpp:13-37 and
 // src/constraints/calculate_constraint_residual_gradient.hpp:61-565
@@ -1213,6 +1327,8 @@ bool Step(const StepParameters& p, System& s) {
         ResetSolver(s);
         UpdateTangentOperator(p, s);
         UpdateSystemVariablesBeams(p, s);
+        UpdateSystemVariablesMasses(p, s);
+        UpdateSystemVariablesSprings(s);
         AssembleSystemResidual(s);
         AssembleSystemMatrix(p, s);
         UpdateConstraintVariables(s);

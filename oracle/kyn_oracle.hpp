@@ -108,9 +108,19 @@ struct ConstraintIn {
     Vec3 axis{0., 0., 0.};
     std::array<double, 7> init{0., 0., 0., 1., 0., 0., 0.};
 };
+struct MassIn {  // src/elements/masses/mass_element.hpp
+    size_t node;
+    Mat6 M;
+};
+struct SpringIn {  // src/elements/springs/spring_element.hpp
+    size_t n1, n2;
+    double k, l0;
+};
 struct ModelIn {
     std::vector<NodeIn> nodes;
     std::vector<BeamIn> beams;
+    std::vector<MassIn> masses;
+    std::vector<SpringIn> springs;
     std::vector<ConstraintIn> constraints;
     Vec3 gravity{0., 0., 0.};
 };
@@ -140,6 +150,21 @@ struct BeamElem {
     std::vector<double> node_FX;                  // [N][6]
     std::vector<double> residual;                 // [N][6]
     std::vector<double> sysmat;                   // [N][N][36]
+};
+
+struct MassElem {  // src/elements/masses/masses.hpp
+    size_t node{};
+    Mat6 Mstar{};
+    std::array<double, 7> x0{};
+    std::array<double, 6> residual{};
+    Mat6 sysmat{};
+};
+struct SpringElem {  // src/elements/springs/springs.hpp
+    size_t n1{}, n2{};
+    Vec3 x0{};  // node 2 x0 - node 1 x0
+    double l_ref{}, k{};
+    std::array<double, 6> residual{};  // [2][3]
+    std::array<double, 36> stiff{};    // [2][2][3][3]
 };
 
 struct Constraint {
@@ -174,6 +199,8 @@ struct System {
     std::vector<size_t> active_dofs, dof_map;
     Vec3 gravity{};
     std::vector<BeamElem> beams;
+    std::vector<MassElem> masses;
+    std::vector<SpringElem> springs;
     std::vector<Constraint> cons;
     size_t num_system_dofs{}, num_dofs{};
     std::vector<double> b, xsol;                 // [num_dofs]
@@ -190,6 +217,8 @@ void ResetConstraints(System& s);
 void ResetSolver(System& s);
 void UpdateTangentOperator(const StepParameters& p, System& s);
 void UpdateSystemVariablesBeams(const StepParameters& p, System& s);
+void UpdateSystemVariablesMasses(const StepParameters& p, System& s);
+void UpdateSystemVariablesSprings(System& s);
 void AssembleSystemResidual(System& s);
 void AssembleSystemMatrix(const StepParameters& p, System& s);
 void UpdateConstraintVariables(System& s);
@@ -206,6 +235,11 @@ bool Step(const StepParameters& p, System& s);
 // (src/system/beams/calculate_quadrature_point_values.hpp:50-172)
 void CalculateQuadraturePointValues(double beta_prime, double gamma_prime, const System& s,
                                     BeamElem& e);
+
+// Point mass and spring element evaluation (src/system/masses/calculate_quadrature_point_values.hpp,
+// src/system/springs/calculate_quadrature_point_values.hpp)
+void CalculateMassValues(double beta_prime, double gamma_prime, const System& s, MassElem& e);
+void CalculateSpringValues(const System& s, SpringElem& e);
 
 // Dense LU with partial pivoting (replaces the reference's dss:: sparse direct solve;
 // KLU's pivot order is not pinned by the reference, any correct pivoted LU is admissible)

@@ -69,6 +69,21 @@ long orc_model_add_constraint(void* m, int type, size_t base, size_t target, con
     return static_cast<long>(mo->constraints.size() - 1);
 }
 
+long orc_model_add_mass_element(void* m, size_t node, const double* M36) {
+    auto* mo = static_cast<ModelIn*>(m);
+    if (node >= mo->nodes.size()) return -1;
+    MassIn e{node, {}};
+    std::copy(M36, M36 + 36, e.M.begin());
+    mo->masses.push_back(e);
+    return static_cast<long>(mo->masses.size() - 1);
+}
+long orc_model_add_spring_element(void* m, size_t n1, size_t n2, double k, double l0) {
+    auto* mo = static_cast<ModelIn*>(m);
+    if (n1 >= mo->nodes.size() || n2 >= mo->nodes.size()) return -1;
+    mo->springs.push_back(SpringIn{n1, n2, k, l0});
+    return static_cast<long>(mo->springs.size() - 1);
+}
+
 void* orc_system_create(void* m, int is_dynamic, size_t max_iter, double h, double rho_inf,
                         double atol, double rtol) {
     try {
@@ -103,6 +118,8 @@ void orc_debug_assemble(void* s) {
     ResetSolver(o->sys);
     UpdateTangentOperator(o->params, o->sys);
     UpdateSystemVariablesBeams(o->params, o->sys);
+    UpdateSystemVariablesMasses(o->params, o->sys);
+    UpdateSystemVariablesSprings(o->sys);
     AssembleSystemMatrix(o->params, o->sys);
     AssembleSystemResidual(o->sys);
     UpdateConstraintVariables(o->sys);

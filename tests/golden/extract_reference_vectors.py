@@ -13,6 +13,9 @@ Sources (paths relative to the reference root):
   tests/regression_tests/regression/iea15_rotor_data.hpp    -> iea15_rotor.json
   tests/unit_tests/system/beams/test_calculate_strain.cpp   -> functor_known_answers.json
   tests/unit_tests/system/masses/*.cpp                      -> functor_known_answers.json
+  tests/regression_tests/regression/test_masses.cpp,
+  tests/regression_tests/regression/test_precession.cpp,
+  tests/regression_tests/regression/test_spring_mass_system.cpp -> masses_springs.json
 
 Usage: python tests/golden/extract_reference_vectors.py [/root/reference]
 """
@@ -199,12 +202,80 @@ def functor_known_answers(root):
     return out
 
 
+def near_values(body, pattern):
+    """EXPECT_NEAR(<expr matching pattern>, value, tol) -> [(expr, value, tol)], in order."""
+    out = []
+    for m in re.finditer(r"EXPECT_NEAR\s*\(", body):
+        args = split_top(balanced(body, m.end() - 1))
+        expr = re.sub(r"\s+", "", args[0])
+        if re.search(pattern, expr):
+            out.append({"expr": expr, "value": numbers(args[1])[0], "tol": numbers(args[2])[0]})
+    return out
+
+
+def symbolic_matrix(text, body):
+    """Entries of a brace-initialized matrix whose entries are literals or named constants
+    (`const auto m = 1.;` / `constexpr auto j = 1.;` in the same body)."""
+    consts = {m.group(1): float(m.group(2)) for m in
+              re.finditer(r"(?:const|constexpr) auto (\w+) = (-?[\d.]+(?:[eE][-+]?\d+)?);", body)}
+    vals = []
+    for tok in re.findall(r"[A-Za-z_]\w*|-?(?:\d+\.\d*|\.\d+|\d+)(?:[eE][-+]?\d+)?", text):
+        vals.append(consts[tok] if tok in consts else float(tok))
+    return vals
+
+
+def masses_springs(root):
+    reg = root / "tests/regression_tests/regression"
+    masses = strip_comments((reg / "test_masses.cpp").read_text())
+    body = masses[masses.index("MassesTest, ExternalForce") :]
+    ext = {
+        "source": "tests/regression_tests/regression/test_masses.cpp (MassesTest.ExternalForce)",
+        "mass_matrix": symbolic_matrix(balanced(body, body.index("{", body.index("model.AddMassElement"))), body),
+        "time_step": numbers(re.search(r"time_step = ([^;]+);", body).group(1))[0],
+        "step_parameters": numbers(re.search(r"StepParameters\(true, ([^)]*)\)", body).group(1)),
+        "force_x": numbers(re.search(r"force_x = ([^;]+);", body).group(1))[0],
+        "torque_y": numbers(re.search(r"torque_y = ([^;]+);", body).group(1))[0],
+        "n_steps": numbers(re.search(r"n_steps = ([^;]+);", body).group(1))[0],
+        "expect_v": near_values(body, r"^v_host"),
+    }
+    prec = strip_comments((reg / "test_precession.cpp").read_text())
+    pbody = function_body(prec, "SetUpPrecessionTest")
+    precession = {
+        "source": "tests/regression_tests/regression/test_precession.cpp",
+        "velocity": numbers(re.search(r"SetVelocity\(([^)]*)\)", pbody).group(1)),
+        "mass_matrix": symbolic_matrix(balanced(pbody, pbody.index("{", pbody.index("model.AddMassElement"))), pbody),
+        "max_iter": numbers(re.search(r"max_iter\((\d+)\)", pbody).group(1))[0],
+        "step_size": numbers(re.search(r"step_size\(([^)]+)\)", pbody).group(1))[0],
+        "rho_inf": numbers(re.search(r"rho_inf\(([^)]+)\)", pbody).group(1))[0],
+        "num_steps": numbers(re.search(r"iota\(0, (\d+)\)", pbody).group(1))[0],
+        "expect_q": near_values(pbody, r"^q_host"),
+    }
+    sm = strip_comments((reg / "test_spring_mass_system.cpp").read_text())
+    sbody = sm[sm.index("SetUpSpringMassSystem") : sm.index("SetUpSpringMassChainSystem")]
+    positions = [numbers(m.group(1)) for m in re.finditer(r"SetPosition\(([^)]*)\)", sbody)]
+    spring_mass = {
+        "source": "tests/regression_tests/regression/test_spring_mass_system.cpp (SpringMassSystemTest)",
+        "positions": positions,
+        "m": numbers(re.search(r"constexpr auto m = ([^;]+);", sbody).group(1))[0],
+        "j": numbers(re.search(r"constexpr auto j = ([^;]+);", sbody).group(1))[0],
+        "k": numbers(re.search(r"const auto k = ([^;]+);", sbody).group(1))[0],
+        "l0": numbers(re.search(r"const auto l0 = ([^;]+);", sbody).group(1))[0],
+        "num_steps": numbers(re.search(r"num_steps = ([^;]+);", sbody).group(1))[0],
+        "max_iter": numbers(re.search(r"max_iter\((\d+)\)", sbody).group(1))[0],
+        "rho_inf": numbers(re.search(r"rho_inf\(([^)]+)\)", sbody).group(1))[0],
+        "expect_half": near_values(sbody[: sbody.index("time_step == num_steps)")], r"^q\(1,0\)"),
+        "expect_end": near_values(sbody[sbody.index("time_step == num_steps)") :], r"^q\(1,0\)"),
+    }
+    return {"masses_external_force": ext, "precession": precession, "spring_mass": spring_mass}
+
+
 def main():
     root = Path(sys.argv[1] if len(sys.argv) > 1 else "/root/reference")
     (HERE / "solver_regression.json").write_text(json.dumps(solver_regression(root), indent=1))
     (HERE / "cantilever_sine.json").write_text(json.dumps(cantilever(root), indent=1))
     (HERE / "iea15_rotor.json").write_text(json.dumps(iea15(root)))
     (HERE / "functor_known_answers.json").write_text(json.dumps(functor_known_answers(root), indent=1))
+    (HERE / "masses_springs.json").write_text(json.dumps(masses_springs(root), indent=1))
     print("wrote fixtures to", HERE)
 
 

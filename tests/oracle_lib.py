@@ -47,6 +47,10 @@ class OracleLib:
         L.orc_model_add_beam_element.restype = C.c_long
         L.orc_model_add_constraint.argtypes = [C.c_void_p, C.c_int, C.c_size_t, C.c_size_t, _dp, _dp]
         L.orc_model_add_constraint.restype = C.c_long
+        L.orc_model_add_mass_element.argtypes = [C.c_void_p, C.c_size_t, _dp]
+        L.orc_model_add_mass_element.restype = C.c_long
+        L.orc_model_add_spring_element.argtypes = [C.c_void_p, C.c_size_t, C.c_size_t, C.c_double, C.c_double]
+        L.orc_model_add_spring_element.restype = C.c_long
         L.orc_system_create.argtypes = [C.c_void_p, C.c_int, C.c_size_t, C.c_double, C.c_double,
                                         C.c_double, C.c_double]
         L.orc_system_create.restype = C.c_void_p
@@ -101,6 +105,13 @@ class OracleLib:
         a4, p4 = _d(quad)
         return self.L.orc_model_add_beam_element(m, len(nodes), nodes.ctypes.data_as(_sp), len(a1), p1, p2, p3,
                                                  a4.size // 2, p4)
+
+    def model_add_mass_element(self, m, node, M):
+        a, p = _d(M)
+        return self.L.orc_model_add_mass_element(m, int(node), p)
+
+    def model_add_spring_element(self, m, n1, n2, k, l0):
+        return self.L.orc_model_add_spring_element(m, int(n1), int(n2), float(k), float(l0))
 
     def model_add_constraint(self, m, ctype, base, target, axis, init):
         a1, p1 = _d(axis)
