@@ -97,6 +97,15 @@ int64_t kyn_model_add_beam_element(kyn_model m, size_t num_nodes, const size_t* 
                                    size_t num_sections, const double* section_s,
                                    const double* section_mass, const double* section_stiffness,
                                    size_t num_qps, const double* qp_xi_w);
+/* Model::AddMassElement (model.hpp:305-311): 6x6 mass matrix in the node's reference frame,
+ * row-major.  Returns the mass element id.  Plain form only (the fused form rejects models
+ * that hold masses or springs with KYN_ERR_UNSUPPORTED). */
+int64_t kyn_model_add_mass_element(kyn_model m, size_t node, const double mass_matrix[36]);
+/* Model::AddSpringElement (model.hpp:376-388): linear spring between two nodes' translational
+ * dofs.  Returns the spring element id.  Plain form only; each spring node must also carry a
+ * beam, a mass or a constraint (six dofs), else kyn_system_create returns KYN_ERR_UNSUPPORTED. */
+int64_t kyn_model_add_spring_element(kyn_model m, size_t node1, size_t node2, double stiffness,
+                                     double undeformed_length);
 /* Model::AddFixedBC / AddPrescribedBC / AddRigidJointConstraint / AddRevoluteJointConstraint
  * (model.hpp:477-541). For one-node constraints `base` is ignored. axis and init may be NULL. */
 int64_t kyn_model_add_constraint(kyn_model m, int type, size_t base, size_t target,

@@ -74,6 +74,10 @@ class KynLib:
         L.kyn_model_add_beam_element.restype = C.c_int64
         L.kyn_model_add_constraint.argtypes = [C.c_void_p, C.c_int, C.c_size_t, C.c_size_t, _dp, _dp]
         L.kyn_model_add_constraint.restype = C.c_int64
+        L.kyn_model_add_mass_element.argtypes = [C.c_void_p, C.c_size_t, _dp]
+        L.kyn_model_add_mass_element.restype = C.c_int64
+        L.kyn_model_add_spring_element.argtypes = [C.c_void_p, C.c_size_t, C.c_size_t, C.c_double, C.c_double]
+        L.kyn_model_add_spring_element.restype = C.c_int64
         L.kyn_system_create.argtypes = [C.POINTER(C.c_void_p), C.c_size_t, C.POINTER(StepParams),
                                         C.POINTER(ExecOptions)]
         L.kyn_system_create.restype = C.c_void_p
@@ -155,6 +159,13 @@ class KynLib:
         a1, p1 = _d(axis)
         a2, p2 = _d(init)
         return self.L.kyn_model_add_constraint(m, int(ctype), int(base), int(target), p1, p2)
+
+    def model_add_mass_element(self, m, node, mass_matrix):
+        a, p = _d(np.asarray(mass_matrix, dtype=np.float64).reshape(36))
+        return self.L.kyn_model_add_mass_element(m, int(node), p)
+
+    def model_add_spring_element(self, m, n1, n2, k, l0):
+        return self.L.kyn_model_add_spring_element(m, int(n1), int(n2), float(k), float(l0))
 
 
 class KynSystem:

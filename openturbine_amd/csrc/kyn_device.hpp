@@ -152,15 +152,12 @@ KYN_DEV void calc_displacement(double h, const double* qd, const double* qp, dou
 // ---------------------------------------------------------------------------------------
 // Results go out as soon as they are final (keeps few 6x6 blocks live): F[12..18) = Fi,
 // F[18..24) = Fg, M6[0..36) = wJ Kuu, M6[72..108) = wJ (beta' Muu + gamma' Guu).
+// `r` is the (already normalized) rotation the section matrix is carried by.
 template <typename TT>
-KYN_DEV void qp_inertial(const double* r_in /*4, unnormalized*/, const double* ud,
-                         const double* w, const double* wd, const double* r0, const double* Ms,
-                         const double* g, double wJ, double bp, double gp, TT* M6, double* F) {
+KYN_DEV void inertial_terms(const double* r, const double* ud, const double* w, const double* wd,
+                            const double* r0, const double* Ms, const double* g, double wJ, double bp,
+                            double gp, TT* M6, double* F) {
     double Muu[36], Fi[6], Fg[6];
-    double r[4];
-    const double rl = sqrt(r_in[0] * r_in[0] + r_in[1] * r_in[1] + r_in[2] * r_in[2] + r_in[3] * r_in[3]);
-#pragma unroll
-    for (int c = 0; c < 4; ++c) r[c] = r_in[c] / rl;
     double xr[4], R[9];
     quat_compose(r, r0, xr);
     quat_to_rot(xr, R);
@@ -271,6 +268,70 @@ KYN_DEV void qp_inertial(const double* r_in /*4, unnormalized*/, const double* u
             }
 #pragma unroll
         for (int c = 0; c < 36; ++c) M6[c] = (TT)(wJ * Kuu[c]);
+    }
+}
+
+// Beam QP (src/system/beams/calculate_quadrature_point_values.hpp): the interpolated QP rotation
+// is normalized before use (interpolate_QP_rotation.hpp)
+template <typename TT>
+KYN_DEV void qp_inertial(const double* r_in /*4, unnormalized*/, const double* ud,
+                         const double* w, const double* wd, const double* r0, const double* Ms,
+                         const double* g, double wJ, double bp, double gp, TT* M6, double* F) {
+    double r[4];
+    const double rl = sqrt(r_in[0] * r_in[0] + r_in[1] * r_in[1] + r_in[2] * r_in[2] + r_in[3] * r_in[3]);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) r[c] = r_in[c] / rl;
+    inertial_terms<TT>(r, ud, w, wd, r0, Ms, g, wJ, bp, gp, M6, F);
+}
+
+// Point mass (src/system/masses/calculate_quadrature_point_values.hpp:41-152): the node's own
+// rotation q[3..7) composed with its reference rotation, no normalization; residual Fi - Fg,
+// matrix beta' Muu + gamma' Guu + Kuu T with T = blkdiag(I, trr) the node's stored tangent
+// (Gemm(1, Kuu, T, 1, STpI): the product is formed first, then added)
+KYN_DEV void mass_terms(const double* r, const double* ud, const double* w, const double* wd,
+                        const double* r0, const double* Ms, const double* g, double bp, double gp,
+                        const double* trr, double* res, double* mat) {
+    double M6[108], F[24];
+    inertial_terms<double>(r, ud, w, wd, r0, Ms, g, 1., bp, gp, M6, F);
+#pragma unroll
+    for (int c = 0; c < 6; ++c) res[c] = F[12 + c] - F[18 + c];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+#pragma unroll
+        for (int j = 0; j < 3; ++j) mat[i * 6 + j] = M6[72 + i * 6 + j] + M6[i * 6 + j];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const double kt = M6[i * 6 + 3] * trr[j] + M6[i * 6 + 4] * trr[3 + j] + M6[i * 6 + 5] * trr[6 + j];
+            mat[i * 6 + 3 + j] = M6[72 + i * 6 + 3 + j] + kt;
+        }
+    }
+}
+
+// Spring (src/system/springs/calculate_quadrature_point_values.hpp:28-90 with
+// calculate_distance_components.hpp, calculate_length.hpp, calculate_force_coefficients.hpp,
+// calculate_force_vectors.hpp, calculate_stiffness_matrix.hpp): r = x0 + u2 - u1, l = |r|,
+// c1 = k (l_ref / l - 1), c2 = k l_ref / l^3, f = c1 r, a = (c1 - c2 l^2) I - c2 r~ r~;
+// residual [f, -f]; the 2 x 2 node blocks are [a, -a; -a, a]
+KYN_DEV void spring_terms(const double* x0, const double* u1, const double* u2, double l_ref, double k,
+                          double* res, double* a) {
+    double r[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) r[c] = x0[c] + u2[c] - u1[c];
+    const double l = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+    const double c1 = k * (l_ref / l - 1.);
+    const double c2 = k * l_ref / (l * l * l);
+    double rt[9], rr[9];
+    tilde(r, rt);
+    mm3(rt, rt, rr);
+    const double diag = c1 - c2 * l * l;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) a[i * 3 + j] = -c2 * rr[i * 3 + j] + (i == j ? diag : 0.);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        res[c] = c1 * r[c];
+        res[3 + c] = -(c1 * r[c]);
     }
 }
 

@@ -64,6 +64,14 @@ struct HBeam {
     std::vector<HSection> sections;
     std::vector<std::array<double, 2>> quad;
 };
+struct HMass {  // Model::AddMassElement (model.hpp:305)
+    size_t node;
+    std::array<double, 36> M;
+};
+struct HSpring {  // Model::AddSpringElement (model.hpp:376)
+    size_t n1, n2;
+    double k, l0;
+};
 struct HConstraint {
     int type;
     size_t base, target;
@@ -76,6 +84,8 @@ struct HConstraint {
 struct kyn_model_s {
     std::vector<HNode> nodes;
     std::vector<HBeam> beams;
+    std::vector<HMass> masses;
+    std::vector<HSpring> springs;
     std::vector<HConstraint> constraints;
     std::array<double, 3> gravity{0., 0., 0.};
 };
@@ -229,6 +239,9 @@ struct Builder {
     std::vector<double> qp_weight, qp_jac, qp_Mstar, qp_Cstar, qp_x0p, qp_r0, shape_interp, shape_deriv;
     std::vector<int> con_type, con_base, con_target, con_rows, con_model_row, con_local, con_block;
     std::vector<double> con_X0, con_axes, con_input;
+    std::vector<int> block_mass_begin, block_mass_end, block_masses, block_spring_begin, block_spring_end,
+        block_springs, mass_node, spring_n1, spring_n2;
+    std::vector<double> mass_Mstar, mass_r0, spring_x0, spring_lref, spring_k;
     std::map<std::string, int> shape_tables;
     std::vector<std::vector<int>> gdof_maps;
     std::vector<int> model_nsys;
@@ -242,6 +255,8 @@ void add_model(Builder& B, const kyn_model_s& mo) {
     const int con0 = static_cast<int>(B.con_type.size());
     const int elem0 = static_cast<int>(B.elem_N.size());
     const int blk0 = static_cast<int>(B.block_n.size());
+    const int mass0 = static_cast<int>(B.mass_node.size());
+    const int spring0 = static_cast<int>(B.spring_n1.size());
     const size_t n = mo.nodes.size();
     const int mid = static_cast<int>(B.model_node_begin.size());
 
@@ -278,14 +293,29 @@ void add_model(Builder& B, const kyn_model_s& mo) {
             throw KynError(KYN_ERR_INVALID, "constraint references an unknown node");
     }
 
+    for (const auto& ms : mo.masses)
+        if (ms.node >= n) throw KynError(KYN_ERR_INVALID, "mass element references an unknown node");
+    for (const auto& sp : mo.springs) {
+        if (sp.n1 >= n || sp.n2 >= n) throw KynError(KYN_ERR_INVALID, "spring element references an unknown node");
+        if (sp.n1 == sp.n2) throw KynError(KYN_ERR_INVALID, "spring element joins a node to itself");
+    }
+
     // ---- DOF tables (assemble_node_freedom_allocation_table.hpp, compute_node_freedom_map_table.hpp)
     std::vector<int> has(n, 0);
     for (const auto& bm : mo.beams)
         for (auto nid : bm.nodes) has[nid] = 1;
+    for (const auto& ms : mo.masses) has[ms.node] = 1;
     for (const auto& c : mo.constraints) {
         has[c.target] = 1;
         if (c.type == 3 || c.type == 4) has[c.base] = 1;
     }
+    // a spring alone gives its nodes the three translational dofs (FreedomSignature::JustPosition);
+    // the device state and block layout carry six dofs per node, so spring nodes must also belong
+    // to a beam, a mass or a constraint
+    for (const auto& sp : mo.springs)
+        if (!has[sp.n1] || !has[sp.n2])
+            throw KynError(KYN_ERR_UNSUPPORTED,
+                           "spring nodes need all six dofs (a beam, mass or constraint on each spring node)");
     std::vector<int> dof_map(n, 0);
     int nsys = 0;
     for (size_t i = 0; i < n; ++i) {
@@ -312,8 +342,9 @@ void add_model(Builder& B, const kyn_model_s& mo) {
         for (size_t k = 1; k < bm.nodes.size(); ++k) unite(bm.nodes[0], bm.nodes[k]);
     for (const auto& c : mo.constraints)
         if (c.type == 3 || c.type == 4) unite(c.base, c.target);
+    for (const auto& sp : mo.springs) unite(sp.n1, sp.n2);
     std::map<size_t, int> root_block;
-    std::vector<std::vector<int>> bnodes, bcons, belems;
+    std::vector<std::vector<int>> bnodes, bcons, belems, bmasses, bsprings;
     for (size_t i = 0; i < n; ++i) {
         if (!has[i]) continue;
         const size_t r = uf_root(parent, i);
@@ -322,6 +353,8 @@ void add_model(Builder& B, const kyn_model_s& mo) {
             bnodes.emplace_back();
             bcons.emplace_back();
             belems.emplace_back();
+            bmasses.emplace_back();
+            bsprings.emplace_back();
         }
         bnodes[root_block[r]].push_back(static_cast<int>(i));
     }
@@ -329,6 +362,10 @@ void add_model(Builder& B, const kyn_model_s& mo) {
         bcons[root_block[uf_root(parent, mo.constraints[k].target)]].push_back(static_cast<int>(k));
     for (size_t k = 0; k < mo.beams.size(); ++k)
         belems[root_block[uf_root(parent, mo.beams[k].nodes[0])]].push_back(static_cast<int>(k));
+    for (size_t k = 0; k < mo.masses.size(); ++k)
+        bmasses[root_block[uf_root(parent, mo.masses[k].node)]].push_back(static_cast<int>(k));
+    for (size_t k = 0; k < mo.springs.size(); ++k)
+        bsprings[root_block[uf_root(parent, mo.springs[k].n1)]].push_back(static_cast<int>(k));
 
     std::vector<int> gmap(ndofs, -1);  // model dof -> rhs index
     std::vector<int> con_loc(mo.constraints.size(), 0), con_blk(mo.constraints.size(), 0);
@@ -358,6 +395,12 @@ void add_model(Builder& B, const kyn_model_s& mo) {
         B.block_elem_begin.push_back(static_cast<int>(B.block_elems.size()));
         for (int ek : belems[bi]) B.block_elems.push_back(elem0 + ek);
         B.block_elem_end.push_back(static_cast<int>(B.block_elems.size()));
+        B.block_mass_begin.push_back(static_cast<int>(B.block_masses.size()));
+        for (int ek : bmasses[bi]) B.block_masses.push_back(mass0 + ek);
+        B.block_mass_end.push_back(static_cast<int>(B.block_masses.size()));
+        B.block_spring_begin.push_back(static_cast<int>(B.block_springs.size()));
+        for (int ek : bsprings[bi]) B.block_springs.push_back(spring0 + ek);
+        B.block_spring_end.push_back(static_cast<int>(B.block_springs.size()));
         B.block_model.push_back(mid);
         B.block_n.push_back(l);
         B.block_nsys.push_back(bnsys);
@@ -429,6 +472,20 @@ void add_model(Builder& B, const kyn_model_s& mo) {
             for (double c : d) B.qp_x0p.push_back(c / J);
             for (double c : r) B.qp_r0.push_back(c);
         }
+    }
+
+    // ---- masses (create_masses.hpp:24-60) and springs (create_springs.hpp:24-58)
+    for (const auto& ms : mo.masses) {
+        B.mass_node.push_back(node0 + static_cast<int>(ms.node));
+        B.mass_Mstar.insert(B.mass_Mstar.end(), ms.M.begin(), ms.M.end());
+        for (int c = 0; c < 4; ++c) B.mass_r0.push_back(mo.nodes[ms.node].x0[3 + c]);
+    }
+    for (const auto& sp : mo.springs) {
+        B.spring_n1.push_back(node0 + static_cast<int>(sp.n1));
+        B.spring_n2.push_back(node0 + static_cast<int>(sp.n2));
+        for (int c = 0; c < 3; ++c) B.spring_x0.push_back(mo.nodes[sp.n2].x0[c] - mo.nodes[sp.n1].x0[c]);
+        B.spring_lref.push_back(sp.l0);
+        B.spring_k.push_back(sp.k);
     }
 
     // ---- constraints (constraints.hpp:76-266)
@@ -635,6 +692,20 @@ int64_t kyn_model_add_beam_element(kyn_model m, size_t num_nodes, const size_t* 
     m->beams.push_back(std::move(b));
     return static_cast<int64_t>(m->beams.size() - 1);
 }
+int64_t kyn_model_add_mass_element(kyn_model m, size_t node, const double mass_matrix[36]) {
+    if (!m || !mass_matrix) return KYN_ERR_INVALID;
+    HMass e{};
+    e.node = node;
+    std::copy(mass_matrix, mass_matrix + 36, e.M.begin());
+    m->masses.push_back(e);
+    return static_cast<int64_t>(m->masses.size() - 1);
+}
+int64_t kyn_model_add_spring_element(kyn_model m, size_t node1, size_t node2, double stiffness,
+                                     double undeformed_length) {
+    if (!m) return KYN_ERR_INVALID;
+    m->springs.push_back(HSpring{node1, node2, stiffness, undeformed_length});
+    return static_cast<int64_t>(m->springs.size() - 1);
+}
 int64_t kyn_model_add_constraint(kyn_model m, int type, size_t base, size_t target, const double axis[3],
                                  const double init_disp[7]) {
     if (!m) return KYN_ERR_INVALID;
@@ -706,7 +777,12 @@ kyn_system kyn_system_create(const kyn_model* models, size_t num_models, const k
         int max_lds = 0;
         hip_check(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev), "attr");
         size_t need = 0;
+        S.num_masses = static_cast<int>(B.mass_node.size());
+        S.num_springs = static_cast<int>(B.spring_n1.size());
         if (sys->exec_mode == KYN_EXEC_FUSED) {
+            if (S.num_masses + S.num_springs > 0)
+                throw KynError(KYN_ERR_UNSUPPORTED, "the fused form takes beam elements only; use the plain form "
+                                                    "for point masses and springs");
             for (int b = 0; b < S.num_blocks; ++b) {
                 if (B.block_elem_end[b] - B.block_elem_begin[b] != 1)
                     throw KynError(KYN_ERR_UNSUPPORTED, "the fused form needs exactly one beam element per independent block");
@@ -722,9 +798,6 @@ kyn_system kyn_system_create(const kyn_model* models, size_t num_models, const k
             throw KynError(KYN_ERR_UNSUPPORTED, "problem needs " + std::to_string(need) + " bytes of LDS, device offers " +
                                                     std::to_string(max_lds));
         hip_check(kyn::set_lds_limits(static_cast<size_t>(max_lds)), "hipFuncSetAttribute");
-        for (int b = 0; b < S.num_blocks; ++b)
-            if (B.block_n[b] > 0 && B.block_elem_end[b] == B.block_elem_begin[b])
-                throw KynError(KYN_ERR_UNSUPPORTED, "independent block without beam elements");
 
         // ---- upload
         auto& Ar = sys->arena;
@@ -798,6 +871,24 @@ kyn_system kyn_system_create(const kyn_model* models, size_t num_models, const k
         S.elem_mat = sys->exec_mode == KYN_EXEC_PLAIN ? Ar.zeros<double>(static_cast<size_t>(B.mat_total)) : nullptr;
         sys->mat_total = B.mat_total;
         sys->A_total = B.A_total;
+        S.block_mass_begin = Ar.up(B.block_mass_begin);
+        S.block_mass_end = Ar.up(B.block_mass_end);
+        S.block_masses = Ar.up(B.block_masses);
+        S.block_spring_begin = Ar.up(B.block_spring_begin);
+        S.block_spring_end = Ar.up(B.block_spring_end);
+        S.block_springs = Ar.up(B.block_springs);
+        S.mass_node = Ar.up(B.mass_node);
+        S.mass_Mstar = Ar.up(B.mass_Mstar);
+        S.mass_r0 = Ar.up(B.mass_r0);
+        S.mass_res = Ar.zeros<double>(B.mass_node.size() * 6);
+        S.mass_mat = Ar.zeros<double>(B.mass_node.size() * 36);
+        S.spring_n1 = Ar.up(B.spring_n1);
+        S.spring_n2 = Ar.up(B.spring_n2);
+        S.spring_x0 = Ar.up(B.spring_x0);
+        S.spring_lref = Ar.up(B.spring_lref);
+        S.spring_k = Ar.up(B.spring_k);
+        S.spring_res = Ar.zeros<double>(B.spring_n1.size() * 6);
+        S.spring_mat = Ar.zeros<double>(B.spring_n1.size() * 9);
         const size_t nc = B.con_type.size();
         S.con_type = Ar.up(B.con_type);
         S.con_base = Ar.up(B.con_base);

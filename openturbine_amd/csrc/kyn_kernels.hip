@@ -1300,6 +1300,37 @@ __global__ void __launch_bounds__(kThreads) k_plain_element(DevSys S, int Nm, in
     element_phase<double, false>(S, e, sm, L, 0);
 }
 
+// UpdateSystemVariablesMasses / UpdateSystemVariablesSprings
+// (src/step/update_system_variables_masses.hpp, update_system_variables_springs.hpp):
+// one thread per point element, masses first
+__global__ void k_plain_points(DevSys S) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < S.num_masses) {
+        const int node = S.mass_node[k];
+        const int m = node_model(S, node);
+        if (m < 0 || !S.model_active[m]) return;
+        double r[4], ud[3], w[3], wd[3];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) r[c] = S.q[node * 7 + 3 + c];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            ud[c] = S.vd[node * 6 + c];
+            wd[c] = S.vd[node * 6 + 3 + c];
+            w[c] = S.v[node * 6 + 3 + c];
+        }
+        mass_terms(r, ud, w, wd, S.mass_r0 + k * 4, S.mass_Mstar + k * 36, S.model_gravity + m * 3,
+                   S.p.beta_prime, S.p.gamma_prime, S.tan_rot + node * 9, S.mass_res + k * 6, S.mass_mat + k * 36);
+        return;
+    }
+    const int j = k - S.num_masses;
+    if (j >= S.num_springs) return;
+    const int n1 = S.spring_n1[j], n2 = S.spring_n2[j];
+    const int m = node_model(S, n1);
+    if (m < 0 || !S.model_active[m]) return;
+    spring_terms(S.spring_x0 + j * 3, S.q + n1 * 7, S.q + n2 * 7, S.spring_lref[j], S.spring_k[j],
+                 S.spring_res + j * 6, S.spring_mat + j * 9);
+}
+
 // UpdateConstraintVariables (src/step/update_constraint_variables.hpp:13-37)
 __global__ void k_plain_constraints(DevSys S) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1395,6 +1426,28 @@ __global__ void __launch_bounds__(kThreads) k_plain_assemble(DevSys S) {
             const int row = S.node_local[S.elem_nodes[noff + i]] + c / 6;
             const int col = S.node_local[S.elem_nodes[noff + j]] + c % 6;
             A[(size_t)row * lda + col] += Me[k] * cond;
+        }
+        __syncthreads();
+    }
+    // ContributeMassesToVector / ContributeMassesToSparseMatrix, masses in order
+    for (int km = S.block_mass_begin[b]; km < S.block_mass_end[b]; ++km) {
+        const int e = S.block_masses[km];
+        const int l = S.node_local[S.mass_node[e]];
+        if (t < 6) rhs[l + t] += S.mass_res[e * 6 + t];
+        if (t < 36) A[(size_t)(l + t / 6) * lda + l + t % 6] += S.mass_mat[e * 36 + t] * cond;
+        __syncthreads();
+    }
+    // ContributeSpringsToVector / ContributeSpringsToSparseMatrix (translational dofs only)
+    for (int ks = S.block_spring_begin[b]; ks < S.block_spring_end[b]; ++ks) {
+        const int e = S.block_springs[ks];
+        const int l1 = S.node_local[S.spring_n1[e]], l2 = S.node_local[S.spring_n2[e]];
+        if (t < 6) rhs[(t < 3 ? l1 : l2) + t % 3] += S.spring_res[e * 6 + t];
+        __syncthreads();
+        if (t < 36) {
+            const int blk = t / 9, rc = t % 9;  // node pair (blk / 2, blk % 2), entry rc
+            const int lr = (blk / 2) ? l2 : l1, lc = (blk % 2) ? l2 : l1;
+            const double sg = (blk == 0 || blk == 3) ? 1. : -1.;
+            A[(size_t)(lr + rc / 3) * lda + lc + rc % 3] += sg * S.spring_mat[e * 9 + rc] * cond;
         }
         __syncthreads();
     }
@@ -1538,8 +1591,11 @@ void launch_plain_iteration(const DevSys& S, int Nm, int Qm, int nm, int iter, b
     hipLaunchKernelGGL(k_plain_clear_flag, dim3(1), dim3(64), 0, st, S);
     hipLaunchKernelGGL(k_plain_tangent, dim3((S.num_nodes + 255) / 256), dim3(256), 0, st, S);
     if (e0) (void)hipEventRecord(e0, st);
-    hipLaunchKernelGGL(k_plain_element, dim3(S.num_elems), dim3(kThreads), element_lds_bytes(Nm, Qm), st, S, Nm, Qm);
+    if (S.num_elems > 0)
+        hipLaunchKernelGGL(k_plain_element, dim3(S.num_elems), dim3(kThreads), element_lds_bytes(Nm, Qm), st, S, Nm, Qm);
     if (e1) (void)hipEventRecord(e1, st);
+    if (S.num_masses + S.num_springs > 0)
+        hipLaunchKernelGGL(k_plain_points, dim3((S.num_masses + S.num_springs + 63) / 64), dim3(64), 0, st, S);
     if (S.num_cons > 0)
         hipLaunchKernelGGL(k_plain_constraints, dim3((S.num_cons + 63) / 64), dim3(64), 0, st, S);
     hipLaunchKernelGGL(k_plain_assemble, dim3(S.num_blocks), dim3(kThreads), 0, st, S);

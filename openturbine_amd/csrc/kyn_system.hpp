@@ -75,6 +75,17 @@ struct DevSys {
     double* elem_res;                          // [sum N][6]
     double* elem_mat;                          // [sum N^2][36] (plain form)
 
+    // ---- point masses (src/elements/masses/masses.hpp) and springs (src/elements/springs/springs.hpp)
+    int num_masses, num_springs;
+    const int *block_mass_begin, *block_mass_end, *block_masses;      // per block, in model order
+    const int *block_spring_begin, *block_spring_end, *block_springs;
+    const int* mass_node;
+    const double *mass_Mstar, *mass_r0;  // [k][36] section mass matrix, [k][4] node x0 rotation
+    double *mass_res, *mass_mat;         // [k][6], [k][36] (beta' M + gamma' G + K T)
+    const int *spring_n1, *spring_n2;
+    const double *spring_x0, *spring_lref, *spring_k;  // [k][3] (x0 of node 2 - x0 of node 1), [k], [k]
+    double *spring_res, *spring_mat;                   // [k][6] ([f, -f]), [k][9] (a; blocks are +-a)
+
     // ---- constraints (src/constraints/constraints.hpp)
     const int *con_type, *con_base, *con_target, *con_rows, *con_model_row;
     const int* con_local;  // block-local index of the first constraint row

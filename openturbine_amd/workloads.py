@@ -395,6 +395,47 @@ def cantilever_config_spec(num_nodes=8, num_qps=11, length=10.0, tip_force=1e5, 
                      step_inputs=step_inputs, meta={"tip_node": tip})
 
 
+def beam_mass_spring_spec(num_steps=20, h=0.005, rho_inf=0.9, max_iter=50):
+    """The cantilever above (tip follower force, clamped root) with a point mass at the tip and a
+    second, offset point mass hung from the tip by a spring: node `extra` sits 1 m below the tip
+    with a rotated reference frame and a mass matrix whose centre of mass is off the node
+    (nonzero eta), under gravity, so every term of the mass and spring elements is exercised in
+    one KKT block with the beam."""
+    base = cantilever_config_spec(num_steps=num_steps, h=h, rho_inf=rho_inf, max_iter=max_iter)
+    tip = base.meta["tip_node"]
+    n0 = base.num_nodes
+    half = 0.3
+    r_extra = np.array([np.cos(half), np.sin(half) * 0.6, 0.0, np.sin(half) * 0.8])
+    x0 = np.vstack([base.x0, np.concatenate([base.x0[tip, :3] + [0.0, 0.0, -1.0], r_extra])])
+    u = np.vstack([base.u, [0, 0, 0, 1, 0, 0, 0]])
+    v = np.vstack([base.v, np.zeros(6)])
+    vd = np.vstack([base.vd, np.zeros(6)])
+    s = np.concatenate([base.s, [1.0]])
+
+    def rigid_mass(m, eta, J):
+        et = np.array([[0, -eta[2], eta[1]], [eta[2], 0, -eta[0]], [-eta[1], eta[0], 0]])
+        M = np.zeros((6, 6))
+        M[:3, :3] = m * np.eye(3)
+        M[3:, :3] = m * et
+        M[:3, 3:] = (m * et).T
+        M[3:, 3:] = J
+        return M.reshape(36)
+
+    J = np.array([[4.0, 0.3, -0.2], [0.3, 3.0, 0.1], [-0.2, 0.1, 5.0]])
+    masses = [MassSpec(tip, rigid_mass(50.0, [0.0, 0.0, 0.0], np.eye(3) * 2.0)),
+              MassSpec(n0, rigid_mass(120.0, [0.05, -0.02, 0.1], J))]
+    springs = [SpringSpec(tip, n0, 4.0e5, 0.8)]
+
+    def step_inputs(step, read_state=None):
+        f = base.step_inputs(step, read_state)["nodal_loads"]
+        return {"nodal_loads": np.vstack([f, np.zeros(6)])}
+
+    return ModelSpec("beam_mass_spring", x0, u, v, vd, s, base.beams, base.constraints,
+                     np.array([0.0, 0.0, -9.81]), max_iter=max_iter, h=h, rho_inf=rho_inf, atol=base.atol,
+                     rtol=base.rtol, num_steps=num_steps, step_inputs=step_inputs,
+                     meta={"tip_node": tip, "extra_node": n0}, masses=masses, springs=springs)
+
+
 def _spd_project(A: np.ndarray) -> np.ndarray:
     """Symmetric part, eigenvalues clipped to >= 1e-6 * lambda_max (stacked [..., 6, 6])."""
     A = 0.5 * (A + np.swapaxes(A, -1, -2))

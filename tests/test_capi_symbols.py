@@ -20,7 +20,7 @@ def test_header_declares_the_boundary():
     names = declared_functions()
     for required in ("kyn_model_create", "kyn_model_add_beam_element", "kyn_system_create", "kyn_step",
                      "kyn_get_state", "kyn_set_constraint_displacement", "kyn_get_convergence_history",
-                     "kyn_write_restart"):
+                     "kyn_write_restart", "kyn_model_add_mass_element", "kyn_model_add_spring_element"):
         assert required in names
 
 
