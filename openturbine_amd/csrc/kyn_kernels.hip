@@ -38,7 +38,7 @@ constexpr int kContractThreads = 252;  // 36 components x 7 node groups
 // ---------------------------------------------------------------------------------------
 struct Lay {
     // persistent
-    int loc, rhs, xs, xs2, piv, red, cres, misc, lg_of, phys_of;
+    int loc, rhs, xs, xs2, piv, red, cres, misc, lg_of, phys_of, lcol;
     // overlay: element view
     int nq, nv, nvd, tr, phi, dphi, w, J, qpF, qpM, phiT;
     // overlay: matrix view
@@ -68,6 +68,7 @@ __host__ __device__ inline Lay make_layout(int Nm, int Qm, int nm, int cm, bool 
     L.misc = take(16 * 4);
     L.lg_of = take(kLuTile * 4);
     L.phys_of = take(kLuTile * 4);
+    L.lcol = take(with_matrix ? 2 * 256 * tt_size : 0);  // two panels' L11, column-major 16 x 16
     const int overlay = o;
     L.nq = take(Nm * 7 * 8);
     L.nv = take(Nm * 6 * 8);
@@ -366,7 +367,12 @@ constexpr int kPanel = 16;
 struct LuScratch {
     int* lg_of;    // logical row of each physical row (kLuTile)
     int* phys_of;  // physical row of each logical row (kLuTile)
+    void* lcol;    // two 16 x 16 column-major L11 tiles (panel p in tile p & 1), storage type
 };
+template <typename TT>
+__device__ __forceinline__ TT* panel_l11(const LuScratch& W, int c0) {
+    return reinterpret_cast<TT*>(W.lcol) + ((c0 / kPanel) & 1) * kPanel * kPanel;
+}
 
 constexpr int kDppRowMirror = 0x140, kDppRowHalfMirror = 0x141, kDppQuad3210 = 0x1B, kDppQuad1032 = 0xB1;
 
@@ -442,15 +448,92 @@ __device__ __forceinline__ TT slot_tag(TT v) {
     return v;
 }
 
-// Factor panel columns [c0, c0 + w) in wave 0.  Row bookkeeping stays in registers (lg per
-// lane row) and is written back once at the end of the panel.
-template <typename TT>
-__device__ __forceinline__ void lu_panel(TT* A, int n, int lda, int c0, int w, int* piv, const LuScratch& W) {
-    const int lane = threadIdx.x & 63;
-    TT P[3][kPanel];
-    int lg[3];
+// One pivot column (J) of the wave-0 panel factorization: pivot search over the 64 lanes, pivot
+// row broadcast, multipliers and the rank-1 update of the panel's later columns.
+template <typename TT, int NS, bool kFull, int J>
+__device__ __forceinline__ void panel_columns(TT (&P)[NS][kPanel], int (&lg)[NS], int (&pl)[kPanel], int c0, int w,
+                                              int lane) {
+    if constexpr (J < kPanel) {
+        if (kFull || J < w) {
+            const int k = c0 + J;
+            // largest |a_rk| over eligible rows (NaN never wins), then the lowest key holding it;
+            // both reduced over the 64 lanes, so every lane sees the same pivot
+            TT vs[NS];
+            TT bv = TT(-1);
+            bool dnan = false;
 #pragma unroll
-    for (int s = 0; s < 3; ++s) {
+            for (int s = 0; s < NS; ++s) {
+                const TT v = P[s][J];
+                vs[s] = lg[s] >= k ? tabs(v) : TT(-1);
+                bv = fmax(bv, vs[s]);
+                dnan |= (lg[s] == k) & (v != v);
+            }
+            bv = wave_max(bv);
+            int bk = 0x7fffffff;
+#pragma unroll
+            for (int s = 0; s < NS; ++s) {
+                const int key = (lg[s] << 8) | (lane + 64 * s);
+                bk = ((lg[s] >= k) & (vs[s] == bv)) ? (key < bk ? key : bk) : bk;
+            }
+            bk = wave_min(bk);
+            int key = __builtin_amdgcn_readfirstlane(bk);
+            if (__any(dnan)) {  // the oracle keeps row k when a_kk is NaN
+                int kp = 0;
+#pragma unroll
+                for (int s = 0; s < NS; ++s) {
+                    const unsigned long long m = __ballot(lg[s] == k);
+                    kp = m ? (__builtin_ctzll(m) + 64 * s) : kp;
+                }
+                key = (k << 8) | kp;
+            }
+            const int plog = key >> 8, pphys = key & 255;
+            const int ol = pphys & 63, slot = pphys >> 6;
+            // pivot row (panel part) to every lane (a uniform branch on the slot: fewer
+            // instructions than selecting among the slots in every lane)
+            TT prow[kPanel];
+#pragma unroll
+            for (int jj = 0; jj < kPanel; ++jj) prow[jj] = TT(0);
+            if (NS == 1 || slot == 0) {
+#pragma unroll
+                for (int jj = J; jj < kPanel; ++jj) prow[jj] = rdlane(slot_tag<0>(P[0][jj]), ol);
+            } else if (NS == 2 || slot == 1) {
+#pragma unroll
+                for (int jj = J; jj < kPanel; ++jj) prow[jj] = rdlane(slot_tag<1>(P[NS > 1 ? 1 : 0][jj]), ol);
+            } else {
+#pragma unroll
+                for (int jj = J; jj < kPanel; ++jj) prow[jj] = rdlane(slot_tag<2>(P[NS > 2 ? 2 : 0][jj]), ol);
+            }
+            const TT d = prow[J];
+            pl[J] = plog;
+#pragma unroll
+            for (int s = 0; s < NS; ++s) lg[s] = lg[s] == k ? plog : (lg[s] == plog ? k : lg[s]);
+            if (d != TT(0)) {  // a zero pivot leaves the panel unchanged (the oracle skips the column)
+                const TT inv = TT(1) / d;
+#pragma unroll
+                for (int s = 0; s < NS; ++s) {
+                    const bool below = lg[s] > k;
+                    const TT l = below ? P[s][J] * inv : TT(0);
+                    P[s][J] = below ? l : P[s][J];
+#pragma unroll
+                    for (int jj = J + 1; jj < kPanel; ++jj) P[s][jj] -= l * prow[jj];
+                }
+            }
+        }
+        panel_columns<TT, NS, kFull, J + 1>(P, lg, pl, c0, w, lane);
+    }
+}
+
+// Factor panel columns [c0, c0 + w) in wave 0 (NS = row slots per lane, n <= 64 NS).  Row
+// bookkeeping stays in registers (lg per lane row) and is written back once at the end of the
+// panel, together with the panel's unit-lower L11 as a compact column-major tile.
+template <typename TT, int NS, bool kFull>
+__device__ __forceinline__ void lu_panel_ns(TT* A, int n, int lda, int c0, int w, int* piv, const LuScratch& W) {
+    if (kFull) w = kPanel;
+    const int lane = threadIdx.x & 63;
+    TT P[NS][kPanel];
+    int lg[NS];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
         const int r = lane + 64 * s;
         const bool ok = r < n;
         const int rc = ok ? r : 0;  // clamped: loads are unconditional, values selected after
@@ -463,75 +546,20 @@ __device__ __forceinline__ void lu_panel(TT* A, int n, int lda, int c0, int w, i
             P[s][jj] = (ok && jj < w) ? P[s][jj] : TT(0);
         }
     }
+    // Columns as compile-time steps (constant register indices without relying on unrolling:
+    // the tile stays in registers).  A single wave issues about one instruction per four cycles,
+    // so each step keeps its instruction count down: uniform branches on the pivot's slot and
+    // on a zero pivot rather than per-element selects, and only NS row slots per lane.
+    int pl[kPanel];
+    panel_columns<TT, NS, kFull, 0>(P, lg, pl, c0, w, lane);
+    if (lane == 0) {
 #pragma unroll
-    for (int j = 0; j < kPanel; ++j) {
-        if (j >= w) continue;
-        const int k = c0 + j;
-        // largest |a_rk| over eligible rows (NaN never wins), then the lowest key holding it;
-        // both reduced over the 64 lanes, so every lane sees the same pivot
-        TT vs[3];
-        TT bv = TT(-1);
-        bool dnan = false;
-#pragma unroll
-        for (int s = 0; s < 3; ++s) {
-            const TT v = P[s][j];
-            vs[s] = lg[s] >= k ? tabs(v) : TT(-1);
-            bv = fmax(bv, vs[s]);
-            dnan |= (lg[s] == k) & (v != v);
-        }
-        bv = wave_max(bv);
-        int bk = 0x7fffffff;
-#pragma unroll
-        for (int s = 0; s < 3; ++s) {
-            const int key = (lg[s] << 8) | (lane + 64 * s);
-            bk = ((lg[s] >= k) & (vs[s] == bv)) ? (key < bk ? key : bk) : bk;
-        }
-        bk = wave_min(bk);
-        int key = __builtin_amdgcn_readfirstlane(bk);
-        if (__any(dnan)) {  // the oracle keeps row k when a_kk is NaN
-            int kp = -1;
-#pragma unroll
-            for (int s = 0; s < 3; ++s) kp = lg[s] == k ? lane + 64 * s : kp;
-            for (int off = 32; off > 0; off >>= 1) {
-                const int o = __shfl_xor(kp, off, 64);
-                kp = o > kp ? o : kp;
-            }
-            key = (k << 8) | __builtin_amdgcn_readfirstlane(kp);
-        }
-        const int plog = key >> 8, pphys = key & 255;
-        const int ol = pphys & 63, slot = pphys >> 6;
-        // pivot row (panel part) to every lane
-        TT prow[kPanel];
-#pragma unroll
-        for (int jj = 0; jj < kPanel; ++jj) prow[jj] = TT(0);
-        if (slot == 0) {
-#pragma unroll
-            for (int jj = j; jj < kPanel; ++jj) prow[jj] = rdlane(slot_tag<0>(P[0][jj]), ol);
-        } else if (slot == 1) {
-#pragma unroll
-            for (int jj = j; jj < kPanel; ++jj) prow[jj] = rdlane(slot_tag<1>(P[1][jj]), ol);
-        } else {
-#pragma unroll
-            for (int jj = j; jj < kPanel; ++jj) prow[jj] = rdlane(slot_tag<2>(P[2][jj]), ol);
-        }
-        const TT d = prow[j];
-        if (lane == 0) piv[k] = plog;
-#pragma unroll
-        for (int s = 0; s < 3; ++s) lg[s] = lg[s] == k ? plog : (lg[s] == plog ? k : lg[s]);
-        if (d != TT(0)) {
-            const TT inv = TT(1) / d;
-#pragma unroll
-            for (int s = 0; s < 3; ++s) {
-                const bool below = lg[s] > k;
-                const TT l = below ? P[s][j] * inv : TT(0);
-                P[s][j] = below ? l : P[s][j];
-#pragma unroll
-                for (int jj = j + 1; jj < kPanel; ++jj) P[s][jj] -= l * prow[jj];
-            }
-        }
+        for (int j = 0; j < kPanel; ++j)
+            if (kFull || j < w) piv[c0 + j] = pl[j];
     }
+    TT* Lc = panel_l11<TT>(W, c0);
 #pragma unroll
-    for (int s = 0; s < 3; ++s) {
+    for (int s = 0; s < NS; ++s) {
         const int r = lane + 64 * s;
         if (r < n) {
 #pragma unroll
@@ -539,17 +567,70 @@ __device__ __forceinline__ void lu_panel(TT* A, int n, int lda, int c0, int w, i
                 if (jj < w) A[r * lda + c0 + jj] = P[s][jj];
             W.lg_of[r] = lg[s];
             W.phys_of[lg[s]] = r;
+            const int i = lg[s] - c0;  // row of L11 this lane holds, if any
+            if (i >= 0 && i < w) {
+#pragma unroll
+                for (int k = 0; k < kPanel; ++k) Lc[k * kPanel + i] = k < i ? P[s][k] : TT(0);
+            }
         }
     }
+}
+template <typename TT>
+__device__ __forceinline__ void lu_panel(TT* A, int n, int lda, int c0, int w, int* piv, const LuScratch& W) {
+    if (w < kPanel) {
+        if (n <= 64)
+            lu_panel_ns<TT, 1, false>(A, n, lda, c0, w, piv, W);
+        else if (n <= 128)
+            lu_panel_ns<TT, 2, false>(A, n, lda, c0, w, piv, W);
+        else
+            lu_panel_ns<TT, 3, false>(A, n, lda, c0, w, piv, W);
+    } else if (n <= 64) {
+        lu_panel_ns<TT, 1, true>(A, n, lda, c0, w, piv, W);
+    } else if (n <= 128) {
+        lu_panel_ns<TT, 2, true>(A, n, lda, c0, w, piv, W);
+    } else {
+        lu_panel_ns<TT, 3, true>(A, n, lda, c0, w, piv, W);
+    }
+}
+
+
+
+// Columns KG .. KG + 3 of the U-row solve: u_i -= l_ik u_k (i > k).  All 16-byte loads of the
+// four L11 columns are issued first, then one wait, then the FMAs.
+template <typename TT, int KG>
+__device__ __forceinline__ void u_rows_group(const TT* Lc, TT* u) {
+    constexpr int VW = 16 / sizeof(TT);
+    typedef TT vec_t __attribute__((ext_vector_type(16 / sizeof(TT))));
+    TT lk[4][kPanel];
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk) {
+#pragma unroll
+        for (int q = 0; q < kPanel / VW; ++q) {
+            if (KG + kk < kPanel - 1 && (q + 1) * VW > KG + kk + 1) {
+                const vec_t v = *reinterpret_cast<const vec_t*>(Lc + (KG + kk) * kPanel + q * VW);
+#pragma unroll
+                for (int e = 0; e < VW; ++e) lk[kk][q * VW + e] = v[e];
+            }
+        }
+    }
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk)
+#pragma unroll
+        for (int i = KG + kk + 1; i < kPanel; ++i) pin(lk[kk][i]);
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk)
+#pragma unroll
+        for (int i = KG + kk + 1; i < kPanel; ++i) u[i] -= lk[kk][i] * u[KG + kk];
 }
 
 // Applies panel [c0, c0 + w) to this wave's share of the columns [ja, jb): columns
 // ja + g, ja + g + gs, ... (wave g of gs).  First the panel's U rows (U12 = L11^-1 A12, lane <->
-// column, written in place into the pivot rows), then the trailing rows a_rj -= sum_k l_rk u_kj
-// (k ascending; lane <-> rows lane, lane + 64, lane + 128 with their multipliers in registers,
-// zero for rows already pivoted, which are then rewritten unchanged; two columns at a time).
+// column, written in place into the pivot rows) and the tail rows, then the trailing rows
+// a_rj -= sum_k l_rk u_kj (k ascending; lane <-> rows lane, lane + 64 (NSA slots) with their
+// multipliers in registers, zero for rows already pivoted, which are then rewritten unchanged;
+// two columns at a time).
 // Runs within one wave: its LDS writes are seen by its own later reads without a barrier.
-template <typename TT, bool kFull>
+template <typename TT, bool kFull, int NSA>
 __device__ __forceinline__ void lu_apply_panel(TT* A, int n, int lda, int c0, int w, int ja, int jb, int g, int gs,
                                                const LuScratch& W) {
     if (kFull) w = kPanel;
@@ -560,7 +641,10 @@ __device__ __forceinline__ void lu_apply_panel(TT* A, int n, int lda, int c0, in
     int pr[kPanel];
 #pragma unroll
     for (int i = 0; i < kPanel; ++i) pr[i] = __builtin_amdgcn_readfirstlane(W.phys_of[c0 + (i < w ? i : 0)]);
-    // U rows
+    // U rows: u_i -= l_ik u_k for k ascending (each entry's FMA sequence as in the unblocked
+    // loop), one column per lane; L11 from the panel's compact column-major tile, where column
+    // k's entries below the diagonal are contiguous (vector loads, same address on all lanes)
+    const TT* Lc = panel_l11<TT>(W, c0);
     for (int t = lane; t < m; t += 64) {
         const int j = ja + g + gs * t;
         TT u[kPanel];
@@ -569,26 +653,35 @@ __device__ __forceinline__ void lu_apply_panel(TT* A, int n, int lda, int c0, in
             u[i] = A[pr[i] * lda + j];
             pin(u[i]);
         }
-#pragma unroll
-        for (int i = 1; i < kPanel; ++i) {
-            const TT* Ar = A + pr[i] * lda + c0;
-            TT acc = u[i];
-#pragma unroll
-            for (int i2 = 0; i2 < i; ++i2) acc -= Ar[i2] * u[i2];
-            u[i] = (kFull || i < w) ? acc : TT(0);
-        }
+        u_rows_group<TT, 0>(Lc, u);
+        u_rows_group<TT, 4>(Lc, u);
+        u_rows_group<TT, 8>(Lc, u);
+        u_rows_group<TT, 12>(Lc, u);
 #pragma unroll
         for (int i = 1; i < kPanel; ++i)
             if (kFull || i < w) A[pr[i] * lda + j] = u[i];
+        // tail rows (physical rows >= 64 NSA, at most 16): updated here, lane <-> column, with
+        // this column's U values still in registers (a lane <-> row slot for so few rows would
+        // issue a full slot of instructions for a handful of active lanes)
+        for (int r = 64 * NSA; r < n; ++r) {
+            if (W.lg_of[r] >= j0) {
+                TT* Ar = A + r * lda;
+                TT a = Ar[j];
+#pragma unroll
+                for (int i = 0; i < kPanel; ++i)
+                    if (kFull || i < w) a -= Ar[c0 + i] * u[i];
+                Ar[j] = a;
+            }
+        }
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    // trailing rows
-    TT L[3][kPanel];
-    bool ok[3];
+    // trailing rows of the NSA lane <-> row slots
+    TT L[NSA][kPanel];
+    bool ok[NSA];
 #pragma unroll
-    for (int s = 0; s < 3; ++s) {
+    for (int s = 0; s < NSA; ++s) {
         const int r = lane + 64 * s;
         ok[s] = r < n;
         const int rc = ok[s] ? r : 0;
@@ -617,7 +710,7 @@ __device__ __forceinline__ void lu_apply_panel(TT* A, int n, int lda, int c0, in
             }
         }
 #pragma unroll
-        for (int s = 0; s < 3; ++s) {
+        for (int s = 0; s < NSA; ++s) {
             if (ok[s]) {
                 TT* Ar = A + (lane + 64 * s) * lda;
                 TT a0 = Ar[j];
@@ -637,10 +730,17 @@ __device__ __forceinline__ void lu_apply_panel(TT* A, int n, int lda, int c0, in
 template <typename TT>
 __device__ __forceinline__ void lu_apply(TT* A, int n, int lda, int c0, int w, int ja, int jb, int g, int gs,
                                          const LuScratch& W) {
-    if (w == kPanel)
-        lu_apply_panel<TT, true>(A, n, lda, c0, w, ja, jb, g, gs, W);
-    else
-        lu_apply_panel<TT, false>(A, n, lda, c0, w, ja, jb, g, gs, W);
+    // NSA lane <-> row slots; the remaining rows (at most 16) are tail rows
+    if (n > 80) {
+        if (w == kPanel)
+            lu_apply_panel<TT, true, 2>(A, n, lda, c0, w, ja, jb, g, gs, W);
+        else
+            lu_apply_panel<TT, false, 2>(A, n, lda, c0, w, ja, jb, g, gs, W);
+    } else if (w == kPanel) {
+        lu_apply_panel<TT, true, 1>(A, n, lda, c0, w, ja, jb, g, gs, W);
+    } else {
+        lu_apply_panel<TT, false, 1>(A, n, lda, c0, w, ja, jb, g, gs, W);
+    }
 }
 
 // Panels with look-ahead: all waves first apply panel p to the next panel's columns; then wave
@@ -1130,7 +1230,8 @@ __device__ __forceinline__ void factor_solve_uncondition(const DevSys& S, int b,
     }
     for (int i = t; i < n; i += blockDim.x) A[i * lda + n] = (TT)rhs[i];
     __syncthreads();
-    const LuScratch W{reinterpret_cast<int*>(sm + L.lg_of), reinterpret_cast<int*>(sm + L.phys_of)};
+    const LuScratch W{reinterpret_cast<int*>(sm + L.lg_of), reinterpret_cast<int*>(sm + L.phys_of),
+                      reinterpret_cast<void*>(sm + L.lcol)};
     KYN_PHASE_MARK(7);
     lu_factor_augmented<TT>(A, n, lda, piv, W);
     KYN_PHASE_MARK(8);
