@@ -380,21 +380,9 @@ template <int C>
 __device__ __forceinline__ int dpp_x(int v) {
     return __builtin_amdgcn_update_dpp(0, v, C, 0xF, 0xF, false);
 }
-// lane ^ 16 / lane ^ 32 (gfx950 permlane swaps: rows 2i+1 of the first operand trade with
-// rows 2i of the second, resp. rows 2,3 with rows 0,1)
-__device__ __forceinline__ int xor16_x(int v) {
-    const auto r = __builtin_amdgcn_permlane16_swap(v, v, false, false);
-    return ((threadIdx.x >> 4) & 1) ? (int)r[0] : (int)r[1];
-}
-__device__ __forceinline__ int xor32_x(int v) {
-    const auto r = __builtin_amdgcn_permlane32_swap(v, v, false, false);
-    return ((threadIdx.x >> 5) & 1) ? (int)r[0] : (int)r[1];
-}
 template <int C>
 __device__ __forceinline__ int lane_x(int v) {
-    if constexpr (C == 16) return xor16_x(v);
-    else if constexpr (C == 32) return xor32_x(v);
-    else return dpp_x<C>(v);
+    return dpp_x<C>(v);
 }
 template <int C>
 __device__ __forceinline__ float lane_x(float v) {
@@ -406,24 +394,64 @@ __device__ __forceinline__ double lane_x(double v) {
     const int lo = lane_x<C>((int)(b & 0xffffffffll)), hi = lane_x<C>((int)(b >> 32));
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
-template <typename TT>
-__device__ __forceinline__ TT wave_max(TT v) {
-    v = fmax(v, lane_x<kDppRowMirror>(v));
-    v = fmax(v, lane_x<kDppRowHalfMirror>(v));
-    v = fmax(v, lane_x<kDppQuad3210>(v));
-    v = fmax(v, lane_x<kDppQuad1032>(v));
-    v = fmax(v, lane_x<16>(v));
-    return fmax(v, lane_x<32>(v));
+// Row broadcasts (GFX9 DPP): lane 15 of each row to the next row (written rows 1 and 3), and
+// lane 31 to rows 2 and 3; rows outside the mask keep `old`.
+constexpr int kDppRowBcast15 = 0x142, kDppRowBcast31 = 0x143;
+template <int C, int RM>
+__device__ __forceinline__ int dpp_old(int old, int v) {
+    return __builtin_amdgcn_update_dpp(old, v, C, RM, 0xF, false);
 }
+template <int C, int RM>
+__device__ __forceinline__ float dpp_old(float old, float v) {
+    return __int_as_float(dpp_old<C, RM>(__float_as_int(old), __float_as_int(v)));
+}
+template <int C, int RM>
+__device__ __forceinline__ double dpp_old(double old, double v) {
+    const long long o = __double_as_longlong(old), b = __double_as_longlong(v);
+    const int lo = dpp_old<C, RM>((int)(o & 0xffffffffll), (int)(b & 0xffffffffll));
+    const int hi = dpp_old<C, RM>((int)(o >> 32), (int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+// Reduction over the 64 lanes whose result is valid in lane 63: four in-row exchanges, then the
+// two row broadcasts (no cross-row swaps).
+template <typename T, typename Op>
+__device__ __forceinline__ T wave_reduce_to63(T v, Op op) {
+    v = op(v, lane_x<kDppQuad1032>(v));
+    v = op(v, lane_x<kDppQuad3210>(v));
+    v = op(v, lane_x<kDppRowHalfMirror>(v));
+    v = op(v, lane_x<kDppRowMirror>(v));
+    v = op(v, dpp_old<kDppRowBcast15, 0xA>(v, v));
+    return op(v, dpp_old<kDppRowBcast31, 0xC>(v, v));
+}
+__device__ __forceinline__ int imax(int a, int b) { return a > b ? a : b; }
 __device__ __forceinline__ int imin(int a, int b) { return a < b ? a : b; }
-__device__ __forceinline__ int wave_min(int v) {
-    v = imin(v, lane_x<kDppRowMirror>(v));
-    v = imin(v, lane_x<kDppRowHalfMirror>(v));
-    v = imin(v, lane_x<kDppQuad3210>(v));
-    v = imin(v, lane_x<kDppQuad1032>(v));
-    v = imin(v, lane_x<16>(v));
-    return imin(v, lane_x<32>(v));
+// max over lanes (to every lane through lane 63)
+__device__ __forceinline__ int wave_max(int v) {
+    return __builtin_amdgcn_readlane(wave_reduce_to63(v, imax), 63);
 }
+__device__ __forceinline__ double wave_max(double v) {
+    const double r = wave_reduce_to63(v, [](double a, double b) { return fmax(a, b); });
+    const long long b = __double_as_longlong(r);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), 63), hi = __builtin_amdgcn_readlane((int)(b >> 32), 63);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ int wave_min(int v) {
+    return __builtin_amdgcn_readlane(wave_reduce_to63(v, imin), 63);
+}
+// pivot candidate magnitude: FP32 as the bit pattern of |v| (an int ordered like |v| for
+// non-negative floats, so the reduction is an integer max), FP64 as |v|; -1 = not a candidate
+__device__ __forceinline__ int cand_of(float v, bool ok) { return ok && v == v ? __float_as_int(fabsf(v)) : -1; }
+__device__ __forceinline__ double cand_of(double v, bool ok) { return ok ? fabs(v) : -1.; }
+template <typename TT>
+struct Cand;
+template <>
+struct Cand<float> {
+    using type = int;
+};
+template <>
+struct Cand<double> {
+    using type = double;
+};
 
 __device__ __forceinline__ float rdlane(float v, int l) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
@@ -458,14 +486,15 @@ __device__ __forceinline__ void panel_columns(TT (&P)[NS][kPanel], int (&lg)[NS]
             const int k = c0 + J;
             // largest |a_rk| over eligible rows (NaN never wins), then the lowest key holding it;
             // both reduced over the 64 lanes, so every lane sees the same pivot
-            TT vs[NS];
-            TT bv = TT(-1);
+            using CT = typename Cand<TT>::type;
+            CT vs[NS];
+            CT bv = CT(-1);
             bool dnan = false;
 #pragma unroll
             for (int s = 0; s < NS; ++s) {
                 const TT v = P[s][J];
-                vs[s] = lg[s] >= k ? tabs(v) : TT(-1);
-                bv = fmax(bv, vs[s]);
+                vs[s] = cand_of(v, lg[s] >= k);
+                bv = vs[s] > bv ? vs[s] : bv;
                 dnan |= (lg[s] == k) & (v != v);
             }
             bv = wave_max(bv);
@@ -475,8 +504,7 @@ __device__ __forceinline__ void panel_columns(TT (&P)[NS][kPanel], int (&lg)[NS]
                 const int key = (lg[s] << 8) | (lane + 64 * s);
                 bk = ((lg[s] >= k) & (vs[s] == bv)) ? (key < bk ? key : bk) : bk;
             }
-            bk = wave_min(bk);
-            int key = __builtin_amdgcn_readfirstlane(bk);
+            int key = wave_min(bk);
             if (__any(dnan)) {  // the oracle keeps row k when a_kk is NaN
                 int kp = 0;
 #pragma unroll
@@ -495,13 +523,13 @@ __device__ __forceinline__ void panel_columns(TT (&P)[NS][kPanel], int (&lg)[NS]
             for (int jj = 0; jj < kPanel; ++jj) prow[jj] = TT(0);
             if (NS == 1 || slot == 0) {
 #pragma unroll
-                for (int jj = J; jj < kPanel; ++jj) prow[jj] = rdlane(slot_tag<0>(P[0][jj]), ol);
+                for (int jj = J; jj < kPanel; ++jj) prow[jj] = rdlane(P[0][jj], ol);
             } else if (NS == 2 || slot == 1) {
 #pragma unroll
-                for (int jj = J; jj < kPanel; ++jj) prow[jj] = rdlane(slot_tag<1>(P[NS > 1 ? 1 : 0][jj]), ol);
+                for (int jj = J; jj < kPanel; ++jj) prow[jj] = rdlane(P[NS > 1 ? 1 : 0][jj], ol);
             } else {
 #pragma unroll
-                for (int jj = J; jj < kPanel; ++jj) prow[jj] = rdlane(slot_tag<2>(P[NS > 2 ? 2 : 0][jj]), ol);
+                for (int jj = J; jj < kPanel; ++jj) prow[jj] = rdlane(P[NS > 2 ? 2 : 0][jj], ol);
             }
             const TT d = prow[J];
             pl[J] = plog;
