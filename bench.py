@@ -158,7 +158,7 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); the ensemble "
-              "shards by rotor with no per-iteration exchange, see DESIGN.md", file=sys.stderr)
+              "would shard by rotor across GPUs with no per-iteration exchange", file=sys.stderr)
         sys.exit(2)
 
     lib = ota.KynLib()

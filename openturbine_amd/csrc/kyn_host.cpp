@@ -6,6 +6,8 @@
 // create_beams.hpp + populate_element_views.hpp + calculate_jacobian.hpp, src/constraints/
 // constraints.hpp:76-266 and src/dof_management/*.hpp.  It runs once per system.
 #include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
 
 #include <algorithm>
 #include <array>
@@ -30,7 +32,11 @@ void launch_fused_step(const DevSys& S, int Nm, int Qm, int nm, hipStream_t st);
 void launch_advance_slot(const DevSys& S, hipStream_t st);
 void launch_plain_predict(const DevSys& S, hipStream_t st);
 void launch_plain_iteration(const DevSys& S, int Nm, int Qm, int nm, int iter, bool solve,
-                            hipStream_t st, hipEvent_t e0, hipEvent_t e1);
+                            hipStream_t st, hipEvent_t e0, hipEvent_t e1, bool defer_update);
+void launch_plain_update(const DevSys& S, int iter, hipStream_t st);
+void launch_large_uncondition(const DevSys& S, int b, int n, hipStream_t st);
+void launch_f64_to_f32(const double* x, float* y, size_t count, hipStream_t st);
+void launch_add_f32_to_f64(const float* d, double* x, int count, hipStream_t st);
 void launch_plain_epilogue(const DevSys& S, hipStream_t st);
 }  // namespace kyn
 
@@ -205,6 +211,14 @@ struct kyn_system_s {
     // saved state (CopyStateData)
     std::vector<double> saved;
     bool has_saved{false};
+    // large blocks (library LU): block ids, sizes, matrix offsets, pivots, scratch vectors
+    std::vector<int> large_blocks, large_n, large_nsys;
+    std::vector<int64_t> large_off;
+    rocblas_handle blas{};
+    rocblas_int *ipiv{}, *info{};
+    double *rvec{}, *bvec{};
+    float* r32{};
+    std::vector<int> model_active_host;
     // graph of one fused step
     hipGraph_t graph{};
     hipGraphExec_t graph_exec{};
@@ -217,6 +231,7 @@ struct kyn_system_s {
         if (ev1) (void)hipEventDestroy(ev1);
         if (ek0) (void)hipEventDestroy(ek0);
         if (ek1) (void)hipEventDestroy(ek1);
+        if (blas) (void)rocblas_destroy_handle(blas);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -576,6 +591,54 @@ double* field_ptr(kyn_system_s* s, int field, int& width) {
     }
 }
 
+void blas_check(rocblas_status st, const char* what) {
+    if (st != rocblas_status_success)
+        throw KynError(KYN_ERR_HIP, std::string(what) + " failed: " + rocblas_status_to_string(st));
+}
+
+// SolveSystem for the large blocks (src/step/solve_system.hpp:14-39 with a dense library LU in
+// place of KLU): LU with partial pivoting of the column-major KKT matrix (rocSOLVER getrf:
+// largest |a_ik|, first index on ties, as the small-block LU) and the triangular solves on the
+// conditioned rhs; FP32 option: the factorization in FP32 and FP64 iterative refinement against
+// the FP64 matrix, r = b - A x, `refine_sweeps` correction solves.  Then UnconditionSolution.
+// Blocks of models that are no longer iterating are skipped.
+void solve_large_blocks(kyn_system_s* s) {
+    auto& S = s->S;
+    const hipStream_t st = s->stream;
+    for (size_t k = 0; k < s->large_blocks.size(); ++k) {
+        const int b = s->large_blocks[k], n = s->large_n[k];
+        if (!s->model_active_host[s->block_model[b]]) continue;
+        double* A = S.Abig + s->large_off[k];
+        double* x = S.xsol + s->block_vec_off[b];
+        const double* rhs = S.rhs + s->block_vec_off[b];
+        if (S.precision == KYN_TANGENT_FP64) {
+            hip_check(hipMemcpyAsync(x, rhs, n * sizeof(double), hipMemcpyDeviceToDevice, st), "D2D");
+            blas_check(rocsolver_dgetrf(s->blas, n, n, A, n, s->ipiv, s->info), "rocsolver_dgetrf");
+            blas_check(rocsolver_dgetrs(s->blas, rocblas_operation_none, n, 1, A, n, s->ipiv, x, n), "rocsolver_dgetrs");
+        } else {
+            float* A32 = S.Abig32 + s->large_off[k];
+            kyn::launch_f64_to_f32(A, A32, static_cast<size_t>(n) * n, st);
+            blas_check(rocsolver_sgetrf(s->blas, n, n, A32, n, s->ipiv, s->info), "rocsolver_sgetrf");
+            // x = A32^-1 b, then refinement sweeps
+            hip_check(hipMemsetAsync(x, 0, n * sizeof(double), st), "memset");
+            hip_check(hipMemcpyAsync(s->rvec, rhs, n * sizeof(double), hipMemcpyDeviceToDevice, st), "D2D");
+            for (int sweep = 0; sweep <= S.refine_sweeps; ++sweep) {
+                kyn::launch_f64_to_f32(s->rvec, s->r32, static_cast<size_t>(n), st);
+                blas_check(rocsolver_sgetrs(s->blas, rocblas_operation_none, n, 1, A32, n, s->ipiv, s->r32, n),
+                           "rocsolver_sgetrs");
+                kyn::launch_add_f32_to_f64(s->r32, x, n, st);
+                if (sweep == S.refine_sweeps) break;
+                // r = b - A x (FP64)
+                hip_check(hipMemcpyAsync(s->rvec, rhs, n * sizeof(double), hipMemcpyDeviceToDevice, st), "D2D");
+                const double alpha = -1., beta = 1.;
+                blas_check(rocblas_dgemv(s->blas, rocblas_operation_none, n, n, &alpha, A, n, x, 1, &beta, s->rvec, 1),
+                           "rocblas_dgemv");
+            }
+        }
+        kyn::launch_large_uncondition(S, b, n, st);
+    }
+}
+
 int run_step(kyn_system_s* s, bool solve = true) {
     auto& S = s->S;
     hip_check(hipEventRecord(s->ev0, s->stream), "event");
@@ -590,10 +653,20 @@ int run_step(kyn_system_s* s, bool solve = true) {
         kyn::launch_plain_predict(S, s->stream);
         hip_check(hipGetLastError(), "k_plain_predict launch");
         int count = 0;
+        const bool large = !s->large_blocks.empty();
+        if (large) s->model_active_host.assign(s->num_models, 1);
         for (int iter = 0; iter < S.p.max_iter; ++iter) {
-            kyn::launch_plain_iteration(S, s->Nm, s->Qm, s->nm, iter, solve, s->stream, s->ek0, s->ek1);
+            kyn::launch_plain_iteration(S, s->Nm, s->Qm, s->nm, iter, solve, s->stream, s->ek0, s->ek1, large);
             hip_check(hipGetLastError(), "plain iteration launch");
             if (!solve) break;
+            if (large) {
+                solve_large_blocks(s);
+                kyn::launch_plain_update(S, iter, s->stream);
+                hip_check(hipGetLastError(), "k_plain_update launch");
+                hip_check(hipMemcpyAsync(s->model_active_host.data(), S.model_active, s->num_models * sizeof(int),
+                                         hipMemcpyDeviceToHost, s->stream),
+                          "D2H");
+            }
             int any = 0;
             hip_check(hipMemcpyAsync(&any, S.any_active, sizeof(int), hipMemcpyDeviceToHost, s->stream), "D2H");
             hip_check(hipStreamSynchronize(s->stream), "sync");
@@ -753,8 +826,6 @@ kyn_system kyn_system_create(const kyn_model* models, size_t num_models, const k
         S.precision = options ? options->precision : KYN_TANGENT_FP64;
         S.refine_sweeps = options ? options->refine_sweeps : 0;
         if (S.refine_sweeps < 0 || S.refine_sweeps > 8) throw KynError(KYN_ERR_INVALID, "refine_sweeps in [0, 8]");
-        if (sys->exec_mode == KYN_EXEC_PLAIN && S.precision != KYN_TANGENT_FP64)
-            throw KynError(KYN_ERR_UNSUPPORTED, "the plain form computes the tangent in FP64 only");
 
         // ---- sizes and eligibility
         S.num_models = static_cast<int>(num_models);
@@ -765,10 +836,34 @@ kyn_system kyn_system_create(const kyn_model* models, size_t num_models, const k
         int Nm = 2, Qm = 1, nm = 1;
         for (int N : B.elem_N) Nm = std::max(Nm, N);
         for (int Q : B.elem_Q) Qm = std::max(Qm, Q);
-        for (int n : B.block_n) nm = std::max(nm, n);
-        if (nm > kyn::kMaxBlockDofs)
-            throw KynError(KYN_ERR_UNSUPPORTED, "independent KKT block larger than " + std::to_string(kyn::kMaxBlockDofs) +
-                                                    " dofs");
+        // blocks above one workgroup's LU take the large path (plain form): column-major HBM
+        // matrix, library LU; the others keep the per-block LDS factorization
+        std::vector<int> block_large(B.block_n.size(), 0);
+        int64_t A_small = 0, A_big = 0;
+        for (size_t b = 0; b < B.block_n.size(); ++b) {
+            const int n = B.block_n[b];
+            if (n > kyn::kMaxBlockDofs) {
+                if (sys->exec_mode == KYN_EXEC_FUSED)
+                    throw KynError(KYN_ERR_UNSUPPORTED, "the fused form takes independent KKT blocks of at most " +
+                                                            std::to_string(kyn::kMaxBlockDofs) +
+                                                            " dofs; use the plain form for larger structures");
+                block_large[b] = 1;
+                sys->large_blocks.push_back(static_cast<int>(b));
+                sys->large_n.push_back(n);
+                sys->large_nsys.push_back(B.block_nsys[b]);
+                sys->large_off.push_back(A_big);
+                B.block_A_off[b] = 0;
+                A_big += static_cast<int64_t>(n) * n;
+            } else {
+                nm = std::max(nm, n);
+                B.block_A_off[b] = A_small;
+                A_small += static_cast<int64_t>(n) * (n + 1);
+            }
+        }
+        B.A_total = A_small;
+        if (sys->exec_mode == KYN_EXEC_PLAIN && S.precision != KYN_TANGENT_FP64 && sys->large_blocks.empty())
+            throw KynError(KYN_ERR_UNSUPPORTED, "the plain form takes the FP32 option only for large blocks "
+                                                "(mixed-precision library LU); small blocks factor in FP64");
         sys->Nm = Nm;
         sys->Qm = Qm;
         sys->nm = nm;
@@ -846,6 +941,71 @@ kyn_system kyn_system_create(const kyn_model* models, size_t num_models, const k
         S.block_node_end = Ar.up(B.block_node_end);
         S.block_nodes = Ar.up(B.block_nodes);
         S.block_A_off = Ar.up(B.block_A_off);
+        S.block_large = Ar.up(block_large);
+        {
+            std::vector<int64_t> big_off(B.block_n.size(), 0);
+            for (size_t k = 0; k < sys->large_blocks.size(); ++k) big_off[sys->large_blocks[k]] = sys->large_off[k];
+            S.block_big_off = Ar.up(big_off);
+            S.Abig = Ar.zeros<double>(static_cast<size_t>(A_big));
+            S.Abig32 = (S.precision != KYN_TANGENT_FP64 && A_big > 0) ? Ar.zeros<float>(static_cast<size_t>(A_big)) : nullptr;
+            // work items (nodes, then constraints, of each large block) and node incidence lists
+            std::vector<int> items;
+            for (int b : sys->large_blocks) {
+                for (int k = B.block_node_begin[b]; k < B.block_node_end[b]; ++k) items.push_back(B.block_nodes[k]);
+                for (int k = B.block_con_begin[b]; k < B.block_con_end[b]; ++k) items.push_back(~B.block_cons[k]);
+            }
+            S.num_big_items = static_cast<int>(items.size());
+            S.big_item = Ar.up(items);
+            const size_t nn = B.node_block.size();
+            std::vector<std::vector<int>> ne(nn), nmass(nn), nspr(nn), ncon(nn);
+            for (size_t e = 0; e < B.elem_N.size(); ++e)
+                for (int i = 0; i < B.elem_N[e]; ++i) {
+                    auto& v = ne[B.elem_nodes[B.elem_node_off[e] + i]];
+                    v.push_back(static_cast<int>(e));
+                    v.push_back(i);
+                }
+            for (size_t k = 0; k < B.mass_node.size(); ++k) nmass[B.mass_node[k]].push_back(static_cast<int>(k));
+            for (size_t k = 0; k < B.spring_n1.size(); ++k) {
+                nspr[B.spring_n1[k]].push_back(static_cast<int>(k));
+                nspr[B.spring_n2[k]].push_back(static_cast<int>(k));
+            }
+            for (size_t k = 0; k < B.con_type.size(); ++k) {
+                ncon[B.con_target[k]].push_back(static_cast<int>(k));
+                if ((B.con_type[k] == 3 || B.con_type[k] == 4) && B.con_base[k] != B.con_target[k])
+                    ncon[B.con_base[k]].push_back(static_cast<int>(k));
+            }
+            auto csr = [&](const std::vector<std::vector<int>>& lists, int width, std::vector<int>& begin,
+                           std::vector<int>& flat) {
+                begin.assign(nn + 1, 0);
+                for (size_t i = 0; i < nn; ++i) {
+                    begin[i + 1] = begin[i] + static_cast<int>(lists[i].size()) / width;
+                    flat.insert(flat.end(), lists[i].begin(), lists[i].end());
+                }
+            };
+            std::vector<int> b1, f1, b2, f2, b3, f3, b4, f4;
+            csr(ne, 2, b1, f1);
+            csr(nmass, 1, b2, f2);
+            csr(nspr, 1, b3, f3);
+            csr(ncon, 1, b4, f4);
+            S.node_elem_begin = Ar.up(b1);
+            S.node_elem = Ar.up(f1);
+            S.node_mass_begin = Ar.up(b2);
+            S.node_mass = Ar.up(f2);
+            S.node_spring_begin = Ar.up(b3);
+            S.node_spring = Ar.up(f3);
+            S.node_con_begin = Ar.up(b4);
+            S.node_con = Ar.up(f4);
+            if (!sys->large_blocks.empty()) {
+                const int nbig = *std::max_element(sys->large_n.begin(), sys->large_n.end());
+                sys->ipiv = Ar.zeros<rocblas_int>(static_cast<size_t>(nbig));
+                sys->info = Ar.zeros<rocblas_int>(1);
+                sys->rvec = Ar.zeros<double>(static_cast<size_t>(nbig));
+                sys->bvec = Ar.zeros<double>(static_cast<size_t>(nbig));
+                sys->r32 = Ar.zeros<float>(static_cast<size_t>(nbig));
+                if (rocblas_create_handle(&sys->blas) != rocblas_status_success)
+                    throw KynError(KYN_ERR_HIP, "rocblas_create_handle failed");
+            }
+        }
         S.block_vec_off = Ar.up(B.block_vec_off);
         S.node_block = Ar.up(B.node_block);
         S.node_local = Ar.up(B.node_local);
@@ -954,6 +1114,8 @@ kyn_system kyn_system_create(const kyn_model* models, size_t num_models, const k
         sys->model_gdof_to_block_vec = B.gdof_maps;
 
         hip_check(hipStreamCreateWithFlags(&sys->stream, hipStreamNonBlocking), "hipStreamCreate");
+        if (sys->blas && rocblas_set_stream(sys->blas, sys->stream) != rocblas_status_success)
+            throw KynError(KYN_ERR_HIP, "rocblas_set_stream failed");
         hip_check(hipEventCreate(&sys->ev0), "event");
         hip_check(hipEventCreate(&sys->ev1), "event");
         hip_check(hipEventCreate(&sys->ek0), "event");

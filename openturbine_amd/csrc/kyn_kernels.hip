@@ -1524,7 +1524,7 @@ __global__ void __launch_bounds__(kThreads) k_plain_assemble(DevSys S) {
     __shared__ double rhs[512];
     const int b = blockIdx.x, t = threadIdx.x;
     const int m = S.block_model[b];
-    if (!S.model_active[m]) return;
+    if (!S.model_active[m] || S.block_large[b]) return;
     const int n = S.block_n[b];
     double* A = reinterpret_cast<double*>(S.A) + S.block_A_off[b];
     const int lda = n;
@@ -1632,7 +1632,7 @@ __global__ void __launch_bounds__(kThreads) k_plain_solve(DevSys S, int nm) {
     const Lay L = make_layout(1, 1, nm, 0, false, 8, true);
     const int b = blockIdx.x, t = threadIdx.x;
     const int m = S.block_model[b];
-    if (!S.model_active[m]) return;
+    if (!S.model_active[m] || S.block_large[b]) return;
     const int n = S.block_n[b];
     double* A = reinterpret_cast<double*>(sm + L.A);
     double* rhs = reinterpret_cast<double*>(sm + L.rhs);
@@ -1644,6 +1644,132 @@ __global__ void __launch_bounds__(kThreads) k_plain_solve(DevSys S, int nm) {
     for (int i = t; i < n; i += blockDim.x) rhs[i] = S.rhs[S.block_vec_off[b] + i];
     __syncthreads();
     factor_solve_uncondition<double>(S, b, A, nullptr, L.lda, piv, rhs, xs, xs2, sm, L);
+}
+
+// Large blocks: assembly of the column-major KKT matrix, one workgroup per item.  A node item
+// owns its 6 columns and its 6 rhs entries, a constraint item its rows' columns and rhs
+// entries; each entry receives its contributions in the order of k_plain_assemble (elements in
+// order, masses, springs, then constraints), so the matrix equals the small-block one.
+__global__ void __launch_bounds__(kThreads) k_large_assemble(DevSys S) {
+    const int item = S.big_item[blockIdx.x], t = threadIdx.x;
+    const bool is_node = item >= 0;
+    const int node = is_node ? item : -1, con = is_node ? -1 : ~item;
+    const int b = is_node ? S.node_block[node] : S.node_block[S.con_target[con]];
+    const int m = S.block_model[b];
+    if (!S.model_active[m]) return;
+    const int n = S.block_n[b], nsys = S.block_nsys[b], off = S.block_vec_off[b];
+    double* A = S.Abig + S.block_big_off[b];
+    const double cond = S.p.conditioner;
+    if (!is_node) {
+        const int rows = S.con_rows[con], lr = S.con_local[con];
+        const int type = S.con_type[con];
+        const bool two = (type == 3 || type == 4);
+        const int lt = S.node_local[S.con_target[con]];
+        const int lb = two ? S.node_local[S.con_base[con]] : 0;
+        for (int k = t; k < rows * n; k += blockDim.x) A[(size_t)(lr + k / n) * n + k % n] = 0.;
+        __syncthreads();
+        for (int q = t; q < rows * 6; q += blockDim.x) {
+            const int i = q / 6, j = q % 6;
+            A[(size_t)(lr + i) * n + lt + j] += S.con_tgradT[con * 36 + j * 6 + i];
+            if (two) A[(size_t)(lr + i) * n + lb + j] += S.con_bgradT[con * 36 + j * 6 + i];
+        }
+        if (t < rows) {
+            const double raw = 0. + S.con_res[con * 6 + t];
+            S.rhs_raw[off + lr + t] = raw;
+            S.rhs[off + lr + t] = -1. * raw;  // constraint rows are not scaled
+        }
+        return;
+    }
+    const int l = S.node_local[node];
+    for (int k = t; k < 6 * n; k += blockDim.x) A[(size_t)(l + k / n) * n + k % n] = 0.;
+    __syncthreads();
+    // beams (ContributeBeamsToSparseMatrix), elements in order
+    for (int q = S.node_elem_begin[node]; q < S.node_elem_begin[node + 1]; ++q) {
+        const int e = S.node_elem[2 * q], jl = S.node_elem[2 * q + 1];
+        const int N = S.elem_N[e], noff = S.elem_node_off[e];
+        const double* Me = S.elem_mat + S.elem_mat_off[e];
+        for (int k = t; k < N * 36; k += blockDim.x) {
+            const int i = k / 36, rc = k % 36;
+            const int row = S.node_local[S.elem_nodes[noff + i]] + rc / 6;
+            A[(size_t)(l + rc % 6) * n + row] += Me[(size_t)(i * N + jl) * 36 + rc] * cond;
+        }
+        __syncthreads();
+    }
+    // masses (diagonal node block)
+    for (int q = S.node_mass_begin[node]; q < S.node_mass_begin[node + 1]; ++q) {
+        const int e = S.node_mass[q];
+        if (t < 36) A[(size_t)(l + t % 6) * n + l + t / 6] += S.mass_mat[e * 36 + t] * cond;
+        __syncthreads();
+    }
+    // springs: this node's column block of [a, -a; -a, a]
+    for (int q = S.node_spring_begin[node]; q < S.node_spring_begin[node + 1]; ++q) {
+        const int e = S.node_spring[q];
+        const int n1 = S.spring_n1[e], n2 = S.spring_n2[e];
+        const int l1 = S.node_local[n1], l2 = S.node_local[n2];
+        if (t < 18) {
+            const int half = t / 9, rc = t % 9;  // row node (n1, n2) of the column node
+            const int lr = half ? l2 : l1;
+            const bool diag = (half == 0) == (node == n1);
+            A[(size_t)(l + rc % 3) * n + lr + rc / 3] += (diag ? 1. : -1.) * S.spring_mat[e * 9 + rc] * cond;
+        }
+        __syncthreads();
+    }
+    // constraint gradients in this node's columns (B T: constraint rows, node dofs)
+    for (int q = S.node_con_begin[node]; q < S.node_con_begin[node + 1]; ++q) {
+        const int c = S.node_con[q];
+        const int rows = S.con_rows[c], lr = S.con_local[c];
+        const bool tgt = S.con_target[c] == node;
+        const double* G = tgt ? S.con_tgrad + c * 36 : S.con_bgrad + c * 36;
+        for (int k = t; k < rows * 6; k += blockDim.x) {
+            const int i = k / 6, j = k % 6;
+            A[(size_t)(l + j) * n + lr + i] += G[i * 6 + j];
+        }
+        __syncthreads();
+    }
+    // residual entries of this node
+    if (t < 6) {
+        double r = 0.;
+        r -= S.f[node * 6 + t];
+        for (int q = S.node_elem_begin[node]; q < S.node_elem_begin[node + 1]; ++q) {
+            const int e = S.node_elem[2 * q], jl = S.node_elem[2 * q + 1];
+            r += S.elem_res[(size_t)(S.elem_node_off[e] + jl) * 6 + t];
+        }
+        for (int q = S.node_mass_begin[node]; q < S.node_mass_begin[node + 1]; ++q) r += S.mass_res[S.node_mass[q] * 6 + t];
+        for (int q = S.node_spring_begin[node]; q < S.node_spring_begin[node + 1]; ++q) {
+            const int e = S.node_spring[q];
+            if (t < 3) r += S.spring_res[e * 6 + (S.spring_n1[e] == node ? 0 : 3) + t];
+        }
+        for (int q = S.node_con_begin[node]; q < S.node_con_begin[node + 1]; ++q) {
+            const int c = S.node_con[q];
+            if (S.con_target[c] == node) r += S.con_sys_res[c * 6 + t];
+        }
+        for (int q = S.node_con_begin[node]; q < S.node_con_begin[node + 1]; ++q) {
+            const int c = S.node_con[q];
+            const int type = S.con_type[c];
+            if ((type == 3 || type == 4) && S.con_base[c] == node) r += S.con_blres[c * 6 + t];
+            if (S.con_target[c] == node) r += S.con_tlres[c * 6 + t];
+        }
+        S.rhs_raw[off + l + t] = r;
+        S.rhs[off + l + t] = -1. * (l + t < nsys ? r * cond : r);
+    }
+}
+
+// UnconditionSolution for a large block: x (in xsol) -> lambda rows divided by the conditioner
+__global__ void k_large_uncondition(DevSys S, int b) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = S.block_n[b], nsys = S.block_nsys[b], off = S.block_vec_off[b];
+    if (i >= n || !S.model_active[S.block_model[b]]) return;
+    if (i >= nsys) S.xsol[off + i] /= S.p.conditioner;
+}
+
+// mixed precision helpers for the large-block refinement
+__global__ void k_f64_to_f32(const double* x, float* y, size_t count) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) y[i] = (float)x[i];
+}
+__global__ void k_add_f32_to_f64(const float* d, double* x, int count) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) x[i] += (double)d[i];
 }
 
 // CalculateConvergenceError + UpdateStatePrediction + UpdateConstraintPrediction
@@ -1715,8 +1841,9 @@ void launch_advance_slot(const DevSys& S, hipStream_t st) {
 void launch_plain_predict(const DevSys& S, hipStream_t st) {
     hipLaunchKernelGGL(k_plain_predict, dim3(S.num_models), dim3(kThreads), 0, st, S);
 }
+void launch_plain_update(const DevSys& S, int iter, hipStream_t st);
 void launch_plain_iteration(const DevSys& S, int Nm, int Qm, int nm, int iter, bool solve,
-                            hipStream_t st, hipEvent_t e0, hipEvent_t e1) {
+                            hipStream_t st, hipEvent_t e0, hipEvent_t e1, bool defer_update) {
     hipLaunchKernelGGL(k_plain_clear_flag, dim3(1), dim3(64), 0, st, S);
     hipLaunchKernelGGL(k_plain_tangent, dim3((S.num_nodes + 255) / 256), dim3(256), 0, st, S);
     if (e0) (void)hipEventRecord(e0, st);
@@ -1728,9 +1855,23 @@ void launch_plain_iteration(const DevSys& S, int Nm, int Qm, int nm, int iter, b
     if (S.num_cons > 0)
         hipLaunchKernelGGL(k_plain_constraints, dim3((S.num_cons + 63) / 64), dim3(64), 0, st, S);
     hipLaunchKernelGGL(k_plain_assemble, dim3(S.num_blocks), dim3(kThreads), 0, st, S);
+    if (S.num_big_items > 0)
+        hipLaunchKernelGGL(k_large_assemble, dim3(S.num_big_items), dim3(kThreads), 0, st, S);
     if (!solve) return;
     hipLaunchKernelGGL(k_plain_solve, dim3(S.num_blocks), dim3(kThreads), solve_lds_bytes(nm), st, S, nm);
+    if (!defer_update) launch_plain_update(S, iter, st);
+}
+void launch_plain_update(const DevSys& S, int iter, hipStream_t st) {
     hipLaunchKernelGGL(k_plain_update, dim3(S.num_models), dim3(kThreads), 0, st, S, iter);
+}
+void launch_large_uncondition(const DevSys& S, int b, int n, hipStream_t st) {
+    hipLaunchKernelGGL(k_large_uncondition, dim3((n + 255) / 256), dim3(256), 0, st, S, b);
+}
+void launch_f64_to_f32(const double* x, float* y, size_t count, hipStream_t st) {
+    hipLaunchKernelGGL(k_f64_to_f32, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, st, x, y, count);
+}
+void launch_add_f32_to_f64(const float* d, double* x, int count, hipStream_t st) {
+    hipLaunchKernelGGL(k_add_f32_to_f64, dim3((count + 255) / 256), dim3(256), 0, st, d, x, count);
 }
 void launch_plain_epilogue(const DevSys& S, hipStream_t st) {
     hipLaunchKernelGGL(k_plain_epilogue, dim3(S.num_models), dim3(kThreads), 0, st, S);

@@ -95,6 +95,20 @@ struct DevSys {
     double *con_tgrad, *con_bgrad, *con_tgradT, *con_bgradT;  // [c][36] (plain form)
     double *con_tlres, *con_blres;                            // [c][6]
 
+    // ---- large blocks (more dofs than one workgroup's LU holds; plain form only): the KKT
+    //      matrix is assembled column-major in HBM by one workgroup per node / constraint
+    //      (its columns, contributions in the oracle's order) and factored by a library LU
+    const int* block_large;           // 1 if the block takes the large path
+    const int64_t* block_big_off;     // into Abig / Abig32 (n x n, column-major)
+    double* Abig;
+    float* Abig32;                    // FP32 copy for the mixed-precision factorization
+    int num_big_items;
+    const int* big_item;              // >= 0: node id, < 0: constraint ~id
+    const int *node_elem_begin, *node_elem;      // [n+1], pairs {element, local node index}
+    const int *node_mass_begin, *node_mass;      // [n+1], mass ids
+    const int *node_spring_begin, *node_spring;  // [n+1], spring ids
+    const int *node_con_begin, *node_con;        // [n+1], constraint ids (target or base)
+
     // ---- solver
     void* A;              // blocks' dense KKT matrices (double, or float for FP32 tangent)
     void* Acopy;          // unfactored copy for refinement (FP32 tangent with sweeps > 0)

@@ -473,6 +473,30 @@ def section_properties(seed: int, blade_key: int, num_stations: int = 21, size_f
     return s, out[0], out[1]
 
 
+def chain_section_properties(seed: int, chain_key: int, element: int, num_elements: int, num_stations: int = 5,
+                             size_factor: float = 1.0):
+    """Sections of element `element` of a chain of `num_elements`: the generator above evaluated
+    on the chain coordinate (the 20x taper runs root to tip over the whole chain, noise keyed by
+    chain and station), returned at element-local positions s in [0, 1]."""
+    s_loc = np.linspace(0.0, 1.0, num_stations)
+    s_chain = (element + s_loc) / num_elements
+    iu = np.triu_indices(6, 1)
+    out = []
+    for mat_id, base in enumerate((C1_MASS * size_factor**SCALE_EXPONENTS, C1_STIFF * size_factor**SCALE_EXPONENTS)):
+        st = (np.uint64(chain_key) * np.uint64(1 << 20) + np.uint64(element * num_stations)
+              + np.arange(num_stations, dtype=np.uint64)) * np.uint64(128)
+        keys = st[:, None] + np.uint64(mat_id * 64) + np.arange(21, dtype=np.uint64)[None, :]
+        u = uniform01(seed, keys)
+        d = base[None, :] * (20.0 ** (-s_chain))[:, None] * np.exp((2.0 * u[:, :6] - 1.0) * math.log(1.1))
+        A = np.zeros((num_stations, 6, 6))
+        A[:, np.arange(6), np.arange(6)] = d
+        off = 0.05 * np.sqrt(d[:, iu[0]] * d[:, iu[1]]) * (2.0 * u[:, 6:21] - 1.0)
+        A[:, iu[0], iu[1]] = off
+        A[:, iu[1], iu[0]] = off
+        out.append(_spd_project(A).reshape(num_stations, 36))
+    return s_loc, out[0], out[1]
+
+
 def _blade_frame(azimuth: float) -> np.ndarray:
     """Orientation taking material x onto the blade axis (0, cos a, sin a)."""
     return quat_compose(quat_from_rotvec([azimuth, 0.0, 0.0]), quat_from_rotvec([0.0, 0.0, math.pi / 2]))
@@ -631,3 +655,69 @@ def ensemble_schedule(specs, num_steps, first_step=0):
             loads[si, k:k + nb] = f[sp.meta["tips"]]
         k += nb
     return np.ascontiguousarray(cdisp), np.array(tips, dtype=np.uint64), loads
+
+
+def structure_config_spec(num_nodes=21, num_qps=40, tower_elems=8, shaft_elems=8, blade_elems=16,
+                          tower_height=80.0, shaft_length=8.0, blade_length=60.0, hub_radius=2.0,
+                          h=0.005, num_steps=1000, rho_inf=0.9, max_iter=50, atol=1e-10, rtol=1e-6,
+                          seed=SEED + 2, tip_load_sigma=1e4, gravity=(0.0, 0.0, -9.81)):
+    """One large structure: a tower of `tower_elems` elements along z (FixedBC at the base), a
+    shaft of `shaft_elems` elements along x from the tower top, a revolute joint (axis x, zero
+    torque) from the shaft end to a hub node, and three blades of `blade_elems` elements in the
+    y-z plane tied to the hub by rigid joints at `hub_radius`.  Every element has its own
+    `num_nodes` GLL nodes (a node's element location is unique, as in the reference's Node);
+    consecutive elements of a chain are joined by rigid joints between coincident end nodes.
+    Sections: the seeded generator along each chain, the 10 m base section scaled geometrically
+    to the chain (tower height / 10, shaft 4, blade length / 10; as the ensemble's blades).  Blade
+    tips take normal(0, sigma) loads per step.  All blocks form one KKT system (about 8k dofs at the default sizes)."""
+    nt = _NodeTable()
+    beams, cons = [], []
+    xi = gll_points(num_nodes - 1)
+    gx, gw = gauss_legendre(num_qps)
+    quad = np.stack([gx, gw], axis=1)
+
+    def chain(start, direction, frame, length, ne, key, size_factor):
+        ends = []
+        for k in range(ne):
+            ids = []
+            for x in xi:
+                sl = (x + 1) / 2
+                p = start + direction * length * (k + sl) / ne
+                ids.append(nt.add(np.concatenate([p, frame]), s=sl))
+            ss, Ms, Cs = chain_section_properties(seed, key, k, ne, size_factor=size_factor)
+            beams.append(BeamSpec(np.array(ids), ss, Ms, Cs, quad))
+            ends.append((ids[0], ids[-1]))
+        for k in range(1, ne):
+            cons.append(ConstraintSpec(RIGID_JOINT, ends[k - 1][1], ends[k][0]))
+        return ends
+
+    tower_frame = quat_from_rotvec([0.0, -math.pi / 2, 0.0])  # material x -> +z
+    tower = chain(np.zeros(3), np.array([0.0, 0.0, 1.0]), tower_frame, tower_height, tower_elems, 0, tower_height / 10.0)
+    cons.insert(0, ConstraintSpec(FIXED_BC, 0, tower[0][0]))
+    top = np.array([0.0, 0.0, tower_height])
+    shaft = chain(top, np.array([1.0, 0.0, 0.0]), np.array([1.0, 0, 0, 0]), shaft_length, shaft_elems, 1, 4.0)
+    cons.append(ConstraintSpec(RIGID_JOINT, tower[-1][1], shaft[0][0]))
+    hub_pos = top + np.array([shaft_length, 0.0, 0.0])
+    hub = nt.add(np.concatenate([hub_pos, [1.0, 0, 0, 0]]))
+    cons.append(ConstraintSpec(REVOLUTE_JOINT, shaft[-1][1], hub, axis=np.array([1.0, 0.0, 0.0])))
+    tips = []
+    for b in range(3):
+        az = 2.0 * math.pi * b / 3.0
+        d = np.array([0.0, math.cos(az), math.sin(az)])
+        blade = chain(hub_pos + hub_radius * d, d, _blade_frame(az), blade_length, blade_elems, 2 + b,
+                      blade_length / 10.0)
+        cons.append(ConstraintSpec(RIGID_JOINT, hub, blade[0][0]))
+        tips.append(blade[-1][1])
+    x0, u, v, vd, s = nt.arrays()
+    n = x0.shape[0]
+    tips = np.array(tips)
+
+    def step_inputs(step, _state=None):
+        f = np.zeros((n, 6))
+        keys = (np.uint64(step) * np.uint64(3) + np.arange(3, dtype=np.uint64)) * np.uint64(3)
+        f[tips, :3] = tip_load_sigma * normal(seed + 7, keys[:, None] + np.arange(3, dtype=np.uint64)[None, :])
+        return {"nodal_loads": f}
+
+    return ModelSpec("structure", x0, u, v, vd, s, beams, cons, np.array(gravity), max_iter=max_iter, h=h,
+                     rho_inf=rho_inf, atol=atol, rtol=rtol, num_steps=num_steps, step_inputs=step_inputs,
+                     meta={"tips": tips, "hub": hub})

@@ -1,7 +1,7 @@
 """GPU parity: the HIP library (through the C-ABI) against the CPU oracle and the
 reference's golden vectors.
 
-Bars (BASELINE north star): the FP64 GPU forms match the FP64 oracle to <= 1e-12 relative in
+Bars: the FP64 GPU forms match the FP64 oracle to <= 1e-12 relative in
 residuals and displacement/rotation state (relative = max |gpu - oracle| / max |oracle| per
 field) and reproduce the oracle's Newton iteration counts exactly; reference golden vectors
 are met at the reference's own tolerances.  Rates carry the generalized-alpha gains of the
