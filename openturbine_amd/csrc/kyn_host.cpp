@@ -35,6 +35,12 @@ void launch_plain_iteration(const DevSys& S, int Nm, int Qm, int nm, int iter, b
                             hipStream_t st, hipEvent_t e0, hipEvent_t e1, bool defer_update);
 void launch_plain_update(const DevSys& S, int iter, hipStream_t st);
 void launch_large_uncondition(const DevSys& S, int b, int n, hipStream_t st);
+size_t cond_lds(int ne, int me);
+void launch_cond_factor(const DevSys& S, size_t lds, bool fp32, const double* rin, hipStream_t st);
+void launch_cond_interface(const DevSys& S, int b, const double* rin, hipStream_t st);
+void launch_cond_backsub(const DevSys& S, double* xout, hipStream_t st);
+void launch_cond_scatter_z(const DevSys& S, int b, int nz, double* xout, hipStream_t st);
+void launch_block_axpy_copy(const DevSys& S, int b, int n, const double* d, double* x, double* r, hipStream_t st);
 void launch_f64_to_f32(const double* x, float* y, size_t count, hipStream_t st);
 void launch_add_f32_to_f64(const float* d, double* x, int count, hipStream_t st);
 void launch_plain_epilogue(const DevSys& S, hipStream_t st);
@@ -214,9 +220,13 @@ struct kyn_system_s {
     // large blocks (library LU): block ids, sizes, matrix offsets, pivots, scratch vectors
     std::vector<int> large_blocks, large_n, large_nsys;
     std::vector<int64_t> large_off;
+    std::vector<int> large_cond, large_nz;  // condensed solve (else dense), interface size
+    std::vector<int64_t> large_Z_off;
+    size_t cond_lds_bytes{0};
     rocblas_handle blas{};
     rocblas_int *ipiv{}, *info{};
     double *rvec{}, *bvec{};
+    double *rfull{}, *dfull{};  // refinement of condensed blocks: residual and correction (rhs layout)
     float* r32{};
     std::vector<int> model_active_host;
     // graph of one fused step
@@ -591,6 +601,15 @@ double* field_ptr(kyn_system_s* s, int field, int& width) {
     }
 }
 
+int zoff_host(const kyn_system_s* s, int b) {
+    int off = 0;
+    for (size_t k = 0; k < s->large_blocks.size(); ++k) {
+        if (s->large_blocks[k] == b) return off;
+        off += s->large_cond[k] ? s->large_nz[k] : 0;
+    }
+    return off;
+}
+
 void blas_check(rocblas_status st, const char* what) {
     if (st != rocblas_status_success)
         throw KynError(KYN_ERR_HIP, std::string(what) + " failed: " + rocblas_status_to_string(st));
@@ -605,9 +624,63 @@ void blas_check(rocblas_status st, const char* what) {
 void solve_large_blocks(kyn_system_s* s) {
     auto& S = s->S;
     const hipStream_t st = s->stream;
+    // condensed blocks: element eliminations (all elements at once), interface LU per block,
+    // element back substitution.  FP32 option: element factors in FP32, FP64 refinement sweeps
+    // (each re-runs the condensed solve on the FP64 residual r = b - A x).
+    if (S.num_cond_elems > 0) {
+        const bool fp32 = S.precision != KYN_TANGENT_FP64;
+        auto cond_solve = [&](const double* rin, double* xout) {
+            kyn::launch_cond_factor(S, s->cond_lds_bytes, fp32, rin, st);
+            hip_check(hipGetLastError(), "k_cond_factor launch");
+            for (size_t k = 0; k < s->large_blocks.size(); ++k) {
+                const int b = s->large_blocks[k], nz = s->large_nz[k];
+                if (!s->large_cond[k] || !s->model_active_host[s->block_model[b]]) continue;
+                kyn::launch_cond_interface(S, b, rin, st);
+                double* Z = S.Zbuf + s->large_Z_off[k];
+                double* z = S.zrhs + zoff_host(s, b);
+                blas_check(rocsolver_dgetrf(s->blas, nz, nz, Z, nz, s->ipiv, s->info), "rocsolver_dgetrf");
+                blas_check(rocsolver_dgetrs(s->blas, rocblas_operation_none, nz, 1, Z, nz, s->ipiv, z, nz),
+                           "rocsolver_dgetrs");
+            }
+            kyn::launch_cond_backsub(S, xout, st);
+            for (size_t k = 0; k < s->large_blocks.size(); ++k) {
+                const int b = s->large_blocks[k];
+                if (!s->large_cond[k] || !s->model_active_host[s->block_model[b]]) continue;
+                kyn::launch_cond_scatter_z(S, b, s->large_nz[k], xout, st);
+            }
+        };
+        if (!fp32) {
+            cond_solve(S.rhs, S.xsol);
+        } else {
+            cond_solve(S.rhs, S.xsol);
+            for (int sweep = 0; sweep < S.refine_sweeps; ++sweep) {
+                for (size_t k = 0; k < s->large_blocks.size(); ++k) {
+                    const int b = s->large_blocks[k], n = s->large_n[k];
+                    if (!s->large_cond[k] || !s->model_active_host[s->block_model[b]]) continue;
+                    // r = b - A x
+                    kyn::launch_block_axpy_copy(S, b, n, nullptr, nullptr, s->rfull, st);  // r = b
+                    const double alpha = -1., beta = 1.;
+                    blas_check(rocblas_dgemv(s->blas, rocblas_operation_none, n, n, &alpha, S.Abig + s->large_off[k], n,
+                                             S.xsol + s->block_vec_off[b], 1, &beta, s->rfull + s->block_vec_off[b], 1),
+                               "rocblas_dgemv");
+                }
+                cond_solve(s->rfull, s->dfull);
+                for (size_t k = 0; k < s->large_blocks.size(); ++k) {
+                    const int b = s->large_blocks[k], n = s->large_n[k];
+                    if (!s->large_cond[k] || !s->model_active_host[s->block_model[b]]) continue;
+                    kyn::launch_block_axpy_copy(S, b, n, s->dfull, S.xsol, nullptr, st);  // x += d
+                }
+            }
+        }
+        for (size_t k = 0; k < s->large_blocks.size(); ++k) {
+            const int b = s->large_blocks[k];
+            if (!s->large_cond[k] || !s->model_active_host[s->block_model[b]]) continue;
+            kyn::launch_large_uncondition(S, b, s->large_n[k], st);
+        }
+    }
     for (size_t k = 0; k < s->large_blocks.size(); ++k) {
         const int b = s->large_blocks[k], n = s->large_n[k];
-        if (!s->model_active_host[s->block_model[b]]) continue;
+        if (s->large_cond[k] || !s->model_active_host[s->block_model[b]]) continue;
         double* A = S.Abig + s->large_off[k];
         double* x = S.xsol + s->block_vec_off[b];
         const double* rhs = S.rhs + s->block_vec_off[b];
@@ -995,6 +1068,117 @@ kyn_system kyn_system_create(const kyn_model* models, size_t num_models, const k
             S.node_spring = Ar.up(f3);
             S.node_con_begin = Ar.up(b4);
             S.node_con = Ar.up(f4);
+            // condensed solve tables (FP64 factorization, no springs, elements with disjoint nodes)
+            {
+                std::vector<int> ce_elem, ce_block, rows_off{0}, rows, lam_off{0}, lam, lam_z, z_off(B.block_n.size() + 1, 0),
+                    zl, ce_begin(B.block_n.size() + 1, 0);
+                std::vector<int64_t> Y_off, S_off, Z_off(B.block_n.size(), 0);
+                int64_t Y_tot = 0, S_tot = 0, Z_tot = 0;
+                std::vector<int> nz_of(B.block_n.size(), 0), cond_of(B.block_n.size(), 0), nce_of(B.block_n.size(), 0);
+                for (int b : sys->large_blocks) {
+                    bool ok = true;
+                    for (size_t k = 0; k < B.spring_n1.size() && ok; ++k)
+                        if (B.node_block[B.spring_n1[k]] == b) ok = false;
+                    std::map<int, int> owner;  // node -> element
+                    for (int k = B.block_elem_begin[b]; k < B.block_elem_end[b] && ok; ++k) {
+                        const int e = B.block_elems[k];
+                        if (6 * B.elem_N[e] > kyn::kMaxBlockDofs) ok = false;
+                        for (int i = 0; i < B.elem_N[e] && ok; ++i)
+                            if (!owner.emplace(B.elem_nodes[B.elem_node_off[e] + i], e).second) ok = false;
+                    }
+                    std::vector<std::vector<int>> elam;
+                    size_t need = 0;
+                    if (ok) {
+                        for (int k = B.block_elem_begin[b]; k < B.block_elem_end[b]; ++k) {
+                            const int e = B.block_elems[k];
+                            std::vector<int> l;
+                            for (int q = B.block_con_begin[b]; q < B.block_con_end[b]; ++q) {
+                                const int c = B.block_cons[q];
+                                const bool two = B.con_type[c] == 3 || B.con_type[c] == 4;
+                                auto in_e = [&](int node) {
+                                    auto it = owner.find(node);
+                                    return it != owner.end() && it->second == e;
+                                };
+                                if (in_e(B.con_target[c]) || (two && in_e(B.con_base[c])))
+                                    for (int r = 0; r < B.con_rows[c]; ++r) l.push_back(B.con_local[c] + r);
+                            }
+                            if (static_cast<int>(l.size()) > kyn::kCondMaxLam) ok = false;
+                            need = std::max(need, kyn::cond_lds(6 * B.elem_N[e], static_cast<int>(l.size())));
+                            elam.push_back(l);
+                        }
+                        if (need > static_cast<size_t>(max_lds)) ok = false;
+                    }
+                    if (!ok) continue;
+                    cond_of[b] = 1;
+                    sys->cond_lds_bytes = std::max(sys->cond_lds_bytes, need);
+                    // interface rows: dofs of nodes outside every element, and all constraint rows
+                    std::vector<int> zidx(B.block_n[b], -1);
+                    for (int q = B.block_node_begin[b]; q < B.block_node_end[b]; ++q) {
+                        const int node = B.block_nodes[q];
+                        if (!owner.count(node))
+                            for (int c = 0; c < 6; ++c) zidx[B.node_local[node] + c] = 0;
+                    }
+                    for (int i = B.block_nsys[b]; i < B.block_n[b]; ++i) zidx[i] = 0;
+                    int nz = 0;
+                    for (int i = 0; i < B.block_n[b]; ++i)
+                        if (zidx[i] == 0) {
+                            zidx[i] = nz++;
+                            zl.push_back(i);
+                        }
+                    nz_of[b] = nz;
+                    Z_off[b] = Z_tot;
+                    Z_tot += static_cast<int64_t>(nz) * nz;
+                    int idx = 0;
+                    for (int k = B.block_elem_begin[b]; k < B.block_elem_end[b]; ++k, ++idx) {
+                        const int e = B.block_elems[k];
+                        ce_elem.push_back(e);
+                        ce_block.push_back(b);
+                        for (int i = 0; i < B.elem_N[e]; ++i)
+                            for (int c = 0; c < 6; ++c) rows.push_back(B.node_local[B.elem_nodes[B.elem_node_off[e] + i]] + c);
+                        rows_off.push_back(static_cast<int>(rows.size()));
+                        for (int r : elam[idx]) {
+                            lam.push_back(r);
+                            lam_z.push_back(zidx[r]);
+                        }
+                        lam_off.push_back(static_cast<int>(lam.size()));
+                        const int ne = 6 * B.elem_N[e], me = static_cast<int>(elam[idx].size());
+                        Y_off.push_back(Y_tot);
+                        Y_tot += static_cast<int64_t>(ne) * (me + 1);
+                        S_off.push_back(S_tot);
+                        S_tot += static_cast<int64_t>(me) * (me + 1);
+                    }
+                    nce_of[b] = B.block_elem_end[b] - B.block_elem_begin[b];
+                }
+                // per-block ranges (blocks in order; only condensed blocks are non-empty)
+                for (size_t b = 0; b + 1 < z_off.size(); ++b) {
+                    z_off[b + 1] = z_off[b] + nz_of[b];
+                    ce_begin[b + 1] = ce_begin[b] + nce_of[b];
+                }
+                S.num_cond_elems = static_cast<int>(ce_elem.size());
+                S.cond_elem = Ar.up(ce_elem);
+                S.cond_elem_block = Ar.up(ce_block);
+                S.cond_rows_off = Ar.up(rows_off);
+                S.cond_rows = Ar.up(rows);
+                S.cond_lam_off = Ar.up(lam_off);
+                S.cond_lam = Ar.up(lam);
+                S.cond_lam_z = Ar.up(lam_z);
+                S.cond_Y_off = Ar.up(Y_off);
+                S.cond_S_off = Ar.up(S_off);
+                S.Ybuf = Ar.zeros<double>(static_cast<size_t>(Y_tot));
+                S.Sbuf = Ar.zeros<double>(static_cast<size_t>(S_tot));
+                S.block_z_off = Ar.up(z_off);
+                S.cond_z = Ar.up(zl);
+                S.block_ce_begin = Ar.up(ce_begin);
+                S.block_Z_off = Ar.up(Z_off);
+                S.Zbuf = Ar.zeros<double>(static_cast<size_t>(Z_tot));
+                S.zrhs = Ar.zeros<double>(static_cast<size_t>(z_off.back()));
+                for (size_t k = 0; k < sys->large_blocks.size(); ++k) {
+                    const int b = sys->large_blocks[k];
+                    sys->large_cond.push_back(cond_of[b]);
+                    sys->large_nz.push_back(nz_of[b]);
+                    sys->large_Z_off.push_back(Z_off[b]);
+                }
+            }
             if (!sys->large_blocks.empty()) {
                 const int nbig = *std::max_element(sys->large_n.begin(), sys->large_n.end());
                 sys->ipiv = Ar.zeros<rocblas_int>(static_cast<size_t>(nbig));
@@ -1002,6 +1186,8 @@ kyn_system kyn_system_create(const kyn_model* models, size_t num_models, const k
                 sys->rvec = Ar.zeros<double>(static_cast<size_t>(nbig));
                 sys->bvec = Ar.zeros<double>(static_cast<size_t>(nbig));
                 sys->r32 = Ar.zeros<float>(static_cast<size_t>(nbig));
+                sys->rfull = Ar.zeros<double>(static_cast<size_t>(B.vec_total));
+                sys->dfull = Ar.zeros<double>(static_cast<size_t>(B.vec_total));
                 if (rocblas_create_handle(&sys->blas) != rocblas_status_success)
                     throw KynError(KYN_ERR_HIP, "rocblas_create_handle failed");
             }

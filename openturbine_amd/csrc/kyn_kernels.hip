@@ -777,10 +777,11 @@ __device__ __forceinline__ void lu_apply(TT* A, int n, int lda, int c0, int w, i
 // they use (physical rows of panel p's pivots, whether a row is still trailing) do not change
 // there.
 template <typename TT>
-__device__ __forceinline__ void lu_factor_augmented(TT* A, int n, int lda, int* piv, const LuScratch& W) {
+__device__ __forceinline__ void lu_factor_augmented(TT* A, int n, int lda, int* piv, const LuScratch& W,
+                                                    int nrhs = 1) {
     const int tid = threadIdx.x, wv = tid >> 6;
     const int nw = blockDim.x >> 6;
-    const int nc = n + 1;
+    const int nc = n + nrhs;  // right-hand sides ride along as columns n .. n + nrhs - 1
     for (int r = tid; r < kLuTile; r += blockDim.x) {
         W.lg_of[r] = r;
         W.phys_of[r] = r;
@@ -909,6 +910,28 @@ __device__ __forceinline__ void back_substitute(const TT* A, int n, int lda, con
             if (lane + 64 * a < n) xs[lane + 64 * a] = r[a];
     }
     __syncthreads();
+}
+
+// Back substitution U X = Y for the right-hand sides in columns n + c, c in [0, ncols), of the
+// factored augmented matrix; waves take columns in turn; X column c -> out[c * ldo + i] (HBM).
+template <typename TT>
+__device__ __forceinline__ void back_substitute_cols(const TT* A, int n, int lda, const int* phys_of, int ncols,
+                                                     double* out, int ldo) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    int pr[3];
+    double dg[3], dinv[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) pr[a] = lane + 64 * a < n ? phys_of[lane + 64 * a] : 0;
+    diag_of<TT>(A, n, lda, pr, dg, dinv);
+    for (int c = wv; c < ncols; c += nw) {
+        double r[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) r[a] = lane + 64 * a < n ? (double)A[pr[a] * lda + n + c] : 0.;
+        back_sweep<TT>(A, n, lda, pr, dg, dinv, r);
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+            if (lane + 64 * a < n) out[(size_t)c * ldo + lane + 64 * a] = r[a];
+    }
 }
 
 // Triangular solves with the LDS factors (logical row i at physical row phys_of[i]), FP64
@@ -1754,6 +1777,128 @@ __global__ void __launch_bounds__(kThreads) k_large_assemble(DevSys S) {
     }
 }
 
+// Condensed solve of a large block (static condensation onto the interface z: free nodes and
+// constraint rows).  Element e's dofs couple to the rest only through its constraint rows, so
+// with K_e its diagonal block, B_e the constraint rows touching it and b_e its rhs:
+//   x_e = K_e^-1 (b_e - B_e^T lambda),  (Z - sum_e B_e K_e^-1 B_e^T) z = r_z - sum_e B_e K_e^-1 b_e.
+// k_cond_factor: one workgroup per element, LU of [K_e | B_e^T | b_e] in LDS (the blocked LU
+// above, several right-hand sides), Y = K_e^-1 [B_e^T | b_e] -> Ybuf, S_e = B_e Y -> Sbuf.
+
+__host__ __device__ inline int cond_lds_bytes(int ne, int me) {
+    const int lda = (ne + me + 1) | 1;
+    return align16(ne * lda * 8) + 2 * kLuTile * 4 + 2 * 256 * 8 + align16(ne * 4);
+}
+
+// rin: right-hand side vector in the layout of S.rhs (the conditioned rhs, or a refinement
+// residual).  TT: storage type of the element factors (FP64, or FP32 with FP64 accumulation in
+// the triangular solves).
+template <typename TT>
+__device__ __forceinline__ void cond_factor_body(DevSys& S, const double* rin) {
+    extern __shared__ __align__(16) char sm[];
+    const int ce = blockIdx.x, t = threadIdx.x;
+    const int b = S.cond_elem_block[ce];
+    if (!S.model_active[S.block_model[b]]) return;
+    const int n = S.block_n[b];
+    const double* A = S.Abig + S.block_big_off[b];
+    const int r0 = S.cond_rows_off[ce], ne = S.cond_rows_off[ce + 1] - r0;
+    const int l0 = S.cond_lam_off[ce], me = S.cond_lam_off[ce + 1] - l0;
+    const int* rows = S.cond_rows + r0;
+    const int* lam = S.cond_lam + l0;
+    const int nrhs = me + 1;
+    const int lda = (ne + nrhs) | 1;
+    TT* M = reinterpret_cast<TT*>(sm);
+    int off = align16(ne * lda * 8);
+    int* lg = reinterpret_cast<int*>(sm + off);
+    int* ph = lg + kLuTile;
+    off += 2 * kLuTile * 4;
+    void* lc = sm + off;
+    off += 2 * 256 * 8;
+    int* piv = reinterpret_cast<int*>(sm + off);
+    const double* rhs = rin + S.block_vec_off[b];
+    for (int k = t; k < ne * (ne + nrhs); k += blockDim.x) {  // column by column: coalesced reads
+        const int i = k % ne, j = k / ne;
+        double v;
+        if (j < ne) v = A[(size_t)rows[j] * n + rows[i]];
+        else if (j < ne + me) v = A[(size_t)lam[j - ne] * n + rows[i]];
+        else v = rhs[rows[i]];
+        M[i * lda + j] = (TT)v;
+    }
+    __syncthreads();
+    const LuScratch W{lg, ph, lc};
+    lu_factor_augmented<TT>(M, ne, lda, piv, W, nrhs);
+    double* Y = S.Ybuf + S.cond_Y_off[ce];
+    back_substitute_cols<TT>(M, ne, lda, ph, nrhs, Y, ne);
+    __syncthreads();
+    double* Sm = S.Sbuf + S.cond_S_off[ce];
+    for (int q = t; q < me * nrhs; q += blockDim.x) {
+        const int k = q % me, c = q / me;
+        double acc = 0.;
+        for (int i = 0; i < ne; ++i) acc += A[(size_t)rows[i] * n + lam[k]] * Y[(size_t)c * ne + i];
+        Sm[(size_t)c * me + k] = acc;
+    }
+}
+__global__ void __launch_bounds__(kThreads) k_cond_factor_f64(DevSys S, const double* rin) {
+    cond_factor_body<double>(S, rin);
+}
+__global__ void __launch_bounds__(kThreads) k_cond_factor_f32(DevSys S, const double* rin) {
+    cond_factor_body<float>(S, rin);
+}
+
+// Interface system of block b: Z = A[z, z] - sum_e S_e (elements in order), r_z - sum_e t_e
+__global__ void __launch_bounds__(kThreads) k_cond_interface(DevSys S, int b, const double* rin) {
+    const int t = threadIdx.x;
+    if (!S.model_active[S.block_model[b]]) return;
+    const int n = S.block_n[b], z0 = S.block_z_off[b], nz = S.block_z_off[b + 1] - z0;
+    const int* zr = S.cond_z + z0;
+    const double* A = S.Abig + S.block_big_off[b];
+    double* Z = S.Zbuf + S.block_Z_off[b];
+    double* rz = S.zrhs + z0;
+    for (int k = t; k < nz * nz; k += blockDim.x) Z[k] = A[(size_t)zr[k / nz] * n + zr[k % nz]];
+    for (int i = t; i < nz; i += blockDim.x) rz[i] = rin[S.block_vec_off[b] + zr[i]];
+    __syncthreads();
+    for (int ce = S.block_ce_begin[b]; ce < S.block_ce_begin[b + 1]; ++ce) {
+        const int l0 = S.cond_lam_off[ce], me = S.cond_lam_off[ce + 1] - l0;
+        const int* lz = S.cond_lam_z + l0;
+        const double* Sm = S.Sbuf + S.cond_S_off[ce];
+        for (int q = t; q < me * me; q += blockDim.x) Z[(size_t)lz[q / me] * nz + lz[q % me]] -= Sm[q];
+        if (t < me) rz[lz[t]] -= Sm[(size_t)me * me + t];
+        __syncthreads();
+    }
+}
+
+// Element dofs from the interface solution: x_e = Y_b - Y_B lambda_e; interface rows copied
+__global__ void __launch_bounds__(kThreads) k_cond_backsub(DevSys S, double* xout) {
+    const int ce = blockIdx.x, t = threadIdx.x;
+    const int b = S.cond_elem_block[ce];
+    if (!S.model_active[S.block_model[b]]) return;
+    const int r0 = S.cond_rows_off[ce], ne = S.cond_rows_off[ce + 1] - r0;
+    const int l0 = S.cond_lam_off[ce], me = S.cond_lam_off[ce + 1] - l0;
+    const int* rows = S.cond_rows + r0;
+    const int* lz = S.cond_lam_z + l0;
+    const double* Y = S.Ybuf + S.cond_Y_off[ce];
+    const double* z = S.zrhs + S.block_z_off[b];
+    double* x = xout + S.block_vec_off[b];
+    for (int i = t; i < ne; i += blockDim.x) {
+        double v = Y[(size_t)me * ne + i];
+        for (int k = 0; k < me; ++k) v -= Y[(size_t)k * ne + i] * z[lz[k]];
+        x[rows[i]] = v;
+    }
+}
+__global__ void k_cond_scatter_z(DevSys S, int b, double* xout) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int z0 = S.block_z_off[b], nz = S.block_z_off[b + 1] - z0;
+    if (i >= nz || !S.model_active[S.block_model[b]]) return;
+    xout[S.block_vec_off[b] + S.cond_z[z0 + i]] = S.zrhs[z0 + i];
+}
+// refinement bookkeeping of one block: x += d (x non-null), r = rhs (r non-null, before r -= A x)
+__global__ void k_block_axpy_copy(DevSys S, int b, const double* d, double* x, double* r) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= S.block_n[b] || !S.model_active[S.block_model[b]]) return;
+    const int k = S.block_vec_off[b] + i;
+    if (x) x[k] += d[k];
+    if (r) r[k] = S.rhs[k];
+}
+
 // UnconditionSolution for a large block: x (in xsol) -> lambda rows divided by the conditioner
 __global__ void k_large_uncondition(DevSys S, int b) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1825,6 +1970,10 @@ hipError_t set_lds_limits(size_t bytes) {
     if (e != hipSuccess) return e;
     e = hipFuncSetAttribute((const void*)k_plain_element, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
     if (e != hipSuccess) return e;
+    e = hipFuncSetAttribute((const void*)k_cond_factor_f64, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (e != hipSuccess) return e;
+    e = hipFuncSetAttribute((const void*)k_cond_factor_f32, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (e != hipSuccess) return e;
     return hipFuncSetAttribute((const void*)k_plain_solve, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
 
@@ -1863,6 +2012,25 @@ void launch_plain_iteration(const DevSys& S, int Nm, int Qm, int nm, int iter, b
 }
 void launch_plain_update(const DevSys& S, int iter, hipStream_t st) {
     hipLaunchKernelGGL(k_plain_update, dim3(S.num_models), dim3(kThreads), 0, st, S, iter);
+}
+size_t cond_lds(int ne, int me) { return (size_t)cond_lds_bytes(ne, me); }
+void launch_cond_factor(const DevSys& S, size_t lds, bool fp32, const double* rin, hipStream_t st) {
+    if (fp32)
+        hipLaunchKernelGGL(k_cond_factor_f32, dim3(S.num_cond_elems), dim3(kThreads), lds, st, S, rin);
+    else
+        hipLaunchKernelGGL(k_cond_factor_f64, dim3(S.num_cond_elems), dim3(kThreads), lds, st, S, rin);
+}
+void launch_cond_interface(const DevSys& S, int b, const double* rin, hipStream_t st) {
+    hipLaunchKernelGGL(k_cond_interface, dim3(1), dim3(kThreads), 0, st, S, b, rin);
+}
+void launch_cond_backsub(const DevSys& S, double* xout, hipStream_t st) {
+    hipLaunchKernelGGL(k_cond_backsub, dim3(S.num_cond_elems), dim3(kThreads), 0, st, S, xout);
+}
+void launch_cond_scatter_z(const DevSys& S, int b, int nz, double* xout, hipStream_t st) {
+    hipLaunchKernelGGL(k_cond_scatter_z, dim3((nz + 255) / 256), dim3(256), 0, st, S, b, xout);
+}
+void launch_block_axpy_copy(const DevSys& S, int b, int n, const double* d, double* x, double* r, hipStream_t st) {
+    hipLaunchKernelGGL(k_block_axpy_copy, dim3((n + 255) / 256), dim3(256), 0, st, S, b, d, x, r);
 }
 void launch_large_uncondition(const DevSys& S, int b, int n, hipStream_t st) {
     hipLaunchKernelGGL(k_large_uncondition, dim3((n + 255) / 256), dim3(256), 0, st, S, b);

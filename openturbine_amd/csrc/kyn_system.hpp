@@ -18,6 +18,7 @@ constexpr int kMaxElemNodes = 21;  // register tile of the element contraction (
 constexpr int kMaxElemQps = 64;
 constexpr int kThreads = 256;      // workgroup size of every per-element / per-block kernel
 constexpr int kMaxBlockDofs = 143; // register LU tile holds n + 1 <= 144 columns ([A | b])
+constexpr int kCondMaxLam = 16;    // condensed solve: constraint rows touching one element
 
 struct StepConsts {
     double h, alpha_m, alpha_f, gamma, beta, gamma_prime, beta_prime, conditioner, atol, rtol;
@@ -108,6 +109,27 @@ struct DevSys {
     const int *node_mass_begin, *node_mass;      // [n+1], mass ids
     const int *node_spring_begin, *node_spring;  // [n+1], spring ids
     const int *node_con_begin, *node_con;        // [n+1], constraint ids (target or base)
+
+    // ---- condensed solve of large blocks whose elements have disjoint nodes: element dofs are
+    //      eliminated per element (LDS LU, one workgroup each), the interface (free nodes and
+    //      constraint rows, "z") is solved densely, then element dofs are recovered
+    int num_cond_elems;
+    const int* cond_elem;             // element id (element order within each block)
+    const int* cond_elem_block;
+    const int* cond_rows_off;         // [ce+1] into cond_rows (block-local rows of the element's dofs)
+    const int* cond_rows;
+    const int* cond_lam_off;          // [ce+1] into cond_lam / cond_lam_z
+    const int* cond_lam;              // block-local constraint rows touching the element
+    const int* cond_lam_z;            // their interface indices
+    const int64_t* cond_Y_off;        // into Ybuf: n_e x (m_e + 1), column-major (K_e^-1 [B_e^T | b_e])
+    const int64_t* cond_S_off;        // into Sbuf: m_e x (m_e + 1), column-major (B_e K_e^-1 [B_e^T | b_e])
+    double *Ybuf, *Sbuf;
+    const int* block_z_off;           // [b+1] into cond_z (interface rows of the block, ascending)
+    const int* cond_z;
+    const int* block_ce_begin;        // [b+1] condensed elements of the block
+    const int64_t* block_Z_off;       // into Zbuf (nz x nz column-major)
+    double* Zbuf;
+    double* zrhs;                     // [sum nz]
 
     // ---- solver
     void* A;              // blocks' dense KKT matrices (double, or float for FP32 tangent)
