@@ -18,6 +18,9 @@ HIP-event-timed duration; flop rates (contraction 144 N^2 Q; with LU 2/3 n^3 and
 as extra fields; measured traffic from the committed counter summary.
 cpu_baseline: the CPU oracle (same algorithm, dense pivoted LU instead of KLU) on the same
 ensemble (all rotors, 4 steps by default), OpenMP over rotors on the host cores.
+structure (extra field): the large single structure (tower 8 + shaft 8 + 3 x 16 blade elements
+of 21 nodes / 40 points, joints, 8459 dofs) in the plain form, FP64 and FP32 element factors
+with two FP64 refinement sweeps; steps/s and the element kernel's algorithmic HBM rate.
 """
 
 import argparse
@@ -121,6 +124,34 @@ def copy_calibration(nbytes):
           flush=True)
 
 
+def structure_legs(lib, steps):
+    """Large single structure, plain form: host-driven steps (tip loads set each step)."""
+    spec = W.structure_config_spec(num_steps=steps + 1)
+    out = {"workload": "tower 8 + shaft 8 + 3 blades x 16 elements, N=21, Q=40, rigid joints between "
+                       "elements, revolute hub joint, h=0.005, tip loads normal(0, 1e4 N)",
+           "dofs": int(spec.num_nodes * 6 + sum(5 if c.type == W.REVOLUTE_JOINT else 6 for c in spec.constraints))}
+    elem_bytes = sum(8.0 * (67 * len(b.nodes) + 87 * len(b.quad)) + 36.0 * 8 * len(b.nodes) ** 2 for b in spec.beams)
+    for name, prec, refine in (("fp64", ota.TANGENT_FP64, 0), ("fp32", ota.TANGENT_FP32, 2)):
+        g = ota.KynSystem(lib, spec, exec_mode=ota.EXEC_PLAIN, precision=prec, refine_sweeps=refine)
+        assert g.step(spec.step_inputs(0)) == 0  # warm-up
+        iters, kms, fails = 0, 0.0, 0
+        t0 = time.perf_counter()
+        for k in range(1, steps + 1):
+            fails += g.step(spec.step_inputs(k)) != 0
+            iters += len(g.convergence())
+            kms += g.last_kernel_ms()
+        wall = time.perf_counter() - t0
+        out[name] = {"steps_per_s": round(steps / wall, 3), "newton_iters_per_step": round(iters / steps, 2),
+                     "failed_steps": fails}
+        if name == "fp64":
+            ms = kms / steps
+            out["element_kernel"] = {"kernel": "k_plain_element", "ms_per_launch": round(ms, 4),
+                                     "algorithmic_bytes_per_launch": elem_bytes,
+                                     "GB_s": round(elem_bytes / (ms / 1e3) / 1e9, 1)}
+        log(f"  structure {name}: {steps} steps in {wall:.3f}s, {iters / steps:.1f} iters/step, fails {fails}")
+    return out
+
+
 def cpu_baseline(specs_sample, steps):
     sys.path.insert(0, str(ROOT / "tests"))
     from oracle_lib import OracleLib, OracleSystem, step_many
@@ -150,6 +181,7 @@ def main():
     ap.add_argument("--skip-plain", action="store_true")
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--refine", type=int, default=0, help="FP64 refinement sweeps of the FP32-tangent solve")
+    ap.add_argument("--structure-steps", type=int, default=10, help="steps of the large-structure legs (0: skip)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run one device copy of BYTES (counter calibration) and exit")
     args = ap.parse_args()
@@ -198,6 +230,11 @@ def main():
         log(f"  {args.plain_steps} steps in {rp['wall_s']:.3f}s, fails {rp['fails']}")
         del rp["system"]
 
+    structure = None
+    if args.structure_steps > 0:
+        log("leg: large structure (plain form, condensed solve)")
+        structure = structure_legs(lib, args.structure_steps)
+
     value = args.steps / r32["wall_s"]
     ms_per_step = 1000.0 * r32["wall_s"] / args.steps
     # roofline of the fused FP32 step kernel (HIP events around each graph launch of the timed
@@ -243,6 +280,7 @@ def main():
                      "all_flops_per_launch": flops_launch,
                      "all_TFLOPs": round(flops_launch / kern_s / 1e12, 3)},
         "cpu_baseline": None,
+        "structure": structure,
     }
     if not args.skip_cpu:
         sample = specs[: args.cpu_rotors]
