@@ -174,6 +174,11 @@ int kyn_get_constraint_values(kyn_system s, size_t id, int which, double* out);
 int kyn_get_rhs(kyn_system s, size_t model, double* b);
 int kyn_get_solution(kyn_system s, size_t model, double* x);
 
+/* HostState::CopyFromState (src/interfaces/host_state.hpp:48-53) in one transfer: per node
+ * x[7], q[7], v[6], vd[6] packed on the device as out[n][26] (the reference makes four
+ * device-to-host copies per step). */
+int kyn_get_motion(kyn_system s, double* out);
+
 /* ---- state management: CopyStateData (copy_state_data.hpp:20-34), restart files ---- */
 int kyn_save_state(kyn_system s);
 int kyn_restore_state(kyn_system s);
@@ -187,6 +192,82 @@ int kyn_read_restart(kyn_system s, size_t model, const unsigned char* in, size_t
 int kyn_debug_assemble(kyn_system s);
 int kyn_get_element_terms(kyn_system s, size_t element, double* residual /* [N][6] */,
                           double* system_matrix /* [N][N][36] */);
+
+/* ---- model inspection (host only) ---- */
+size_t kyn_model_num_nodes(kyn_model m);
+int kyn_model_get_node(kyn_model m, size_t id, double x0[7], double u[7], double v[6], double vd[6], double* s);
+/* sizes of beam element `elem`, then its node ids [N], sections {s [S], M* [S][36], C* [S][36]}
+ * and quadrature {xi, w} [Q][2]; NULL outputs are skipped */
+int kyn_model_beam_element_info(kyn_model m, size_t elem, size_t* num_nodes, size_t* num_sections,
+                                size_t* num_qps);
+int kyn_model_get_beam_element(kyn_model m, size_t elem, size_t* node_ids, double* section_s,
+                               double* section_mass, double* section_stiffness, double* qp_xi_w);
+
+/* ---- blade interface: interfaces::BladeInterface (src/interfaces/blade/blade_interface.hpp:17-66)
+ *      built from components::SolutionInput and components::BeamInput
+ *      (src/interfaces/components/solution_input.hpp, beam_input.hpp) ---- */
+typedef struct kyn_blade_interface_s* kyn_blade_interface;
+
+#define KYN_REF_AXIS_X 0 /* components::ReferenceAxisOrientation::X */
+#define KYN_REF_AXIS_Z 1 /* ::Z: reference axis points and section matrices rotated from z onto x */
+
+typedef struct {
+    double gravity[3];
+    int dynamic_solve;
+    double time_step;
+    double rho_inf;
+    size_t max_iter;
+    double absolute_error_tolerance;
+    double relative_error_tolerance;
+} kyn_solution_input;
+
+typedef struct {
+    size_t element_order;              /* nodes - 1 */
+    size_t section_refinement;         /* interpolated sections between given ones */
+    int reference_axis_orientation;    /* KYN_REF_AXIS_X or KYN_REF_AXIS_Z */
+    size_t num_ref_axis_points;
+    const double* ref_axis_grid;       /* [k] positions in [0, 1] */
+    const double* ref_axis_coordinates;/* [k][3] */
+    size_t num_twist_points;
+    const double* twist_grid;          /* [t] */
+    const double* twist;               /* [t], as BeamInput::ref_axis.twist */
+    size_t num_sections;
+    const double* section_grid;        /* [s] */
+    const double* section_mass;        /* [s][36], row-major */
+    const double* section_stiffness;   /* [s][36], row-major */
+    int prescribe_root_motion;         /* adds a PrescribedBC on the root node */
+    double root_position[7];           /* {x, y, z, qw, qx, qy, qz} */
+    double root_velocity[6];
+    double root_acceleration[6];
+} kyn_beam_input;
+
+/* components::Beam::Beam (beam.cpp:17-24) alone, host only: adds the blade's nodes, its beam
+ * element and (if requested) the root PrescribedBC to `m`; returns their ids. */
+int kyn_beam_build_model(kyn_model m, const kyn_beam_input* in, int64_t* beam_element_id,
+                         int64_t* root_constraint_id);
+/* BladeInterface::BladeInterface (blade_interface.cpp:11-45); options may be NULL (fused FP64) */
+kyn_blade_interface kyn_blade_interface_create(const kyn_solution_input* solution, const kyn_beam_input* blade,
+                                               const kyn_exec_options* options);
+void kyn_blade_interface_destroy(kyn_blade_interface bi);
+size_t kyn_blade_interface_num_nodes(kyn_blade_interface bi);
+/* NodeData::loads of blade node `node` (beam.hpp: nodes[node].loads) */
+int kyn_blade_interface_set_node_loads(kyn_blade_interface bi, size_t node, const double loads[6]);
+int kyn_blade_interface_get_node_loads(kyn_blade_interface bi, size_t node, double loads[6]);
+/* Beam::AddPointLoad (beam.cpp:33-45) and Beam::ClearLoads (:47-51) */
+int kyn_blade_interface_add_point_load(kyn_blade_interface bi, double s, const double loads[6]);
+int kyn_blade_interface_clear_loads(kyn_blade_interface bi);
+/* BladeInterface::Step (blade_interface.cpp:54-82): KYN_OK, or KYN_NOT_CONVERGED (node motion
+ * then keeps the previous step's values) */
+int kyn_blade_interface_step(kyn_blade_interface bi);
+/* NodeData::position / displacement / velocity / acceleration (node_data.cpp:12-21); NULL skips */
+int kyn_blade_interface_get_node_motion(kyn_blade_interface bi, size_t node, double position[7],
+                                        double displacement[7], double velocity[6], double acceleration[6]);
+/* BladeInterface::SaveState / RestoreState (blade_interface.cpp:84-96), SetRootDisplacement (:98-103) */
+int kyn_blade_interface_save_state(kyn_blade_interface bi);
+int kyn_blade_interface_restore_state(kyn_blade_interface bi);
+int kyn_blade_interface_set_root_displacement(kyn_blade_interface bi, const double u[7]);
+/* the underlying system (state access, restart files) */
+kyn_system kyn_blade_interface_system(kyn_blade_interface bi);
 
 #ifdef __cplusplus
 }

@@ -48,6 +48,58 @@ def build(force: bool = True) -> None:
     subprocess.run(["make", "-B" if force else "-s", "-C", str(PKG)], check=True)
 
 
+class SolutionInput(C.Structure):
+    """kyn_solution_input (components::SolutionInput)."""
+    _fields_ = [("gravity", C.c_double * 3), ("dynamic_solve", C.c_int), ("time_step", C.c_double),
+                ("rho_inf", C.c_double), ("max_iter", C.c_size_t), ("absolute_error_tolerance", C.c_double),
+                ("relative_error_tolerance", C.c_double)]
+
+
+class BeamInput(C.Structure):
+    """kyn_beam_input (components::BeamInput); pointer fields refer to arrays kept by the caller."""
+    _fields_ = [("element_order", C.c_size_t), ("section_refinement", C.c_size_t),
+                ("reference_axis_orientation", C.c_int), ("num_ref_axis_points", C.c_size_t),
+                ("ref_axis_grid", C.POINTER(C.c_double)), ("ref_axis_coordinates", C.POINTER(C.c_double)),
+                ("num_twist_points", C.c_size_t), ("twist_grid", C.POINTER(C.c_double)),
+                ("twist", C.POINTER(C.c_double)), ("num_sections", C.c_size_t),
+                ("section_grid", C.POINTER(C.c_double)), ("section_mass", C.POINTER(C.c_double)),
+                ("section_stiffness", C.POINTER(C.c_double)), ("prescribe_root_motion", C.c_int),
+                ("root_position", C.c_double * 7), ("root_velocity", C.c_double * 6),
+                ("root_acceleration", C.c_double * 6)]
+
+
+REF_AXIS_X = 0
+REF_AXIS_Z = 1
+
+
+def beam_input(*, element_order, ref_grid, ref_coords, twist_grid, twist, section_grid, section_mass,
+               section_stiffness, orientation=REF_AXIS_X, section_refinement=0, prescribe_root_motion=False,
+               root_position=(0.0, 0, 0, 1, 0, 0, 0), root_velocity=(0.0,) * 6, root_acceleration=(0.0,) * 6):
+    """A BeamInput plus the arrays it points to (keep the returned tuple alive while in use)."""
+    keep = [np.ascontiguousarray(a, dtype=np.float64) for a in
+            (ref_grid, np.asarray(ref_coords).reshape(-1, 3), twist_grid, twist, section_grid,
+             np.asarray(section_mass).reshape(-1, 36), np.asarray(section_stiffness).reshape(-1, 36))]
+    b = BeamInput()
+    b.element_order = int(element_order)
+    b.section_refinement = int(section_refinement)
+    b.reference_axis_orientation = int(orientation)
+    b.num_ref_axis_points = len(keep[0])
+    b.ref_axis_grid = keep[0].ctypes.data_as(_dp)
+    b.ref_axis_coordinates = keep[1].ctypes.data_as(_dp)
+    b.num_twist_points = len(keep[2])
+    b.twist_grid = keep[2].ctypes.data_as(_dp)
+    b.twist = keep[3].ctypes.data_as(_dp)
+    b.num_sections = len(keep[4])
+    b.section_grid = keep[4].ctypes.data_as(_dp)
+    b.section_mass = keep[5].ctypes.data_as(_dp)
+    b.section_stiffness = keep[6].ctypes.data_as(_dp)
+    b.prescribe_root_motion = int(bool(prescribe_root_motion))
+    b.root_position[:] = list(root_position)
+    b.root_velocity[:] = list(root_velocity)
+    b.root_acceleration[:] = list(root_acceleration)
+    return b, keep
+
+
 def _d(a):
     a = np.ascontiguousarray(a, dtype=np.float64)
     return a, a.ctypes.data_as(_dp)
@@ -116,6 +168,30 @@ class KynLib:
         L.kyn_read_restart.argtypes = [C.c_void_p, C.c_size_t, C.POINTER(C.c_ubyte), C.c_size_t]
         L.kyn_debug_assemble.argtypes = [C.c_void_p]
         L.kyn_get_element_terms.argtypes = [C.c_void_p, C.c_size_t, _dp, _dp]
+        L.kyn_get_motion.argtypes = [C.c_void_p, _dp]
+        L.kyn_model_num_nodes.argtypes = [C.c_void_p]
+        L.kyn_model_num_nodes.restype = C.c_size_t
+        L.kyn_model_get_node.argtypes = [C.c_void_p, C.c_size_t, _dp, _dp, _dp, _dp, _dp]
+        L.kyn_model_beam_element_info.argtypes = [C.c_void_p, C.c_size_t, _sp, _sp, _sp]
+        L.kyn_model_get_beam_element.argtypes = [C.c_void_p, C.c_size_t, _sp, _dp, _dp, _dp, _dp]
+        L.kyn_beam_build_model.argtypes = [C.c_void_p, C.POINTER(BeamInput), C.POINTER(C.c_int64),
+                                           C.POINTER(C.c_int64)]
+        L.kyn_blade_interface_create.argtypes = [C.POINTER(SolutionInput), C.POINTER(BeamInput),
+                                                 C.POINTER(ExecOptions)]
+        L.kyn_blade_interface_create.restype = C.c_void_p
+        L.kyn_blade_interface_destroy.argtypes = [C.c_void_p]
+        L.kyn_blade_interface_num_nodes.argtypes = [C.c_void_p]
+        L.kyn_blade_interface_num_nodes.restype = C.c_size_t
+        for f in ("kyn_blade_interface_set_node_loads", "kyn_blade_interface_get_node_loads"):
+            getattr(L, f).argtypes = [C.c_void_p, C.c_size_t, _dp]
+        L.kyn_blade_interface_add_point_load.argtypes = [C.c_void_p, C.c_double, _dp]
+        for f in ("kyn_blade_interface_clear_loads", "kyn_blade_interface_step", "kyn_blade_interface_save_state",
+                  "kyn_blade_interface_restore_state"):
+            getattr(L, f).argtypes = [C.c_void_p]
+        L.kyn_blade_interface_get_node_motion.argtypes = [C.c_void_p, C.c_size_t, _dp, _dp, _dp, _dp]
+        L.kyn_blade_interface_set_root_displacement.argtypes = [C.c_void_p, _dp]
+        L.kyn_blade_interface_system.argtypes = [C.c_void_p]
+        L.kyn_blade_interface_system.restype = C.c_void_p
 
     def error(self) -> str:
         return self.L.kyn_last_error().decode()
@@ -166,6 +242,106 @@ class KynLib:
 
     def model_add_spring_element(self, m, n1, n2, k, l0):
         return self.L.kyn_model_add_spring_element(m, int(n1), int(n2), float(k), float(l0))
+
+
+class KynBladeInterface:
+    """interfaces::BladeInterface through the C-ABI (kyn_blade_interface_*)."""
+
+    def __init__(self, lib: KynLib, solution: dict, beam, *, exec_mode=EXEC_FUSED, precision=TANGENT_FP64,
+                 refine_sweeps=0):
+        self.lib = lib
+        sol = SolutionInput()
+        sol.gravity[:] = list(solution.get("gravity", (0.0, 0.0, 0.0)))
+        sol.dynamic_solve = int(solution.get("dynamic_solve", True))
+        sol.time_step = float(solution["time_step"])
+        sol.rho_inf = float(solution.get("rho_inf", 0.0))
+        sol.max_iter = int(solution.get("max_iter", 5))
+        sol.absolute_error_tolerance = float(solution.get("atol", 1e-5))
+        sol.relative_error_tolerance = float(solution.get("rtol", 1e-3))
+        b, self._keep = beam
+        opts = ExecOptions(exec_mode, precision, refine_sweeps)
+        self.h = lib.L.kyn_blade_interface_create(C.byref(sol), C.byref(b), C.byref(opts))
+        if not self.h:
+            raise RuntimeError("kyn_blade_interface_create failed: " + lib.error())
+        self.n = lib.L.kyn_blade_interface_num_nodes(self.h)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.lib.L.kyn_blade_interface_destroy(self.h)
+            self.h = None
+
+    def _check(self, rc):
+        if rc < 0:
+            raise RuntimeError(self.lib.error())
+        return rc
+
+    def set_node_loads(self, node, loads):
+        a, p = _d(loads)
+        self._check(self.lib.L.kyn_blade_interface_set_node_loads(self.h, int(node), p))
+
+    def node_loads(self, node):
+        out = np.zeros(6)
+        self._check(self.lib.L.kyn_blade_interface_get_node_loads(self.h, int(node), out.ctypes.data_as(_dp)))
+        return out
+
+    def add_point_load(self, s, loads):
+        a, p = _d(loads)
+        self._check(self.lib.L.kyn_blade_interface_add_point_load(self.h, float(s), p))
+
+    def clear_loads(self):
+        self._check(self.lib.L.kyn_blade_interface_clear_loads(self.h))
+
+    def step(self):
+        return self._check(self.lib.L.kyn_blade_interface_step(self.h))
+
+    def motion(self, node):
+        """(position, displacement, velocity, acceleration) of one blade node."""
+        out = [np.zeros(7), np.zeros(7), np.zeros(6), np.zeros(6)]
+        self._check(self.lib.L.kyn_blade_interface_get_node_motion(self.h, int(node), *[o.ctypes.data_as(_dp) for o in out]))
+        return out
+
+    def save_state(self):
+        self._check(self.lib.L.kyn_blade_interface_save_state(self.h))
+
+    def restore_state(self):
+        self._check(self.lib.L.kyn_blade_interface_restore_state(self.h))
+
+    def set_root_displacement(self, u):
+        a, p = _d(u)
+        self._check(self.lib.L.kyn_blade_interface_set_root_displacement(self.h, p))
+
+    def system_handle(self):
+        return self.lib.L.kyn_blade_interface_system(self.h)
+
+
+def built_beam_model(lib: KynLib, beam):
+    """Runs the C-ABI beam builder on a host-only model; returns (nodes, beam element) as arrays:
+    nodes {x0, u, v, vd, s}, element {node_ids, section_s, M, C, quad}."""
+    b, keep = beam
+    m = lib.L.kyn_model_create()
+    try:
+        be = C.c_int64(-1)
+        rc_ = C.c_int64(-1)
+        rc = lib.L.kyn_beam_build_model(m, C.byref(b), C.byref(be), C.byref(rc_))
+        if rc != 0:
+            raise RuntimeError(lib.error())
+        n = lib.L.kyn_model_num_nodes(m)
+        x0, u, v, vd, s = np.zeros((n, 7)), np.zeros((n, 7)), np.zeros((n, 6)), np.zeros((n, 6)), np.zeros(n)
+        for i in range(n):
+            sv = C.c_double(0.0)
+            lib.L.kyn_model_get_node(m, i, x0[i].ctypes.data_as(_dp), u[i].ctypes.data_as(_dp),
+                                     v[i].ctypes.data_as(_dp), vd[i].ctypes.data_as(_dp), C.byref(sv))
+            s[i] = sv.value
+        nn, ns, nq = C.c_size_t(), C.c_size_t(), C.c_size_t()
+        lib.L.kyn_model_beam_element_info(m, be.value, C.byref(nn), C.byref(ns), C.byref(nq))
+        ids = np.zeros(nn.value, dtype=np.uintp)
+        ss, M, Cm, q = np.zeros(ns.value), np.zeros((ns.value, 36)), np.zeros((ns.value, 36)), np.zeros((nq.value, 2))
+        lib.L.kyn_model_get_beam_element(m, be.value, ids.ctypes.data_as(_sp), ss.ctypes.data_as(_dp),
+                                         M.ctypes.data_as(_dp), Cm.ctypes.data_as(_dp), q.ctypes.data_as(_dp))
+        return ({"x0": x0, "u": u, "v": v, "vd": vd, "s": s},
+                {"node_ids": ids, "section_s": ss, "M": M, "C": Cm, "quad": q, "root_constraint": rc_.value})
+    finally:
+        lib.L.kyn_model_destroy(m)
 
 
 class KynSystem:

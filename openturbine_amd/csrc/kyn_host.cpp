@@ -42,6 +42,7 @@ void launch_cond_backsub(const DevSys& S, double* xout, hipStream_t st);
 void launch_cond_scatter_z(const DevSys& S, int b, int nz, double* xout, hipStream_t st);
 void launch_block_axpy_copy(const DevSys& S, int b, int n, const double* d, double* x, double* r, hipStream_t st);
 void launch_f64_to_f32(const double* x, float* y, size_t count, hipStream_t st);
+void launch_pack_motion(const DevSys& S, double* out, hipStream_t st);
 void launch_add_f32_to_f64(const float* d, double* x, int count, hipStream_t st);
 void launch_plain_epilogue(const DevSys& S, hipStream_t st);
 }  // namespace kyn
@@ -49,6 +50,14 @@ void launch_plain_epilogue(const DevSys& S, hipStream_t st);
 namespace {
 
 thread_local std::string g_err;
+
+}  // namespace
+
+namespace kyn {
+void set_last_error(const std::string& msg) { g_err = msg; }
+}  // namespace kyn
+
+namespace {
 
 struct KynError : std::runtime_error {
     int code;
@@ -227,6 +236,7 @@ struct kyn_system_s {
     rocblas_int *ipiv{}, *info{};
     double *rvec{}, *bvec{};
     double *rfull{}, *dfull{};  // refinement of condensed blocks: residual and correction (rhs layout)
+    double* motion{};           // packed x, q, v, vd for kyn_get_motion (allocated on first use)
     float* r32{};
     std::vector<int> model_active_host;
     // graph of one fused step
@@ -1598,6 +1608,56 @@ int kyn_get_rhs(kyn_system s, size_t model, double* b) { return get_vec(s, model
 int kyn_get_solution(kyn_system s, size_t model, double* x) { return get_vec(s, model, x, s ? s->S.xsol : nullptr); }
 
 // CopyStateData (copy_state_data.hpp:20-34): x, q_delta, q_prev, q, v, vd, a, f
+int kyn_get_motion(kyn_system s, double* out) {
+    return guarded([&] {
+        if (!s || !out) throw KynError(KYN_ERR_INVALID, "null argument");
+        if (!s->motion) s->motion = s->arena.zeros<double>(s->num_nodes * 26);
+        kyn::launch_pack_motion(s->S, s->motion, s->stream);
+        hip_check(hipGetLastError(), "k_pack_motion launch");
+        hip_check(hipMemcpyAsync(out, s->motion, s->num_nodes * 26 * sizeof(double), hipMemcpyDeviceToHost, s->stream),
+                  "D2H");
+        hip_check(hipStreamSynchronize(s->stream), "sync");
+        return KYN_OK;
+    });
+}
+
+size_t kyn_model_num_nodes(kyn_model m) { return m ? m->nodes.size() : 0; }
+int kyn_model_get_node(kyn_model m, size_t id, double x0[7], double u[7], double v[6], double vd[6], double* s) {
+    if (!m || id >= m->nodes.size()) return KYN_ERR_INVALID;
+    const auto& n = m->nodes[id];
+    if (x0) std::copy(n.x0.begin(), n.x0.end(), x0);
+    if (u) std::copy(n.u.begin(), n.u.end(), u);
+    if (v) std::copy(n.v.begin(), n.v.end(), v);
+    if (vd) std::copy(n.vd.begin(), n.vd.end(), vd);
+    if (s) *s = n.s;
+    return KYN_OK;
+}
+int kyn_model_beam_element_info(kyn_model m, size_t elem, size_t* num_nodes, size_t* num_sections, size_t* num_qps) {
+    if (!m || elem >= m->beams.size()) return KYN_ERR_INVALID;
+    const auto& b = m->beams[elem];
+    if (num_nodes) *num_nodes = b.nodes.size();
+    if (num_sections) *num_sections = b.sections.size();
+    if (num_qps) *num_qps = b.quad.size();
+    return KYN_OK;
+}
+int kyn_model_get_beam_element(kyn_model m, size_t elem, size_t* node_ids, double* section_s, double* section_mass,
+                               double* section_stiffness, double* qp_xi_w) {
+    if (!m || elem >= m->beams.size()) return KYN_ERR_INVALID;
+    const auto& b = m->beams[elem];
+    if (node_ids) std::copy(b.nodes.begin(), b.nodes.end(), node_ids);
+    for (size_t k = 0; k < b.sections.size(); ++k) {
+        if (section_s) section_s[k] = b.sections[k].s;
+        if (section_mass) std::copy(b.sections[k].M.begin(), b.sections[k].M.end(), section_mass + 36 * k);
+        if (section_stiffness) std::copy(b.sections[k].C.begin(), b.sections[k].C.end(), section_stiffness + 36 * k);
+    }
+    if (qp_xi_w)
+        for (size_t q = 0; q < b.quad.size(); ++q) {
+            qp_xi_w[2 * q] = b.quad[q][0];
+            qp_xi_w[2 * q + 1] = b.quad[q][1];
+        }
+    return KYN_OK;
+}
+
 int kyn_save_state(kyn_system s) {
     return guarded([&] {
         if (!s) throw KynError(KYN_ERR_INVALID, "null system");

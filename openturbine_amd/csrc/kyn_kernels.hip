@@ -1907,6 +1907,23 @@ __global__ void k_large_uncondition(DevSys S, int b) {
     if (i >= nsys) S.xsol[off + i] /= S.p.conditioner;
 }
 
+// HostState::CopyFromState in one transfer: per node x[7], q[7], v[6], vd[6] -> out[n][26]
+__global__ void k_pack_motion(DevSys S, double* out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= S.num_nodes) return;
+    double* o = out + (size_t)i * 26;
+#pragma unroll
+    for (int c = 0; c < 7; ++c) {
+        o[c] = S.x[i * 7 + c];
+        o[7 + c] = S.q[i * 7 + c];
+    }
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+        o[14 + c] = S.v[i * 6 + c];
+        o[20 + c] = S.vd[i * 6 + c];
+    }
+}
+
 // mixed precision helpers for the large-block refinement
 __global__ void k_f64_to_f32(const double* x, float* y, size_t count) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -2034,6 +2051,9 @@ void launch_block_axpy_copy(const DevSys& S, int b, int n, const double* d, doub
 }
 void launch_large_uncondition(const DevSys& S, int b, int n, hipStream_t st) {
     hipLaunchKernelGGL(k_large_uncondition, dim3((n + 255) / 256), dim3(256), 0, st, S, b);
+}
+void launch_pack_motion(const DevSys& S, double* out, hipStream_t st) {
+    hipLaunchKernelGGL(k_pack_motion, dim3((S.num_nodes + 255) / 256), dim3(256), 0, st, S, out);
 }
 void launch_f64_to_f32(const double* x, float* y, size_t count, hipStream_t st) {
     hipLaunchKernelGGL(k_f64_to_f32, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, st, x, y, count);

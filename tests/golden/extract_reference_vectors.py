@@ -16,6 +16,9 @@ Sources (paths relative to the reference root):
   tests/regression_tests/regression/test_masses.cpp,
   tests/regression_tests/regression/test_precession.cpp,
   tests/regression_tests/regression/test_spring_mass_system.cpp -> masses_springs.json
+  tests/regression_tests/interfaces/test_blade_interface.cpp,
+  tests/regression_tests/interfaces/interfaces_test_files/IEA-15-240-RWT.yaml
+                                                             -> blade_interface.json
 
 Usage: python tests/golden/extract_reference_vectors.py [/root/reference]
 """
@@ -269,8 +272,67 @@ def masses_springs(root):
     return {"masses_external_force": ext, "precession": precession, "spring_mass": spring_mass}
 
 
+def blade_interface(root):
+    """BladeInterfaceTest.RotatingBeam and .BladeWindIO: builder inputs, solution parameters,
+    per-step inputs and the expected node states.  The windIO blade data come from the YAML
+    file the test reads (loaded with yaml.safe_load: data only)."""
+    import yaml
+
+    src = strip_comments((root / "tests/regression_tests/interfaces/test_blade_interface.cpp").read_text())
+    rb = src[src.index("BladeInterfaceTest, RotatingBeam") : src.index("WriteMatrixToFile")]
+    sect = rb[rb.index("builder.Blade().AddSection(") :]
+    mats = re.findall(r"std::array\{\s*(std::array\{[^;]*?\})\s*,\s*std::array\{\s*(std::array\{[^;]*?\}),\s*interfaces", sect, re.S)
+    M = numbers(mats[0][0])
+    K = numbers(mats[0][1])
+    rotating = {
+        "source": "tests/regression_tests/interfaces/test_blade_interface.cpp (BladeInterfaceTest.RotatingBeam)",
+        "time_step": numbers(re.search(r"time_step\{([^}]*)\}", rb).group(1))[0],
+        "omega": numbers(re.search(r"omega = std::array\{([^}]*)\}", rb).group(1)),
+        "x0_root": numbers(re.search(r"x0_root = std::array\{([^}]*)\}", rb).group(1)),
+        "max_iter": numbers(re.search(r"SetMaximumNonlinearIterations\(([^)]*)\)", rb).group(1))[0],
+        "atol": numbers(re.search(r"SetAbsoluteErrorTolerance\(([^)]*)\)", rb).group(1))[0],
+        "rtol": numbers(re.search(r"SetRelativeErrorTolerance\(([^)]*)\)", rb).group(1))[0],
+        "rho_inf": numbers(re.search(r"SetDampingFactor\(([^)]*)\)", rb).group(1))[0],
+        "element_order": numbers(re.search(r"SetElementOrder\(([^)]*)\)", rb).group(1))[0],
+        "node_s": numbers(balanced(rb, rb.index("{", rb.index("node_s = std::vector")))),
+        "section_s": numbers(balanced(rb, rb.index("{", rb.index("section_s")))),
+        "mass_matrix": M,
+        "stiffness_matrix": K,
+        "num_steps": numbers(re.search(r"iota\(1U, (\d+)U\)", rb).group(1))[0] - 1,
+        "expect": near_values(rb, r"^interface\.Blade\(\)\.nodes"),
+    }
+    wi = src[src.index("BladeInterfaceTest, BladeWindIO") : src.index("BladeInterfaceTest, RotatingBeam")]
+    wio = yaml.safe_load((root / "tests/regression_tests/interfaces/interfaces_test_files/IEA-15-240-RWT.yaml").read_text())
+    blade = wio["components"]["blade"]
+    ra = blade["outer_shape_bem"]["reference_axis"]
+    six = blade["elastic_properties_mb"]["six_x_six"]
+    windio = {
+        "source": "tests/regression_tests/interfaces/test_blade_interface.cpp (BladeInterfaceTest.BladeWindIO), "
+                  "blade data from interfaces_test_files/IEA-15-240-RWT.yaml",
+        "time_step": numbers(re.search(r"time_step\{([^}]*)\}", wi).group(1))[0],
+        "max_iter": numbers(re.search(r"SetMaximumNonlinearIterations\(([^)]*)\)", wi).group(1))[0],
+        "atol": numbers(re.search(r"SetAbsoluteErrorTolerance\(([^)]*)\)", wi).group(1))[0],
+        "rtol": numbers(re.search(r"SetRelativeErrorTolerance\(([^)]*)\)", wi).group(1))[0],
+        "rho_inf": numbers(re.search(r"SetDampingFactor\(([^)]*)\)", wi).group(1))[0],
+        "n_nodes": numbers(re.search(r"n_nodes\{([^}]*)\}", wi).group(1))[0],
+        "num_steps": numbers(re.search(r"iota\(1U, (\d+)U\)", wi).group(1))[0] - 1,
+        "tip_moment_amplitude": numbers(re.search(r"loads\[1\] = ([^*]+)\*", wi).group(1))[0],
+        "tip_moment_frequency": numbers(re.search(r"numbers::pi \* ([\d.]+) \* t", wi).group(1))[0],
+        "ref_axis_grid": ra["x"]["grid"],
+        "ref_axis_x": ra["x"]["values"], "ref_axis_y": ra["y"]["values"], "ref_axis_z": ra["z"]["values"],
+        "twist_grid": blade["outer_shape_bem"]["twist"]["grid"],
+        "twist_values_deg": blade["outer_shape_bem"]["twist"]["values"],
+        "section_grid": six["stiff_matrix"]["grid"],
+        "mass_upper21": six["inertia_matrix"]["values"],
+        "stiffness_upper21": six["stiff_matrix"]["values"],
+        "expect": near_values(wi, r"^tip_node\.position"),
+    }
+    return {"rotating_beam": rotating, "windio": windio}
+
+
 def main():
     root = Path(sys.argv[1] if len(sys.argv) > 1 else "/root/reference")
+    (HERE / "blade_interface.json").write_text(json.dumps(blade_interface(root), indent=1))
     (HERE / "solver_regression.json").write_text(json.dumps(solver_regression(root), indent=1))
     (HERE / "cantilever_sine.json").write_text(json.dumps(cantilever(root), indent=1))
     (HERE / "iea15_rotor.json").write_text(json.dumps(iea15(root)))

@@ -20,7 +20,10 @@ def test_header_declares_the_boundary():
     names = declared_functions()
     for required in ("kyn_model_create", "kyn_model_add_beam_element", "kyn_system_create", "kyn_step",
                      "kyn_get_state", "kyn_set_constraint_displacement", "kyn_get_convergence_history",
-                     "kyn_write_restart", "kyn_model_add_mass_element", "kyn_model_add_spring_element"):
+                     "kyn_write_restart", "kyn_model_add_mass_element", "kyn_model_add_spring_element",
+                     "kyn_get_motion", "kyn_beam_build_model", "kyn_blade_interface_create",
+                     "kyn_blade_interface_step", "kyn_blade_interface_get_node_motion",
+                     "kyn_blade_interface_set_root_displacement"):
         assert required in names
 
 
