@@ -41,6 +41,7 @@ extern "C" {
 #define KYN_PRESCRIBED_BC 2
 #define KYN_RIGID_JOINT 3
 #define KYN_REVOLUTE_JOINT 4
+#define KYN_ROTATION_CONTROL 5 /* base rotation composed with a commanded rotation about `axis` */
 
 /* ---- execution form ---- */
 #define KYN_EXEC_PLAIN 0 /* one kernel per stage, host reads the error every iteration */
@@ -106,8 +107,10 @@ int64_t kyn_model_add_mass_element(kyn_model m, size_t node, const double mass_m
  * beam, a mass or a constraint (six dofs), else kyn_system_create returns KYN_ERR_UNSUPPORTED. */
 int64_t kyn_model_add_spring_element(kyn_model m, size_t node1, size_t node2, double stiffness,
                                      double undeformed_length);
-/* Model::AddFixedBC / AddPrescribedBC / AddRigidJointConstraint / AddRevoluteJointConstraint
- * (model.hpp:477-541). For one-node constraints `base` is ignored. axis and init may be NULL. */
+/* Model::AddFixedBC / AddPrescribedBC / AddRigidJointConstraint / AddRevoluteJointConstraint /
+ * AddRotationControl (model.hpp:477-560). For one-node constraints `base` is ignored. axis and
+ * init may be NULL.  A rotation control constraint takes its commanded angle (radians about the
+ * unit `axis`) from kyn_set_constraint_control. */
 int64_t kyn_model_add_constraint(kyn_model m, int type, size_t base, size_t target,
                                  const double axis[3], const double init_disp[7]);
 
@@ -151,7 +154,8 @@ double kyn_last_kernel_ms(kyn_system s);
 int kyn_set_constraint_displacement(kyn_system s, size_t id, const double disp[7]);
 int kyn_set_constraint_displacements(kyn_system s, size_t count, const size_t* ids,
                                      const double* disp /* [count][7] */);
-/* control signal read at UpdateViews (constraints.hpp:346-353): revolute torque */
+/* control signal read at UpdateViews (constraints.hpp:346-353): revolute torque, or the
+ * commanded angle of a rotation control constraint */
 int kyn_set_constraint_control(kyn_system s, size_t id, double value);
 /* State::f nodal loads (blade_interface.cpp:56-61), all nodes [n][6] */
 int kyn_set_nodal_loads(kyn_system s, const double* f);

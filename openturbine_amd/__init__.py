@@ -423,6 +423,8 @@ class KynSystem:
         for e, fx in inputs.get("element_node_loads", {}).items():
             a, p = _d(fx)
             self._check(self.lib.L.kyn_set_element_node_loads(self.h, e, p))
+        for cid, val in inputs.get("control", {}).items():
+            self._check(self.lib.L.kyn_set_constraint_control(self.h, self.con_off[model] + cid, float(val)))
 
     def set_schedule(self, num_steps, cdisp=None, load_nodes=None, loads=None):
         cp = None

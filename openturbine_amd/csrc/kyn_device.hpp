@@ -4,6 +4,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include "kyn_system.hpp"
+
 namespace kyn {
 
 #define KYN_DEV __device__ __forceinline__
@@ -455,9 +457,9 @@ KYN_DEV void ax_matrix(const double* A, double* R) {
     }
 }
 
-// Constraint residual and gradients for FixedBC, PrescribedBC, RigidJoint, RevoluteJoint
-// (src/constraints/calculate_constraint_residual_gradient.hpp:61-565 and the kernels
-// calculate_{fixed_bc,prescribed_bc,rigid_joint,revolute_joint}_constraint.hpp,
+// Constraint residual and gradients for FixedBC, PrescribedBC, RigidJoint, RevoluteJoint,
+// RotationControl (src/constraints/calculate_constraint_residual_gradient.hpp:61-565 and the
+// kernels calculate_{fixed_bc,prescribed_bc,rigid_joint,revolute_joint,rotation_control}_constraint.hpp,
 // calculate_revolute_joint_force.hpp).  Outputs: res[6]; tg, bg: gradient (not yet times
 // tangent); sys_res[3..5] (revolute only).
 KYN_DEV void constraint_eval(int type, const double* X0, const double* axes, const double* input,
@@ -474,7 +476,7 @@ KYN_DEV void constraint_eval(int type, const double* X0, const double* axes, con
     if (type == 2) {
         u1[0] = input[0]; u1[1] = input[1]; u1[2] = input[2];
         R1[0] = input[3]; R1[1] = input[4]; R1[2] = input[5]; R1[3] = input[6];
-    } else if (type == 3 || type == 4) {
+    } else if (con_two_node(type)) {
         u1[0] = bq[0]; u1[1] = bq[1]; u1[2] = bq[2];
         R1[0] = bq[3]; R1[1] = bq[4]; R1[2] = bq[5]; R1[3] = bq[6];
     }
@@ -485,9 +487,20 @@ KYN_DEV void constraint_eval(int type, const double* X0, const double* axes, con
     quat_rotate(R1, X0, R1X0);
 #pragma unroll
     for (int k = 0; k < 3; ++k) res[k] = u2[k] + X0[k] - u1[k] - R1X0[k];
-    if (type != 4) {
+    if (type != kConRevoluteJoint) {
         double R2R1t[4], C[9];
-        quat_compose(R2, R1t, R2R1t);
+        if (type == kConRotationControl) {
+            // calculate_rotation_control_constraint.hpp:85-100: C = R(R2 inv(Rc) inv(R1)) with
+            // Rc the commanded rotation input(0) about the unit axis
+            const double rv[3] = {axes[0] * input[0], axes[1] * input[0], axes[2] * input[0]};
+            double Rc[4], RcT[4], R2RcT[4];
+            rotvec_to_quat(rv, Rc);
+            quat_inverse(Rc, RcT);
+            quat_compose(R2, RcT, R2RcT);
+            quat_compose(R2RcT, R1t, R2R1t);
+        } else {
+            quat_compose(R2, R1t, R2R1t);
+        }
         quat_to_rot(R2R1t, C);
         res[3] = (C[7] - C[5]) / 2.;
         res[4] = (C[2] - C[6]) / 2.;
@@ -501,7 +514,7 @@ KYN_DEV void constraint_eval(int type, const double* X0, const double* axes, con
         for (int i = 0; i < 3; ++i)
 #pragma unroll
             for (int j = 0; j < 3; ++j) tg[(3 + i) * 6 + 3 + j] = A[j * 3 + i];
-        if (type == 3) {
+        if (type == kConRigidJoint || type == kConRotationControl) {
             bg[0] = -1.;
             bg[7] = -1.;
             bg[14] = -1.;

@@ -59,6 +59,11 @@ void set_last_error(const std::string& msg) { g_err = msg; }
 
 namespace {
 
+using kyn::con_num_rows;
+using kyn::con_two_node;
+using kyn::kConFixedBC;
+using kyn::kConRotationControl;
+
 struct KynError : std::runtime_error {
     int code;
     KynError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
@@ -323,8 +328,8 @@ void add_model(Builder& B, const kyn_model_s& mo) {
         if (bm.sections.empty()) throw KynError(KYN_ERR_INVALID, "beam element without sections");
     }
     for (const auto& c : mo.constraints) {
-        if (c.type < 1 || c.type > 4) throw KynError(KYN_ERR_UNSUPPORTED, "unsupported constraint type");
-        if (c.target >= n || ((c.type == 3 || c.type == 4) && c.base >= n))
+        if (c.type < kConFixedBC || c.type > kConRotationControl) throw KynError(KYN_ERR_UNSUPPORTED, "unsupported constraint type");
+        if (c.target >= n || (con_two_node(c.type) && c.base >= n))
             throw KynError(KYN_ERR_INVALID, "constraint references an unknown node");
     }
 
@@ -342,7 +347,7 @@ void add_model(Builder& B, const kyn_model_s& mo) {
     for (const auto& ms : mo.masses) has[ms.node] = 1;
     for (const auto& c : mo.constraints) {
         has[c.target] = 1;
-        if (c.type == 3 || c.type == 4) has[c.base] = 1;
+        if (con_two_node(c.type)) has[c.base] = 1;
     }
     // a spring alone gives its nodes the three translational dofs (FreedomSignature::JustPosition);
     // the device state and block layout carry six dofs per node, so spring nodes must also belong
@@ -361,7 +366,7 @@ void add_model(Builder& B, const kyn_model_s& mo) {
     int nrows = 0;
     for (size_t k = 0; k < mo.constraints.size(); ++k) {
         row_first[k] = nrows;
-        nrows += mo.constraints[k].type == 4 ? 5 : 6;
+        nrows += con_num_rows(mo.constraints[k].type);
     }
     const int ndofs = nsys + nrows;
 
@@ -376,7 +381,7 @@ void add_model(Builder& B, const kyn_model_s& mo) {
     for (const auto& bm : mo.beams)
         for (size_t k = 1; k < bm.nodes.size(); ++k) unite(bm.nodes[0], bm.nodes[k]);
     for (const auto& c : mo.constraints)
-        if (c.type == 3 || c.type == 4) unite(c.base, c.target);
+        if (con_two_node(c.type)) unite(c.base, c.target);
     for (const auto& sp : mo.springs) unite(sp.n1, sp.n2);
     std::map<size_t, int> root_block;
     std::vector<std::vector<int>> bnodes, bcons, belems, bmasses, bsprings;
@@ -422,7 +427,7 @@ void add_model(Builder& B, const kyn_model_s& mo) {
             B.block_cons.push_back(con0 + ck);
             con_loc[ck] = l;
             con_blk[ck] = gb;
-            const int rows = mo.constraints[ck].type == 4 ? 5 : 6;
+            const int rows = con_num_rows(mo.constraints[ck].type);
             for (int r = 0; r < rows; ++r) gmap[nsys + row_first[ck] + r] = B.vec_total + l + r;
             l += rows;
         }
@@ -526,11 +531,11 @@ void add_model(Builder& B, const kyn_model_s& mo) {
     // ---- constraints (constraints.hpp:76-266)
     for (size_t k = 0; k < mo.constraints.size(); ++k) {
         const auto& c = mo.constraints[k];
-        const bool two = (c.type == 3 || c.type == 4);
+        const bool two = con_two_node(c.type);
         B.con_type.push_back(c.type);
         B.con_base.push_back(two ? node0 + static_cast<int>(c.base) : node0 + static_cast<int>(c.target));
         B.con_target.push_back(node0 + static_cast<int>(c.target));
-        B.con_rows.push_back(c.type == 4 ? 5 : 6);
+        B.con_rows.push_back(con_num_rows(c.type));
         B.con_model_row.push_back(row_first[k]);
         B.con_local.push_back(con_loc[k]);
         B.con_block.push_back(con_blk[k]);
@@ -560,6 +565,12 @@ void add_model(Builder& B, const kyn_model_s& mo) {
             ax = {cp[0] * cp[0] * kk + dp, cp[1] * cp[0] * kk + cp[2], cp[2] * cp[0] * kk - cp[1],
                   cp[0] * cp[1] * kk - cp[2], cp[1] * cp[1] * kk + dp, cp[2] * cp[1] * kk + cp[0],
                   cp[0] * cp[2] * kk + cp[1], cp[1] * cp[2] * kk - cp[0], cp[2] * cp[2] * kk + dp};
+        } else if (c.type == kConRotationControl) {  // constraints.hpp:324-331: the unit axis
+            const double na = std::sqrt(c.axis[0] * c.axis[0] + c.axis[1] * c.axis[1] + c.axis[2] * c.axis[2]);
+            if (na == 0.) throw KynError(KYN_ERR_INVALID, "Cannot normalize a zero vector");
+            ax[0] = c.axis[0] / na;
+            ax[1] = c.axis[1] / na;
+            ax[2] = c.axis[2] / na;
         } else {
             ax[0] = c.axis[0];
             ax[1] = c.axis[1];
@@ -1054,7 +1065,7 @@ kyn_system kyn_system_create(const kyn_model* models, size_t num_models, const k
             }
             for (size_t k = 0; k < B.con_type.size(); ++k) {
                 ncon[B.con_target[k]].push_back(static_cast<int>(k));
-                if ((B.con_type[k] == 3 || B.con_type[k] == 4) && B.con_base[k] != B.con_target[k])
+                if (con_two_node(B.con_type[k]) && B.con_base[k] != B.con_target[k])
                     ncon[B.con_base[k]].push_back(static_cast<int>(k));
             }
             auto csr = [&](const std::vector<std::vector<int>>& lists, int width, std::vector<int>& begin,
@@ -1104,7 +1115,7 @@ kyn_system kyn_system_create(const kyn_model* models, size_t num_models, const k
                             std::vector<int> l;
                             for (int q = B.block_con_begin[b]; q < B.block_con_end[b]; ++q) {
                                 const int c = B.block_cons[q];
-                                const bool two = B.con_type[c] == 3 || B.con_type[c] == 4;
+                                const bool two = con_two_node(B.con_type[c]);
                                 auto in_e = [&](int node) {
                                     auto it = owner.find(node);
                                     return it != owner.end() && it->second == e;

@@ -1102,12 +1102,12 @@ __device__ __forceinline__ void block_constraints(const DevSys& S, int b, char* 
     if (t < nc) {
         const int c = S.block_cons[c0 + t];
         const int type = S.con_type[c];
-        const bool two = (type == 3 || type == 4);
+        const bool two = con_two_node(type);
         const int tn = S.con_target[c], bn = S.con_base[c];
         double input[7];
 #pragma unroll
         for (int k = 0; k < 7; ++k) input[k] = S.con_input[c * 7 + k];
-        if (type == 4) input[0] = S.con_control[c];  // Constraints::UpdateViews
+        if (con_controlled(type)) input[0] = S.con_control[c];  // Constraints::UpdateViews
         double tq[7], bq[7] = {0., 0., 0., 1., 0., 0., 0.};
 #pragma unroll
         for (int k = 0; k < 7; ++k) tq[k] = S.q[tn * 7 + k];
@@ -1183,7 +1183,7 @@ __device__ __forceinline__ void block_constraints(const DevSys& S, int b, char* 
         for (int k = 0; k < nc; ++k) {
             const int c = S.block_cons[c0 + k];
             const int type = S.con_type[c];
-            if (type == 3 || type == 4) {
+            if (con_two_node(type)) {
                 const int lb = S.node_local[S.con_base[c]];
                 for (int j = 0; j < 6; ++j) rhs[lb + j] += cres[k * 24 + 6 + j];
             }
@@ -1349,6 +1349,11 @@ __device__ __forceinline__ void fused_step_body(const DevSys& S, int Nm, int Qm,
             const int n = S.block_n[b];
             const int e = S.block_elems[S.block_elem_begin[b]];
             KYN_PHASE_MARK(0);
+            // every system row starts at -f (a node outside the element, e.g. a massless hub,
+            // takes only its load and constraint terms); the element rows are then written as
+            // -f + R after the element phase's first barrier
+            for (int i = t; i < S.block_nsys[b]; i += blockDim.x)
+                rhs[i] = -S.f[S.block_nodes[S.block_node_begin[b] + i / 6] * 6 + i % 6];
             element_phase<TT, true>(S, e, sm, L, n);
             KYN_PHASE_MARK(5);
             block_constraints<TT>(S, b, sm, L);
@@ -1491,11 +1496,11 @@ __global__ void k_plain_constraints(DevSys S) {
     const int m = node_model(S, tn);
     if (m < 0 || !S.model_active[m]) return;
     const int type = S.con_type[c];
-    const bool two = (type == 3 || type == 4);
+    const bool two = con_two_node(type);
     double input[7];
 #pragma unroll
     for (int k = 0; k < 7; ++k) input[k] = S.con_input[c * 7 + k];
-    if (type == 4) input[0] = S.con_control[c];
+    if (con_controlled(type)) input[0] = S.con_control[c];
     double tq[7], bq[7] = {0., 0., 0., 1., 0., 0., 0.};
 #pragma unroll
     for (int k = 0; k < 7; ++k) tq[k] = S.q[tn * 7 + k];
@@ -1608,7 +1613,7 @@ __global__ void __launch_bounds__(kThreads) k_plain_assemble(DevSys S) {
     for (int k = c0; k < c1; ++k) {
         const int c = S.block_cons[k];
         const int type = S.con_type[c];
-        const bool two = (type == 3 || type == 4);
+        const bool two = con_two_node(type);
         const int rows = S.con_rows[c], lr = S.con_local[c];
         const int lt = S.node_local[S.con_target[c]];
         const int lb = two ? S.node_local[S.con_base[c]] : 0;
@@ -1632,7 +1637,7 @@ __global__ void __launch_bounds__(kThreads) k_plain_assemble(DevSys S) {
         for (int k = c0; k < c1; ++k) {
             const int c = S.block_cons[k];
             const int type = S.con_type[c];
-            if (type == 3 || type == 4) {
+            if (con_two_node(type)) {
                 const int lb = S.node_local[S.con_base[c]];
                 for (int j = 0; j < 6; ++j) rhs[lb + j] += S.con_blres[c * 6 + j];
             }
@@ -1686,7 +1691,7 @@ __global__ void __launch_bounds__(kThreads) k_large_assemble(DevSys S) {
     if (!is_node) {
         const int rows = S.con_rows[con], lr = S.con_local[con];
         const int type = S.con_type[con];
-        const bool two = (type == 3 || type == 4);
+        const bool two = con_two_node(type);
         const int lt = S.node_local[S.con_target[con]];
         const int lb = two ? S.node_local[S.con_base[con]] : 0;
         for (int k = t; k < rows * n; k += blockDim.x) A[(size_t)(lr + k / n) * n + k % n] = 0.;
@@ -1769,7 +1774,7 @@ __global__ void __launch_bounds__(kThreads) k_large_assemble(DevSys S) {
         for (int q = S.node_con_begin[node]; q < S.node_con_begin[node + 1]; ++q) {
             const int c = S.node_con[q];
             const int type = S.con_type[c];
-            if ((type == 3 || type == 4) && S.con_base[c] == node) r += S.con_blres[c * 6 + t];
+            if (con_two_node(type) && S.con_base[c] == node) r += S.con_blres[c * 6 + t];
             if (S.con_target[c] == node) r += S.con_tlres[c * 6 + t];
         }
         S.rhs_raw[off + l + t] = r;

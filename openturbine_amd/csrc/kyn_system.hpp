@@ -20,6 +20,18 @@ constexpr int kThreads = 256;      // workgroup size of every per-element / per-
 constexpr int kMaxBlockDofs = 143; // register LU tile holds n + 1 <= 144 columns ([A | b])
 constexpr int kCondMaxLam = 16;    // condensed solve: constraint rows touching one element
 
+// Constraint types (src/constraints/constraint_type.hpp:9-21)
+constexpr int kConFixedBC = 1, kConPrescribedBC = 2, kConRigidJoint = 3, kConRevoluteJoint = 4,
+              kConRotationControl = 5;
+// constraints between a base and a target node (constraint_type.hpp:28-33)
+constexpr bool con_two_node(int type) {
+    return type == kConRigidJoint || type == kConRevoluteJoint || type == kConRotationControl;
+}
+// constraints whose input(0) is the control signal at UpdateViews (constraints.hpp:346-353)
+constexpr bool con_controlled(int type) { return type == kConRevoluteJoint || type == kConRotationControl; }
+// constraint rows (constraint_type.hpp:48-62): the revolute joint frees one rotation
+constexpr int con_num_rows(int type) { return type == kConRevoluteJoint ? 5 : 6; }
+
 struct StepConsts {
     double h, alpha_m, alpha_f, gamma, beta, gamma_prime, beta_prime, conditioner, atol, rtol;
     int is_dynamic;

@@ -28,6 +28,7 @@ FIXED_BC = 1
 PRESCRIBED_BC = 2
 RIGID_JOINT = 3
 REVOLUTE_JOINT = 4
+ROTATION_CONTROL = 5
 
 
 # --------------------------------------------------------------------------------------
@@ -166,7 +167,7 @@ class ModelSpec:
     num_steps: int = 1
     # step_inputs(step_index, read_state) -> dict with optional keys
     #   "constraint_disp": {constraint_id: [7]}, "nodal_loads": [n, 6] array,
-    #   "element_node_loads": {elem: [N, 6]}
+    #   "element_node_loads": {elem: [N, 6]}, "control": {constraint_id: value}
     step_inputs: Optional[Callable] = None
     meta: dict = field(default_factory=dict)
     masses: list = field(default_factory=list)

@@ -638,7 +638,7 @@ System CreateSystem(const ModelIn& model) {
         c.type = ci.type;
         c.base = ci.base;
         c.target = ci.target;
-        const bool two = (ci.type == CType::RigidJoint || ci.type == CType::RevoluteJoint);
+        const bool two = TwoNode(ci.type);
         if (ci.type == CType::FixedBC || ci.type == CType::PrescribedBC) {
             c.base_sig = kNone;
             c.target_sig = kAll;
@@ -679,6 +679,13 @@ System CreateSystem(const ModelIn& model) {
             ax = {cp[0] * cp[0] * k + dp, cp[1] * cp[0] * k + cp[2], cp[2] * cp[0] * k - cp[1],
                   cp[0] * cp[1] * k - cp[2], cp[1] * cp[1] * k + dp, cp[2] * cp[1] * k + cp[0],
                   cp[0] * cp[2] * k + cp[1], cp[1] * cp[2] * k - cp[0], cp[2] * cp[2] * k + dp};
+        } else if (ci.type == CType::RotationControl) {  // the unit axis (constraints.hpp:324-331)
+            const double na = std::sqrt(ci.axis[0] * ci.axis[0] + ci.axis[1] * ci.axis[1] +
+                                        ci.axis[2] * ci.axis[2]);
+            if (na == 0.) throw std::invalid_argument("Cannot normalize a zero vector");
+            ax[0] = ci.axis[0] / na;
+            ax[1] = ci.axis[1] / na;
+            ax[2] = ci.axis[2] / na;
         } else {
             ax[0] = ci.axis[0];
             ax[1] = ci.axis[1];
@@ -700,7 +707,7 @@ System CreateSystem(const ModelIn& model) {
     }
     for (const auto& c : s.cons) {
         s.sig[c.target] |= c.target_sig;
-        if (c.type == CType::RigidJoint || c.type == CType::RevoluteJoint) s.sig[c.base] |= c.base_sig;
+        if (TwoNode(c.type)) s.sig[c.base] |= c.base_sig;
     }
     s.active_dofs.assign(n, 0);
     s.dof_map.assign(n, 0);
@@ -714,7 +721,7 @@ System CreateSystem(const ModelIn& model) {
     // create_constraint_freedom_table.hpp
     for (auto& c : s.cons) {
         c.target_dof = s.dof_map[c.target];
-        const bool two = (c.type == CType::RigidJoint || c.type == CType::RevoluteJoint);
+        const bool two = TwoNode(c.type);
         c.base_dof = two ? s.dof_map[c.base] : 0;
     }
     s.num_dofs = s.num_system_dofs + start_row;
@@ -733,7 +740,7 @@ System CreateSystem(const ModelIn& model) {
         for (size_t k = 1; k < e.N; ++k) unite(e.node_idx[0], e.node_idx[k]);
     for (const auto& e : s.springs) unite(e.n1, e.n2);
     for (const auto& c : s.cons)
-        if (c.type == CType::RigidJoint || c.type == CType::RevoluteJoint) unite(c.base, c.target);
+        if (TwoNode(c.type)) unite(c.base, c.target);
     std::vector<long> root_to_block(n, -1);
     s.gdof_block.assign(s.num_dofs, 0);
     s.gdof_local.assign(s.num_dofs, 0);
@@ -1074,7 +1081,8 @@ void AssembleSystemMatrix(const StepParameters& p, System& s) {
 // src/constraints/calculate_constraint_residual_gradient.hpp:61-565
 void UpdateConstraintVariables(System& s) {
     for (auto& c : s.cons) {
-        if (c.type == CType::RevoluteJoint) c.input[0] = c.control;  // Constraints::UpdateViews
+        // Constraints::UpdateViews (constraints.hpp:346-353): the control signal is input(0)
+        if (c.type == CType::RevoluteJoint || c.type == CType::RotationControl) c.input[0] = c.control;
         const double* tq = &s.q[c.target * 7];
         const Vec3 u2{tq[0], tq[1], tq[2]};
         const Quat R2{tq[3], tq[4], tq[5], tq[6]};
@@ -1083,7 +1091,7 @@ void UpdateConstraintVariables(System& s) {
         if (c.type == CType::PrescribedBC) {
             u1 = {c.input[0], c.input[1], c.input[2]};
             R1 = {c.input[3], c.input[4], c.input[5], c.input[6]};
-        } else if (c.type == CType::RigidJoint || c.type == CType::RevoluteJoint) {
+        } else if (TwoNode(c.type)) {
             const double* bq = &s.q[c.base * 7];
             u1 = {bq[0], bq[1], bq[2]};
             R1 = {bq[3], bq[4], bq[5], bq[6]};
@@ -1094,8 +1102,17 @@ void UpdateConstraintVariables(System& s) {
         const Vec3 R1X0 = RotateVectorByQuaternion(R1, c.X0);
         for (int k = 0; k < 3; ++k) res[k] = u2[k] + c.X0[k] - u1[k] - R1X0[k];
         if (c.type != CType::RevoluteJoint) {
-            // calculate_{fixed_bc,prescribed_bc,rigid_joint}_constraint.hpp
-            const Quat R2R1t = QuaternionCompose(R2, R1t);
+            // calculate_{fixed_bc,prescribed_bc,rigid_joint}_constraint.hpp; the rotation
+            // control (calculate_rotation_control_constraint.hpp:71-136) inserts the inverse of
+            // the commanded rotation Rc = exp(axis * input(0)) between target and base
+            Quat R2R1t{};
+            if (c.type == CType::RotationControl) {
+                const Vec3 rv{c.axes[0] * c.input[0], c.axes[1] * c.input[0], c.axes[2] * c.input[0]};
+                const Quat RcT = QuaternionInverse(RotationVectorToQuaternion(rv));
+                R2R1t = QuaternionCompose(QuaternionCompose(R2, RcT), R1t);
+            } else {
+                R2R1t = QuaternionCompose(R2, R1t);
+            }
             const Mat3 C = QuaternionToRotationMatrix(R2R1t);
             const Vec3 V3 = AxialVectorOfMatrix(C);
             for (int k = 0; k < 3; ++k) res[3 + k] = V3[k];
@@ -1103,7 +1120,7 @@ void UpdateConstraintVariables(System& s) {
             const Mat3 A = AX_Matrix(C);
             for (int i = 0; i < 3; ++i)
                 for (int j = 0; j < 3; ++j) tg[(3 + i) * 6 + 3 + j] = A[j * 3 + i];
-            if (c.type == CType::RigidJoint) {
+            if (c.type == CType::RigidJoint || c.type == CType::RotationControl) {
                 for (int k = 0; k < 3; ++k) bg[k * 6 + k] = -1.;
                 const Mat3 Xt = VecTilde(R1X0);
                 put3(bg, 0, 3, Xt);
@@ -1156,7 +1173,7 @@ void UpdateConstraintVariables(System& s) {
             }
         };
         finish(tg, &s.tangent[c.target * 36], c.t_grad, c.t_grad_T, c.t_lres);
-        if (c.type == CType::RigidJoint || c.type == CType::RevoluteJoint)
+        if (TwoNode(c.type))
             finish(bg, &s.tangent[c.base * 36], c.b_grad, c.b_grad_T, c.b_lres);
         c.res = res;
     }

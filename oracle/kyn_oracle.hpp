@@ -82,7 +82,18 @@ Mat6 CalculateTangentOperatorT(double h, const double* q_delta6);  // stores T^T
 // ---------------------------------------------------------------------------------------
 // Model inputs (src/model/model.hpp, node.hpp; src/elements/beams/beam_element.hpp)
 // ---------------------------------------------------------------------------------------
-enum class CType : int { None = 0, FixedBC = 1, PrescribedBC = 2, RigidJoint = 3, RevoluteJoint = 4 };
+enum class CType : int {  // src/constraints/constraint_type.hpp:9-21
+    None = 0,
+    FixedBC = 1,
+    PrescribedBC = 2,
+    RigidJoint = 3,
+    RevoluteJoint = 4,
+    RotationControl = 5
+};
+// constraints between a base and a target node (constraint_type.hpp:28-33)
+inline bool TwoNode(CType t) {
+    return t == CType::RigidJoint || t == CType::RevoluteJoint || t == CType::RotationControl;
+}
 
 struct NodeIn {
     std::array<double, 7> x0{0., 0., 0., 1., 0., 0., 0.};
@@ -173,7 +184,7 @@ struct Constraint {
     Vec3 X0{};
     Mat3 axes{};
     std::array<double, 7> input{0., 0., 0., 1., 0., 0., 0.};
-    double control{0.};  // revolute torque / control signal (input(0) at UpdateViews)
+    double control{0.};  // revolute torque / commanded rotation (input(0) at UpdateViews)
     std::array<double, 3> output{};
     std::array<double, 6> lambda{}, res{}, b_lres{}, t_lres{}, sys_res{};
     Mat6 b_grad{}, t_grad{}, b_grad_T{}, t_grad_T{};

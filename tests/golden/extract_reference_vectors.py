@@ -19,11 +19,13 @@ Sources (paths relative to the reference root):
   tests/regression_tests/interfaces/test_blade_interface.cpp,
   tests/regression_tests/interfaces/interfaces_test_files/IEA-15-240-RWT.yaml
                                                              -> blade_interface.json
+  tests/regression_tests/regression/test_rotating_beam.cpp  -> rotating_beam_constraints.json
 
 Usage: python tests/golden/extract_reference_vectors.py [/root/reference]
 """
 
 import json
+import math
 import re
 import sys
 from pathlib import Path
@@ -330,8 +332,90 @@ def blade_interface(root):
     return {"rotating_beam": rotating, "windio": windio}
 
 
+def test_body(src, suite, name):
+    m = re.search(r"TEST\(\s*" + suite + r"\s*,\s*" + name + r"\s*\)\s*\{", src)
+    return balanced(src, m.end() - 1)
+
+
+def rotating_beam_constraints(root):
+    """RotatingBeamTest cases that join the 6-node beam to hub/ground nodes through constraints:
+    MasslessConstraints (rigid joint + prescribed hub), RotationControlConstraint (pitch
+    control, fixed hub), CompoundRotationControlConstraint (pitch control on a prescribed
+    rotating hub), RevoluteJointConstraint (free hub on a revolute joint to a fixed ground
+    node) and GeneratorTorque_Tilt0/_Tilt90 (generator torque on a revolute shaft).  Stored:
+    the file's section matrices, node locations and quadrature, each case's scalar inputs and
+    its expected values with the test's tolerance (expect_kokkos_view_*_equal: 1e-6)."""
+    src = strip_comments((root / "tests/regression_tests/regression/test_rotating_beam.cpp").read_text())
+    head = src[: src.index("TEST(")]
+
+    def arr(name):
+        i = head.index(name)
+        return numbers(balanced(head, head.index("{", i)))
+
+    def scalar(body, pat):
+        m = re.search(pat, body)
+        return numbers(m.group(1))[0] if m else None
+
+    out = {
+        "source": "tests/regression_tests/regression/test_rotating_beam.cpp",
+        "mass_matrix": arr("mass_matrix = std::array"),
+        "stiffness_matrix": arr("stiffness_matrix = std::array"),
+        "node_s": arr("node_s = std::vector"),
+        "quadrature": arr("quadrature = std::vector"),
+        "view_tol": 1e-6,
+    }
+
+    def params(body):
+        return {
+            "omega": scalar(body, r"omega = ([^;]+);"),
+            "hub_size": scalar(body, r"hub_size = ([^;]+);"),
+            "max_iter": scalar(body, r"max_iter\((\d+)\)"),
+            "step_size": scalar(body, r"step_size\(([^)]+)\)"),
+            "rho_inf": scalar(body, r"rho_inf\(([^)]+)\)"),
+            "num_steps": scalar(body, r"iota\(0, (\d+)\)"),
+        }
+
+    for name in ("MasslessConstraints", "RotationControlConstraint", "RevoluteJointConstraint"):
+        body = test_body(src, "RotatingBeamTest", name)
+        e = params(body)
+        e["expect_q"] = expectations(body)[0]["values"]
+        out[name] = e
+    body = test_body(src, "RotatingBeamTest", "CompoundRotationControlConstraint")
+    e = params(body)
+    e["expect_rv"] = near_values(body, r"^rv\[")
+    out["CompoundRotationControlConstraint"] = e
+    helper = function_body_args(src, "GeneratorTorqueWithAxisTilt")
+    gp = params(helper)
+    gp["torque"] = scalar(helper, r"auto torque = ([^;]+);")
+    out["GeneratorTorque"] = gp
+    for name, tilt in (("GeneratorTorque_Tilt0", 0.0), ("GeneratorTorque_Tilt90", math.pi / 2.0)):
+        body = test_body(src, "RotatingBeamTest", name)
+        groups = [numbers(g) for g in re.findall(r"\{([^{}]*)\}", body[body.index("(") :])]
+        out[name] = {"tilt": tilt, "expect_azimuth_q": groups[0], "expect_azimuth_v": groups[1],
+                     "expect_revolute_output": groups[2]}
+    return out
+
+
+def function_body_args(text, name):
+    """Body of a free function with parameters: `void name(...) {...}`."""
+    m = re.search(r"\bvoid\s+" + re.escape(name) + r"\s*\(", text)
+    i = m.end() - 1
+    balanced(text, i)
+    depth, j = 0, i
+    for j in range(i, len(text)):
+        if text[j] == "(":
+            depth += 1
+        elif text[j] == ")":
+            depth -= 1
+            if depth == 0:
+                break
+    k = text.index("{", j)
+    return balanced(text, k)
+
+
 def main():
     root = Path(sys.argv[1] if len(sys.argv) > 1 else "/root/reference")
+    (HERE / "rotating_beam_constraints.json").write_text(json.dumps(rotating_beam_constraints(root), indent=1))
     (HERE / "blade_interface.json").write_text(json.dumps(blade_interface(root), indent=1))
     (HERE / "solver_regression.json").write_text(json.dumps(solver_regression(root), indent=1))
     (HERE / "cantilever_sine.json").write_text(json.dumps(cantilever(root), indent=1))

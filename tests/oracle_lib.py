@@ -165,6 +165,8 @@ class OracleSystem:
         for e, fx in inputs.get("element_node_loads", {}).items():
             a, p = _d(fx)
             self.lib.L.orc_set_element_node_loads(self.h, e, p)
+        for cid, val in inputs.get("control", {}).items():
+            self.lib.L.orc_set_control(self.h, cid, float(val))
 
     def step(self, inputs=None):
         if inputs is None and self.spec.step_inputs is not None:

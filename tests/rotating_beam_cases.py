@@ -1,0 +1,196 @@
+"""Models of the reference's RotatingBeamTest constraint cases
+(tests/regression_tests/regression/test_rotating_beam.cpp), built from the numbers in
+tests/golden/rotating_beam_constraints.json.
+
+Every case uses the file's 6-node beam (x = 10 s, translated 2 m along x) with the two-station
+uniform section and 7-point Gauss quadrature; node and constraint ids follow the order in which
+the test adds them:
+
+  MasslessConstraints        nodes 0-5 beam, 6 hub.  c0 rigid joint hub->beam root, c1
+                             prescribed hub rotation about z; beam spinning at omega.
+  RotationControlConstraint  nodes 0-5 beam, 6 hub.  c0 rotation control hub->root about x
+                             (pitch = t pi/2), c1 fixed hub.
+  CompoundRotationControl... as above with c1 a prescribed hub rotation about z
+                             (azimuth = t pi/4); 100 steps.
+  RevoluteJointConstraint    nodes 0-5 beam, 6 hub, 7 ground at z = -1.  c0 fixed ground, c1
+                             revolute ground->hub (axis from X0, zero torque), c2 rigid joint
+                             hub->root; beam spinning at omega.
+  GeneratorTorque_Tilt*      nodes 0-5 beam (rotated by the tilt about x), 6 shaft base, 7
+                             azimuth, 8 hub.  c0 fixed shaft base, c1 revolute shaft->azimuth
+                             about the tilted axis with a 100 N m torque, c2 rigid azimuth->hub,
+                             c3 rigid hub->root; rho_inf 0.
+"""
+
+import math
+
+import numpy as np
+
+from openturbine_amd import workloads as W
+
+CASES = ("MasslessConstraints", "RotationControlConstraint", "CompoundRotationControlConstraint",
+         "RevoluteJointConstraint", "GeneratorTorque_Tilt0", "GeneratorTorque_Tilt90")
+
+
+def _beam(fx, nt, tilt_q=None, omega=None):
+    hub = fx["MasslessConstraints"]["hub_size"]
+    ids = []
+    for s in fx["node_s"]:
+        x0 = np.array([10.0 * s + hub, 0.0, 0.0])
+        r = np.array([1.0, 0.0, 0.0, 0.0])
+        if tilt_q is not None:  # Node::RotateAboutPoint about the origin (node.hpp:100-130)
+            x0 = W.quat_rotate(tilt_q, x0)
+            r = W.quat_compose(tilt_q, r)
+        v = np.zeros(6)
+        if omega is not None:  # Node::SetVelocityAboutPoint about the origin (node.hpp:203-225)
+            v[:3] = np.cross([0.0, 0.0, omega], x0)
+            v[5] = omega
+        ids.append(nt.add(np.concatenate([x0, r]), v=v, s=s))
+    beam = W.BeamSpec(np.array(ids), np.array([0.0, 1.0]), np.array([fx["mass_matrix"]] * 2),
+                      np.array([fx["stiffness_matrix"]] * 2), np.array(fx["quadrature"]).reshape(-1, 2))
+    return ids, beam
+
+
+def case_spec(fx, name):
+    """ModelSpec of one case; step_inputs gives the per-step prescribed hub motion and controls."""
+    C = W.ConstraintSpec
+    nt = W._NodeTable()
+    origin = [0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0]
+    if name.startswith("GeneratorTorque"):
+        p = fx["GeneratorTorque"]
+        tilt = fx[name]["tilt"]
+        ids, beam = _beam(fx, nt, tilt_q=W.quat_from_rotvec([tilt, 0.0, 0.0]))
+        shaft = nt.add(origin)
+        azimuth = nt.add(origin)
+        hub = nt.add([0.0, math.sin(tilt), math.cos(tilt), 1.0, 0.0, 0.0, 0.0])
+        cons = [C(W.FIXED_BC, 0, shaft),
+                C(W.REVOLUTE_JOINT, shaft, azimuth, axis=np.array([0.0, math.sin(tilt), math.cos(tilt)])),
+                C(W.RIGID_JOINT, azimuth, hub), C(W.RIGID_JOINT, hub, ids[0])]
+        torque = p["torque"]
+
+        def step_inputs(step, _state=None):
+            return {"control": {1: torque}}
+        meta = {"azimuth_node": azimuth, "revolute": 1}
+    else:
+        p = fx[name]
+        h = p["step_size"]
+        spin = name in ("MasslessConstraints", "RevoluteJointConstraint")
+        ids, beam = _beam(fx, nt, omega=p["omega"] if spin else None)
+        hub = nt.add(origin)
+        meta = {}
+        if name == "MasslessConstraints":
+            cons = [C(W.RIGID_JOINT, hub, ids[0]), C(W.PRESCRIBED_BC, 0, hub)]
+            omega = p["omega"]
+
+            def step_inputs(step, _state=None):
+                q = W.quat_from_rotvec([0.0, 0.0, omega * h * (step + 1)])
+                return {"constraint_disp": {1: np.concatenate([np.zeros(3), q])}}
+        elif name == "RotationControlConstraint":
+            cons = [C(W.ROTATION_CONTROL, hub, ids[0], axis=np.array([1.0, 0.0, 0.0])), C(W.FIXED_BC, 0, hub)]
+
+            def step_inputs(step, _state=None):
+                return {"control": {0: h * (step + 1) * math.pi / 2.0}}
+        elif name == "CompoundRotationControlConstraint":
+            cons = [C(W.ROTATION_CONTROL, hub, ids[0], axis=np.array([1.0, 0.0, 0.0])), C(W.PRESCRIBED_BC, 0, hub)]
+
+            def step_inputs(step, _state=None):
+                t = h * (step + 1)
+                q = W.quat_from_rotvec([0.0, 0.0, 0.5 * t * math.pi / 2.0])
+                return {"control": {0: t * math.pi / 2.0},
+                        "constraint_disp": {1: np.concatenate([np.zeros(3), q])}}
+        elif name == "RevoluteJointConstraint":
+            ground = nt.add([0.0, 0.0, -1.0, 1.0, 0.0, 0.0, 0.0])
+            cons = [C(W.FIXED_BC, 0, ground), C(W.REVOLUTE_JOINT, ground, hub), C(W.RIGID_JOINT, hub, ids[0])]
+
+            def step_inputs(step, _state=None):
+                return {"control": {1: 0.0}}
+        else:
+            raise KeyError(name)
+    x0, u, v, vd, s = nt.arrays()
+    return W.ModelSpec(name, x0, u, v, vd, s, [beam], cons, np.zeros(3), max_iter=int(p["max_iter"]),
+                       h=p["step_size"], rho_inf=p["rho_inf"], atol=1e-5, rtol=1e-3,
+                       num_steps=int(p["num_steps"]), step_inputs=step_inputs, meta=meta)
+
+
+def rotation_vector(q):
+    """QuaternionToRotationVector (src/math/quaternion_operations.hpp:247-265)."""
+    theta = 2.0 * math.acos(q[0])
+    shh = math.sqrt(1.0 - q[0] * q[0])
+    if shh > 1e-12:
+        return np.array([theta * q[1] / shh, theta * q[2] / shh, theta * q[3] / shh])
+    return np.zeros(3)
+
+
+def check_case(fx, name, get, constraint_output):
+    """Assert one case's expected values; get(field) returns the [n, w] state view.  Returns the
+    worst absolute deviation."""
+    tol = fx["view_tol"]
+    worst = 0.0
+    if name in ("MasslessConstraints", "RotationControlConstraint", "RevoluteJointConstraint"):
+        exp = np.array(fx[name]["expect_q"]).reshape(-1, 7)
+        got = get("q")
+        worst = float(np.max(np.abs(got - exp)))
+        assert worst <= tol, (name, worst)
+    elif name == "CompoundRotationControlConstraint":
+        rv = rotation_vector(get("q")[0, 3:])
+        for k, e in enumerate(fx[name]["expect_rv"]):
+            d = abs(rv[k] - e["value"])
+            worst = max(worst, d)
+            assert d <= e["tol"], (name, k, rv[k], e)
+    else:
+        e = fx[name]
+        az = 7
+        for got, exp in ((get("q")[az], e["expect_azimuth_q"]), (get("v")[az], e["expect_azimuth_v"]),
+                         (constraint_output(1)[:3], e["expect_revolute_output"])):
+            d = float(np.max(np.abs(np.asarray(got) - np.asarray(exp))))
+            worst = max(worst, d)
+            assert d <= tol, (name, got, exp)
+    return worst
+
+
+def check_invariants(fx, name, get, tol=1e-12):
+    """Kinematic relations the converged constraints fix exactly (up to the Newton tolerance
+    times the constraint scale): the rigid joint carries the root with the hub, the rotation
+    control sets root = hub * exp(pitch x), the revolute joint keeps the hub's rotation about the
+    joint axis.  Returns the worst violation."""
+    q = get("q")
+    x0 = get("x0")
+    spec = case_spec(fx, name)
+    h = spec.h
+    t = h * spec.num_steps
+    root = 0
+    worst = 0.0
+
+    def qmat(a):
+        return np.array([W.quat_rotate(a, e) for e in np.eye(3)]).T
+
+    if name == "MasslessConstraints":
+        hub = 6
+        X0 = x0[root, :3] - x0[hub, :3]
+        want = q[hub, :3] + W.quat_rotate(q[hub, 3:], X0) - X0
+        worst = max(worst, float(np.max(np.abs(q[root, :3] - want))))
+        worst = max(worst, float(np.max(np.abs(qmat(q[root, 3:]) - qmat(q[hub, 3:])))))
+    elif name in ("RotationControlConstraint", "CompoundRotationControlConstraint"):
+        hub = 6
+        pitch = t * math.pi / 2.0
+        want_r = W.quat_compose(q[hub, 3:], W.quat_from_rotvec([pitch, 0.0, 0.0]))
+        worst = max(worst, float(np.max(np.abs(qmat(q[root, 3:]) - qmat(want_r)))))
+        X0 = x0[root, :3] - x0[hub, :3]
+        want = q[hub, :3] + W.quat_rotate(q[hub, 3:], X0) - X0
+        worst = max(worst, float(np.max(np.abs(q[root, :3] - want))))
+        if name == "CompoundRotationControlConstraint":
+            az = 0.5 * t * math.pi / 2.0
+            want_r = W.quat_compose(W.quat_from_rotvec([0.0, 0.0, az]), W.quat_from_rotvec([pitch, 0.0, 0.0]))
+            worst = max(worst, float(np.max(np.abs(qmat(q[root, 3:]) - qmat(want_r)))))
+    elif name == "RevoluteJointConstraint":
+        hub = 6
+        R = qmat(q[hub, 3:])
+        axis = np.array([0.0, 0.0, 1.0])  # unit X0 from ground (z = -1) to hub
+        worst = max(worst, float(np.max(np.abs(R @ axis - axis))), float(np.max(np.abs(q[hub, :3]))))
+    else:
+        azimuth = 7
+        tilt = fx[name]["tilt"]
+        axis = np.array([0.0, math.sin(tilt), math.cos(tilt)])
+        R = qmat(q[azimuth, 3:])
+        worst = max(worst, float(np.max(np.abs(R @ axis - axis))), float(np.max(np.abs(q[azimuth, :3]))))
+    assert worst <= tol, (name, worst)
+    return worst
