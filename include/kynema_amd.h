@@ -42,6 +42,9 @@ extern "C" {
 #define KYN_RIGID_JOINT 3
 #define KYN_REVOLUTE_JOINT 4
 #define KYN_ROTATION_CONTROL 5 /* base rotation composed with a commanded rotation about `axis` */
+#define KYN_FIXED_BC_3DOFS 6   /* translations of a three-dof node held */
+#define KYN_PRESCRIBED_BC_3DOFS 7
+#define KYN_RIGID_JOINT_6DOFS_TO_3DOFS 8 /* a six-dof base carries a three-dof target */
 
 /* ---- execution form ---- */
 #define KYN_EXEC_PLAIN 0 /* one kernel per stage, host reads the error every iteration */
@@ -103,14 +106,16 @@ int64_t kyn_model_add_beam_element(kyn_model m, size_t num_nodes, const size_t* 
  * that hold masses or springs with KYN_ERR_UNSUPPORTED). */
 int64_t kyn_model_add_mass_element(kyn_model m, size_t node, const double mass_matrix[36]);
 /* Model::AddSpringElement (model.hpp:376-388): linear spring between two nodes' translational
- * dofs.  Returns the spring element id.  Plain form only; each spring node must also carry a
- * beam, a mass or a constraint (six dofs), else kyn_system_create returns KYN_ERR_UNSUPPORTED. */
+ * dofs.  Returns the spring element id.  Plain form only; a node held by springs alone has
+ * three dofs (FreedomSignature::JustPosition), as in the reference. */
 int64_t kyn_model_add_spring_element(kyn_model m, size_t node1, size_t node2, double stiffness,
                                      double undeformed_length);
 /* Model::AddFixedBC / AddPrescribedBC / AddRigidJointConstraint / AddRevoluteJointConstraint /
- * AddRotationControl (model.hpp:477-560). For one-node constraints `base` is ignored. axis and
- * init may be NULL.  A rotation control constraint takes its commanded angle (radians about the
- * unit `axis`) from kyn_set_constraint_control. */
+ * AddRotationControl / AddFixedBC3DOFs / AddPrescribedBC3DOFs / AddRigidJoint6DOFsTo3DOFs
+ * (model.hpp:477-608). For one-node constraints `base` is ignored. axis and init may be NULL.
+ * A rotation control constraint takes its commanded angle (radians about the unit `axis`) from
+ * kyn_set_constraint_control.  A node whose only dofs come from springs or from the three-dof
+ * constraints has the reference's three translational dofs. */
 int64_t kyn_model_add_constraint(kyn_model m, int type, size_t base, size_t target,
                                  const double axis[3], const double init_disp[7]);
 

@@ -458,8 +458,10 @@ KYN_DEV void ax_matrix(const double* A, double* R) {
 }
 
 // Constraint residual and gradients for FixedBC, PrescribedBC, RigidJoint, RevoluteJoint,
-// RotationControl (src/constraints/calculate_constraint_residual_gradient.hpp:61-565 and the
-// kernels calculate_{fixed_bc,prescribed_bc,rigid_joint,revolute_joint,rotation_control}_constraint.hpp,
+// RotationControl and the three-dof variants (src/constraints/
+// calculate_constraint_residual_gradient.hpp:61-565 and the kernels
+// calculate_{fixed_bc,prescribed_bc,rigid_joint,revolute_joint,rotation_control}_constraint.hpp,
+// calculate_{fixed_bc,prescribed_bc,rigid_joint}_3DOF_constraint.hpp,
 // calculate_revolute_joint_force.hpp).  Outputs: res[6]; tg, bg: gradient (not yet times
 // tangent); sys_res[3..5] (revolute only).
 KYN_DEV void constraint_eval(int type, const double* X0, const double* axes, const double* input,
@@ -473,7 +475,7 @@ KYN_DEV void constraint_eval(int type, const double* X0, const double* axes, con
 #pragma unroll
     for (int i = 0; i < 6; ++i) res[i] = 0.;
     double u1[3] = {0., 0., 0.}, R1[4] = {1., 0., 0., 0.};
-    if (type == 2) {
+    if (con_prescribed(type)) {
         u1[0] = input[0]; u1[1] = input[1]; u1[2] = input[2];
         R1[0] = input[3]; R1[1] = input[4]; R1[2] = input[5]; R1[3] = input[6];
     } else if (con_two_node(type)) {
@@ -487,7 +489,24 @@ KYN_DEV void constraint_eval(int type, const double* X0, const double* axes, con
     quat_rotate(R1, X0, R1X0);
 #pragma unroll
     for (int k = 0; k < 3; ++k) res[k] = u2[k] + X0[k] - u1[k] - R1X0[k];
-    if (type != kConRevoluteJoint) {
+    if (con_three_dof(type)) {
+        // calculate_{fixed_bc,prescribed_bc,rigid_joint}_3DOF_constraint.hpp: translational rows
+        // only; target gradient I, base gradient [-I tilde(R1 X0)]
+        tg[0] = 1.;
+        tg[7] = 1.;
+        tg[14] = 1.;
+        if (type == kConRigidJoint6DOFsTo3DOFs) {
+            bg[0] = -1.;
+            bg[7] = -1.;
+            bg[14] = -1.;
+            double Xt[9];
+            tilde(R1X0, Xt);
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j) bg[i * 6 + 3 + j] = Xt[i * 3 + j];
+        }
+    } else if (type != kConRevoluteJoint) {
         double R2R1t[4], C[9];
         if (type == kConRotationControl) {
             // calculate_rotation_control_constraint.hpp:85-100: C = R(R2 inv(Rc) inv(R1)) with

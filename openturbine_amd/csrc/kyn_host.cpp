@@ -60,8 +60,11 @@ void set_last_error(const std::string& msg) { g_err = msg; }
 namespace {
 
 using kyn::con_num_rows;
+using kyn::con_prescribed;
+using kyn::con_three_dof;
 using kyn::con_two_node;
 using kyn::kConFixedBC;
+using kyn::kConRigidJoint6DOFsTo3DOFs;
 using kyn::kConRotationControl;
 
 struct KynError : std::runtime_error {
@@ -273,7 +276,7 @@ struct Builder {
         block_con_begin, block_con_end, block_cons, block_node_begin, block_node_end, block_nodes,
         block_vec_off;
     std::vector<int64_t> block_A_off;
-    std::vector<int> node_block, node_local;
+    std::vector<int> node_block, node_local, node_pin, block_npin;
     std::vector<int> elem_N, elem_Q, elem_node_off, elem_nodes, elem_qp_off, elem_shape_off;
     std::vector<int64_t> elem_mat_off;
     std::vector<double> qp_weight, qp_jac, qp_Mstar, qp_Cstar, qp_x0p, qp_r0, shape_interp, shape_deriv;
@@ -316,6 +319,7 @@ void add_model(Builder& B, const kyn_model_s& mo) {
     for (size_t i = 0; i < n; ++i) {
         B.node_block.push_back(-1);
         B.node_local.push_back(-1);
+        B.node_pin.push_back(0);
     }
     // ---- validate
     for (const auto& bm : mo.beams) {
@@ -328,7 +332,7 @@ void add_model(Builder& B, const kyn_model_s& mo) {
         if (bm.sections.empty()) throw KynError(KYN_ERR_INVALID, "beam element without sections");
     }
     for (const auto& c : mo.constraints) {
-        if (c.type < kConFixedBC || c.type > kConRotationControl) throw KynError(KYN_ERR_UNSUPPORTED, "unsupported constraint type");
+        if (c.type < kConFixedBC || c.type > kConRigidJoint6DOFsTo3DOFs) throw KynError(KYN_ERR_UNSUPPORTED, "unsupported constraint type");
         if (c.target >= n || (con_two_node(c.type) && c.base >= n))
             throw KynError(KYN_ERR_INVALID, "constraint references an unknown node");
     }
@@ -341,26 +345,30 @@ void add_model(Builder& B, const kyn_model_s& mo) {
     }
 
     // ---- DOF tables (assemble_node_freedom_allocation_table.hpp, compute_node_freedom_map_table.hpp)
-    std::vector<int> has(n, 0);
+    // freedom signature bits: 1 translations, 2 rotations.  Springs and the three-dof constraint
+    // targets give a node its translations only (FreedomSignature::JustPosition); the block keeps
+    // six rows per node and holds a translations-only node's rotational rows at zero
+    constexpr int kPos = 1, kAll = 3;
+    std::vector<int> sig(n, 0);
     for (const auto& bm : mo.beams)
-        for (auto nid : bm.nodes) has[nid] = 1;
-    for (const auto& ms : mo.masses) has[ms.node] = 1;
-    for (const auto& c : mo.constraints) {
-        has[c.target] = 1;
-        if (con_two_node(c.type)) has[c.base] = 1;
+        for (auto nid : bm.nodes) sig[nid] |= kAll;
+    for (const auto& ms : mo.masses) sig[ms.node] |= kAll;
+    for (const auto& sp : mo.springs) {
+        sig[sp.n1] |= kPos;
+        sig[sp.n2] |= kPos;
     }
-    // a spring alone gives its nodes the three translational dofs (FreedomSignature::JustPosition);
-    // the device state and block layout carry six dofs per node, so spring nodes must also belong
-    // to a beam, a mass or a constraint
-    for (const auto& sp : mo.springs)
-        if (!has[sp.n1] || !has[sp.n2])
-            throw KynError(KYN_ERR_UNSUPPORTED,
-                           "spring nodes need all six dofs (a beam, mass or constraint on each spring node)");
+    for (const auto& c : mo.constraints) {
+        sig[c.target] |= con_three_dof(c.type) ? kPos : kAll;
+        if (con_two_node(c.type)) sig[c.base] |= kAll;
+    }
+    std::vector<int> has(n, 0);
+    for (size_t i = 0; i < n; ++i) has[i] = sig[i] != 0;
     std::vector<int> dof_map(n, 0);
-    int nsys = 0;
+    int nsys_ref = 0;  // the reference's system dofs (three for a translations-only node)
     for (size_t i = 0; i < n; ++i) {
-        dof_map[i] = nsys;
-        nsys += has[i] ? 6 : 0;
+        dof_map[i] = nsys_ref;
+        nsys_ref += sig[i] == kAll ? 6 : sig[i] == kPos ? 3 : 0;
+        if (sig[i] == kPos) B.node_pin[node0 + i] = 1;
     }
     std::vector<int> row_first(mo.constraints.size());
     int nrows = 0;
@@ -368,7 +376,7 @@ void add_model(Builder& B, const kyn_model_s& mo) {
         row_first[k] = nrows;
         nrows += con_num_rows(mo.constraints[k].type);
     }
-    const int ndofs = nsys + nrows;
+    const int ndofs = nsys_ref + nrows;  // the reference's count (convergence-error normalization)
 
     // ---- blocks (connected components)
     std::vector<size_t> parent(n);
@@ -413,13 +421,17 @@ void add_model(Builder& B, const kyn_model_s& mo) {
         const int gb = blk0 + static_cast<int>(bi);
         int l = 0;
         B.block_node_begin.push_back(static_cast<int>(B.block_nodes.size()));
+        int npin = 0;
         for (int nd : bnodes[bi]) {
             B.block_nodes.push_back(node0 + nd);
             B.node_block[node0 + nd] = gb;
             B.node_local[node0 + nd] = l;
-            for (int c = 0; c < 6; ++c) gmap[dof_map[nd] + c] = B.vec_total + l + c;
+            const int active = B.node_pin[node0 + nd] ? 3 : 6;
+            for (int c = 0; c < active; ++c) gmap[dof_map[nd] + c] = B.vec_total + l + c;
+            npin += B.node_pin[node0 + nd];
             l += 6;
         }
+        B.block_npin.push_back(npin);
         B.block_node_end.push_back(static_cast<int>(B.block_nodes.size()));
         const int bnsys = l;
         B.block_con_begin.push_back(static_cast<int>(B.block_cons.size()));
@@ -428,7 +440,7 @@ void add_model(Builder& B, const kyn_model_s& mo) {
             con_loc[ck] = l;
             con_blk[ck] = gb;
             const int rows = con_num_rows(mo.constraints[ck].type);
-            for (int r = 0; r < rows; ++r) gmap[nsys + row_first[ck] + r] = B.vec_total + l + r;
+            for (int r = 0; r < rows; ++r) gmap[nsys_ref + row_first[ck] + r] = B.vec_total + l + r;
             l += rows;
         }
         B.block_con_end.push_back(static_cast<int>(B.block_cons.size()));
@@ -450,7 +462,7 @@ void add_model(Builder& B, const kyn_model_s& mo) {
         B.A_total += static_cast<int64_t>(l) * (l + 1);
     }
     B.gdof_maps.push_back(gmap);
-    B.model_nsys.push_back(nsys);
+    B.model_nsys.push_back(nsys_ref);
 
     // ---- elements (create_beams.hpp:26-165)
     for (const auto& bm : mo.beams) {
@@ -540,7 +552,7 @@ void add_model(Builder& B, const kyn_model_s& mo) {
         B.con_local.push_back(con_loc[k]);
         B.con_block.push_back(con_blk[k]);
         std::array<double, 3> X0{0., 0., 0.};
-        if (c.type != 2) {
+        if (!con_prescribed(c.type)) {
             const auto& t = mo.nodes[c.target].x0;
             if (!two) X0 = {t[0] - c.axis[0], t[1] - c.axis[1], t[2] - c.axis[2]};
             else {
@@ -578,7 +590,7 @@ void add_model(Builder& B, const kyn_model_s& mo) {
         }
         for (double v : ax) B.con_axes.push_back(v);
         std::array<double, 7> in{0., 0., 0., 1., 0., 0., 0.};
-        if (c.type == 2) in = c.init;
+        if (con_prescribed(c.type)) in = c.init;
         for (double v : in) B.con_input.push_back(v);
     }
 
@@ -1216,6 +1228,8 @@ kyn_system kyn_system_create(const kyn_model* models, size_t num_models, const k
         S.block_vec_off = Ar.up(B.block_vec_off);
         S.node_block = Ar.up(B.node_block);
         S.node_local = Ar.up(B.node_local);
+        S.node_pin = Ar.up(B.node_pin);
+        S.block_npin = Ar.up(B.block_npin);
         S.elem_N = Ar.up(B.elem_N);
         S.elem_Q = Ar.up(B.elem_Q);
         S.elem_node_off = Ar.up(B.elem_node_off);

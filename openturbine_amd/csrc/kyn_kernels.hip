@@ -1092,6 +1092,25 @@ __global__ void k_advance_slot(DevSys S) {
 // ---------------------------------------------------------------------------------------
 // Block phase helpers (fused form): constraints into the LDS KKT block
 // ---------------------------------------------------------------------------------------
+// Translations-only nodes (FreedomSignature::JustPosition: springs or three-dof constraints,
+// three dofs in the reference): the block keeps six rows per node, and a pinned node's three
+// rotational rows and columns hold nothing but a unit diagonal and a zero rhs, so their
+// increments are exactly zero and the elimination of every other row is unchanged.
+template <typename TT>
+__device__ __forceinline__ void pin_rotations(const DevSys& S, int b, TT* A, int lda, double* rhs) {
+    if (S.block_npin[b] == 0) return;  // uniform over the workgroup
+    for (int k = S.block_node_begin[b] + threadIdx.x; k < S.block_node_end[b]; k += blockDim.x) {
+        const int node = S.block_nodes[k];
+        if (!S.node_pin[node]) continue;
+        const int l = S.node_local[node];
+        for (int c = 3; c < 6; ++c) {
+            A[(size_t)(l + c) * lda + l + c] = TT(1);
+            rhs[l + c] = 0.;
+        }
+    }
+    __syncthreads();
+}
+
 template <typename TT>
 __device__ __forceinline__ void block_constraints(const DevSys& S, int b, char* sm, const Lay& L) {
     TT* A = reinterpret_cast<TT*>(sm + L.A);
@@ -1197,6 +1216,7 @@ __device__ __forceinline__ void block_constraints(const DevSys& S, int b, char* 
         }
     }
     __syncthreads();
+    pin_rotations<TT>(S, b, A, L.lda, reinterpret_cast<double*>(sm + L.rhs));
 }
 
 // Convergence-error terms and state update of one block after its solve
@@ -1651,6 +1671,7 @@ __global__ void __launch_bounds__(kThreads) k_plain_assemble(DevSys S) {
         }
     }
     __syncthreads();
+    pin_rotations(S, b, A, lda, rhs);
     condition_rhs(S, b, rhs);
 }
 
@@ -1754,6 +1775,8 @@ __global__ void __launch_bounds__(kThreads) k_large_assemble(DevSys S) {
         }
         __syncthreads();
     }
+    const bool pin = S.node_pin[node] != 0;  // translations only: rotational rows held at zero
+    if (pin && t >= 3 && t < 6) A[(size_t)(l + t) * n + l + t] = 1.;
     // residual entries of this node
     if (t < 6) {
         double r = 0.;
@@ -1777,6 +1800,7 @@ __global__ void __launch_bounds__(kThreads) k_large_assemble(DevSys S) {
             if (con_two_node(type) && S.con_base[c] == node) r += S.con_blres[c * 6 + t];
             if (S.con_target[c] == node) r += S.con_tlres[c * 6 + t];
         }
+        if (pin && t >= 3) r = 0.;
         S.rhs_raw[off + l + t] = r;
         S.rhs[off + l + t] = -1. * (l + t < nsys ? r * cond : r);
     }

@@ -22,15 +22,23 @@ constexpr int kCondMaxLam = 16;    // condensed solve: constraint rows touching 
 
 // Constraint types (src/constraints/constraint_type.hpp:9-21)
 constexpr int kConFixedBC = 1, kConPrescribedBC = 2, kConRigidJoint = 3, kConRevoluteJoint = 4,
-              kConRotationControl = 5;
+              kConRotationControl = 5, kConFixedBC3DOFs = 6, kConPrescribedBC3DOFs = 7,
+              kConRigidJoint6DOFsTo3DOFs = 8;
 // constraints between a base and a target node (constraint_type.hpp:28-33)
 constexpr bool con_two_node(int type) {
-    return type == kConRigidJoint || type == kConRevoluteJoint || type == kConRotationControl;
+    return type == kConRigidJoint || type == kConRevoluteJoint || type == kConRotationControl ||
+           type == kConRigidJoint6DOFsTo3DOFs;
 }
+// constraints on a target with three translational dofs (constraint_type.hpp:64-70)
+constexpr bool con_three_dof(int type) {
+    return type == kConFixedBC3DOFs || type == kConPrescribedBC3DOFs || type == kConRigidJoint6DOFsTo3DOFs;
+}
+constexpr bool con_prescribed(int type) { return type == kConPrescribedBC || type == kConPrescribedBC3DOFs; }
 // constraints whose input(0) is the control signal at UpdateViews (constraints.hpp:346-353)
 constexpr bool con_controlled(int type) { return type == kConRevoluteJoint || type == kConRotationControl; }
-// constraint rows (constraint_type.hpp:48-62): the revolute joint frees one rotation
-constexpr int con_num_rows(int type) { return type == kConRevoluteJoint ? 5 : 6; }
+// constraint rows (constraint_type.hpp:60-79): the revolute joint frees one rotation, the
+// three-dof variants hold the translations only
+constexpr int con_num_rows(int type) { return con_three_dof(type) ? 3 : type == kConRevoluteJoint ? 5 : 6; }
 
 struct StepConsts {
     double h, alpha_m, alpha_f, gamma, beta, gamma_prime, beta_prime, conditioner, atol, rtol;
@@ -74,6 +82,10 @@ struct DevSys {
     // ---- node -> block-local dof
     const int* node_block;
     const int* node_local;        // -1 if the node has no dofs
+    // 1 for a node with the three translational dofs only (FreedomSignature::JustPosition):
+    // its rotational rows in the block are held at x = 0 by a unit diagonal and a zero rhs
+    const int* node_pin;
+    const int* block_npin;        // pinned nodes in the block (0: nothing to do)
 
     // ---- elements (src/elements/beams/beams.hpp)
     const int *elem_N, *elem_Q;

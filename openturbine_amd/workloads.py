@@ -29,6 +29,9 @@ PRESCRIBED_BC = 2
 RIGID_JOINT = 3
 REVOLUTE_JOINT = 4
 ROTATION_CONTROL = 5
+FIXED_BC_3DOFS = 6
+PRESCRIBED_BC_3DOFS = 7
+RIGID_JOINT_6DOFS_TO_3DOFS = 8
 
 
 # --------------------------------------------------------------------------------------
@@ -244,9 +247,11 @@ def precession_spec(fx: dict) -> ModelSpec:
                      masses=[MassSpec(0, np.array(fx["mass_matrix"]))])
 
 
-def spring_mass_spec(fx: dict) -> ModelSpec:
+def spring_mass_spec(fx: dict, anchor_type: int = FIXED_BC) -> ModelSpec:
     """tests/regression_tests/regression/test_spring_mass_system.cpp (SpringMassSystemTest): a
-    point mass on a spring to a clamped node, released 2 m from equilibrium; h = T / steps."""
+    point mass on a spring to a clamped node, released 2 m from equilibrium; h = T / steps.
+    anchor_type FIXED_BC_3DOFS holds the anchor, which only the spring touches, as a three-dof
+    node (the same motion of the mass)."""
     nt = _NodeTable()
     for p in fx["positions"]:
         nt.add(p)
@@ -254,10 +259,46 @@ def spring_mass_spec(fx: dict) -> ModelSpec:
     m, j, k = fx["m"], fx["j"], fx["k"]
     period = 2.0 * np.pi * np.sqrt(m / k)
     M = _diag6([m, m, m, j, j, j])
-    return ModelSpec("spring_mass", x0, u, v, vd, s, [], [ConstraintSpec(FIXED_BC, 0, 0)], np.zeros(3),
+    return ModelSpec("spring_mass", x0, u, v, vd, s, [], [ConstraintSpec(anchor_type, 0, 0)], np.zeros(3),
                      max_iter=int(fx["max_iter"]), h=period / fx["num_steps"], rho_inf=fx["rho_inf"],
                      num_steps=int(fx["num_steps"]), masses=[MassSpec(1, M)],
                      springs=[SpringSpec(0, 1, k, fx["l0"])])
+
+
+def spring_mass_chain_spec(fx: dict) -> ModelSpec:
+    """tests/regression_tests/regression/test_spring_mass_system.cpp (SpringMassChainSystemTest):
+    equidistant point masses joined by zero-length springs between two clamped anchors, in
+    static equilibrium for one period of a single mass on one spring."""
+    nt = _NodeTable()
+    nm = int(fx["number_of_masses"])
+    for i in range(nm + 2):
+        nt.add([i * fx["spacing"], 0, 0, 1, 0, 0, 0])
+    x0, u, v, vd, s = nt.arrays()
+    m, j, k = fx["m"], fx["j"], fx["k"]
+    M = _diag6([m, m, m, j, j, j])
+    return ModelSpec("spring_mass_chain", x0, u, v, vd, s, [],
+                     [ConstraintSpec(FIXED_BC, 0, 0), ConstraintSpec(FIXED_BC, 0, nm + 1)], np.zeros(3),
+                     max_iter=int(fx["max_iter"]), h=2.0 * np.pi * np.sqrt(m / k) / fx["num_steps"],
+                     rho_inf=fx["rho_inf"], num_steps=int(fx["num_steps"]),
+                     masses=[MassSpec(i + 1, M) for i in range(nm)],
+                     springs=[SpringSpec(i, i + 1, k, fx["l0"]) for i in range(nm + 1)])
+
+
+def heavy_top_spec(fx: dict) -> ModelSpec:
+    """tests/regression_tests/regression/test_heavy_top.cpp: a spinning top (point mass with
+    inertia) held at a fixed pivot by a rigid joint to a three-dof ground node under gravity."""
+    nt = _NodeTable()
+    x = np.array(fx["x"])
+    om = np.array(fx["omega"])
+    nt.add(np.concatenate([x, [1, 0, 0, 0]]), v=np.concatenate([np.cross(om, x), om]),
+           vd=np.concatenate([fx["x_ddot"], fx["omega_dot"]]))
+    ground = nt.add([0, 0, 0, 1, 0, 0, 0])
+    x0, u, v, vd, s = nt.arrays()
+    M = _diag6([fx["mass"]] * 3 + list(fx["inertia"]))
+    cons = [ConstraintSpec(RIGID_JOINT_6DOFS_TO_3DOFS, 0, ground), ConstraintSpec(PRESCRIBED_BC_3DOFS, 0, ground)]
+    return ModelSpec("heavy_top", x0, u, v, vd, s, [], cons, np.array(fx["gravity"]), max_iter=int(fx["max_iter"]),
+                     h=fx["step_size"], rho_inf=fx["rho_inf"], atol=fx["atol"], rtol=fx["rtol"],
+                     num_steps=int(fx["num_steps"]), masses=[MassSpec(0, M)])
 
 
 def _diag6(vals):

@@ -639,18 +639,14 @@ System CreateSystem(const ModelIn& model) {
         c.base = ci.base;
         c.target = ci.target;
         const bool two = TwoNode(ci.type);
-        if (ci.type == CType::FixedBC || ci.type == CType::PrescribedBC) {
-            c.base_sig = kNone;
-            c.target_sig = kAll;
-        } else {
-            c.base_sig = kAll;
-            c.target_sig = kAll;
-        }
-        c.row_count = (ci.type == CType::RevoluteJoint) ? 5 : 6;
+        // freedom signatures and rows (constraints.hpp:181-214, constraint_type.hpp:60-79)
+        c.base_sig = two ? kAll : kNone;
+        c.target_sig = ThreeDof(ci.type) ? kJustPosition : kAll;
+        c.row_count = ThreeDof(ci.type) ? 3 : (ci.type == CType::RevoluteJoint) ? 5 : 6;
         c.row_first = start_row;
         start_row += c.row_count;
         Vec3 X0{0., 0., 0.};
-        if (ci.type != CType::PrescribedBC) {
+        if (!Prescribed(ci.type)) {
             const auto& t = model.nodes[ci.target].x0;
             if (!two) X0 = {t[0] - ci.axis[0], t[1] - ci.axis[1], t[2] - ci.axis[2]};
             else {
@@ -692,7 +688,7 @@ System CreateSystem(const ModelIn& model) {
             ax[2] = ci.axis[2];
         }
         c.axes = ax;
-        if (ci.type == CType::PrescribedBC) c.input = ci.init;
+        if (Prescribed(ci.type)) c.input = ci.init;
         s.cons.push_back(c);
     }
 
@@ -1088,7 +1084,7 @@ void UpdateConstraintVariables(System& s) {
         const Quat R2{tq[3], tq[4], tq[5], tq[6]};
         Vec3 u1{0., 0., 0.};
         Quat R1{1., 0., 0., 0.};
-        if (c.type == CType::PrescribedBC) {
+        if (Prescribed(c.type)) {
             u1 = {c.input[0], c.input[1], c.input[2]};
             R1 = {c.input[3], c.input[4], c.input[5], c.input[6]};
         } else if (TwoNode(c.type)) {
@@ -1101,7 +1097,15 @@ void UpdateConstraintVariables(System& s) {
         const Quat R1t = QuaternionInverse(R1);
         const Vec3 R1X0 = RotateVectorByQuaternion(R1, c.X0);
         for (int k = 0; k < 3; ++k) res[k] = u2[k] + c.X0[k] - u1[k] - R1X0[k];
-        if (c.type != CType::RevoluteJoint) {
+        if (ThreeDof(c.type)) {
+            // calculate_{fixed_bc_3DOF,prescribed_bc_3DOF,rigid_joint_3DOF}_constraint.hpp: the
+            // translational rows only, target gradient I, base gradient [-I tilde(R1 X0)]
+            for (int k = 0; k < 3; ++k) tg[k * 6 + k] = 1.;
+            if (c.type == CType::RigidJoint6DOFsTo3DOFs) {
+                for (int k = 0; k < 3; ++k) bg[k * 6 + k] = -1.;
+                put3(bg, 0, 3, VecTilde(R1X0));
+            }
+        } else if (c.type != CType::RevoluteJoint) {
             // calculate_{fixed_bc,prescribed_bc,rigid_joint}_constraint.hpp; the rotation
             // control (calculate_rotation_control_constraint.hpp:71-136) inserts the inverse of
             // the commanded rotation Rc = exp(axis * input(0)) between target and base

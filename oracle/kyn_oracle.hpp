@@ -88,12 +88,21 @@ enum class CType : int {  // src/constraints/constraint_type.hpp:9-21
     PrescribedBC = 2,
     RigidJoint = 3,
     RevoluteJoint = 4,
-    RotationControl = 5
+    RotationControl = 5,
+    FixedBC3DOFs = 6,
+    PrescribedBC3DOFs = 7,
+    RigidJoint6DOFsTo3DOFs = 8
 };
 // constraints between a base and a target node (constraint_type.hpp:28-33)
 inline bool TwoNode(CType t) {
-    return t == CType::RigidJoint || t == CType::RevoluteJoint || t == CType::RotationControl;
+    return t == CType::RigidJoint || t == CType::RevoluteJoint || t == CType::RotationControl ||
+           t == CType::RigidJoint6DOFsTo3DOFs;
 }
+// constraints on a target node with three (translational) dofs (constraint_type.hpp:64-70)
+inline bool ThreeDof(CType t) {
+    return t == CType::FixedBC3DOFs || t == CType::PrescribedBC3DOFs || t == CType::RigidJoint6DOFsTo3DOFs;
+}
+inline bool Prescribed(CType t) { return t == CType::PrescribedBC || t == CType::PrescribedBC3DOFs; }
 
 struct NodeIn {
     std::array<double, 7> x0{0., 0., 0., 1., 0., 0., 0.};

@@ -15,7 +15,8 @@ Sources (paths relative to the reference root):
   tests/unit_tests/system/masses/*.cpp                      -> functor_known_answers.json
   tests/regression_tests/regression/test_masses.cpp,
   tests/regression_tests/regression/test_precession.cpp,
-  tests/regression_tests/regression/test_spring_mass_system.cpp -> masses_springs.json
+  tests/regression_tests/regression/test_spring_mass_system.cpp,
+  tests/regression_tests/regression/test_heavy_top.cpp        -> masses_springs.json
   tests/regression_tests/interfaces/test_blade_interface.cpp,
   tests/regression_tests/interfaces/interfaces_test_files/IEA-15-240-RWT.yaml
                                                              -> blade_interface.json
@@ -271,7 +272,46 @@ def masses_springs(root):
         "expect_half": near_values(sbody[: sbody.index("time_step == num_steps)")], r"^q\(1,0\)"),
         "expect_end": near_values(sbody[sbody.index("time_step == num_steps)") :], r"^q\(1,0\)"),
     }
-    return {"masses_external_force": ext, "precession": precession, "spring_mass": spring_mass}
+    chain = sm[sm.index("SetUpSpringMassChainSystem") :]
+    spring_mass_chain = {
+        "source": "tests/regression_tests/regression/test_spring_mass_system.cpp (SpringMassChainSystemTest)",
+        "number_of_masses": numbers(re.search(r"number_of_masses = (\d+)U;", chain).group(1))[0],
+        "spacing": numbers(re.search(r"constexpr auto displacement = ([^;]+);", chain).group(1))[0],
+        "m": numbers(re.search(r"constexpr auto m = ([^;]+);", chain).group(1))[0],
+        "j": numbers(re.search(r"constexpr auto j = ([^;]+);", chain).group(1))[0],
+        "k": numbers(re.search(r"const auto k = ([^;]+);", chain).group(1))[0],
+        "l0": numbers(re.search(r"AddSpringElement\([^,]+,[^,]+, k, ([^)]+)\)", chain).group(1))[0],
+        "num_steps": numbers(re.search(r"num_steps = ([^;]+);", chain).group(1))[0],
+        "max_iter": numbers(re.search(r"max_iter\((\d+)\)", chain).group(1))[0],
+        "rho_inf": numbers(re.search(r"rho_inf\(([^)]+)\)", chain).group(1))[0],
+        "expect_q_x": 0.0,
+        "tol": numbers(re.search(r"EXPECT_NEAR\(q\(node, 0\), 0\., ([^)]+)\)", chain).group(1))[0],
+    }
+    ht = strip_comments((reg / "test_heavy_top.cpp").read_text())
+    hb = function_body(ht, "SetUpHeavyTopTest")
+
+    def arr3(name):
+        return numbers(re.search(name + r" =\s*std::array\{([^}]*)\}", hb).group(1))
+
+    heavy_top = {
+        "source": "tests/regression_tests/regression/test_heavy_top.cpp",
+        "gravity": numbers(re.search(r"SetGravity\(([^)]*)\)", hb).group(1)),
+        "mass": numbers(re.search(r"constexpr auto mass = ([^;]+);", hb).group(1))[0],
+        "inertia": arr3("inertia"),
+        "x": arr3(r"const auto x"),
+        "omega": arr3("omega"),
+        "omega_dot": arr3("omega_dot"),
+        "x_ddot": arr3("x_ddot"),
+        "max_iter": numbers(re.search(r"max_iter\((\d+)\)", hb).group(1))[0],
+        "step_size": numbers(re.search(r"step_size\(([^)]+)\)", hb).group(1))[0],
+        "rho_inf": numbers(re.search(r"rho_inf\(([^)]+)\)", hb).group(1))[0],
+        "atol": numbers(re.search(r"a_tol\(([^)]+)\)", hb).group(1))[0],
+        "rtol": numbers(re.search(r"r_tol\(([^)]+)\)", hb).group(1))[0],
+        "num_steps": numbers(re.search(r"iota\(0, (\d+)\)", hb).group(1))[0],
+        "expect_q": near_values(hb, r"^q\[0\]"),
+    }
+    return {"masses_external_force": ext, "precession": precession, "spring_mass": spring_mass,
+            "spring_mass_chain": spring_mass_chain, "heavy_top": heavy_top}
 
 
 def blade_interface(root):

@@ -194,3 +194,17 @@ def check_invariants(fx, name, get, tol=1e-12):
         worst = max(worst, float(np.max(np.abs(R @ axis - axis))), float(np.max(np.abs(q[azimuth, :3]))))
     assert worst <= tol, (name, worst)
     return worst
+
+
+def hinged_beam_spec(fx, num_steps=20):
+    """Not a reference case: the same beam hinged at its root under gravity.  The root is carried
+    by a three-dof ground node at the same point (RigidJoint6DOFsTo3DOFs root -> ground, ground
+    held by PrescribedBC3DOFs), so it keeps its position and rotates freely: a pendulum."""
+    nt = W._NodeTable()
+    ids, beam = _beam(fx, nt)
+    ground = nt.add(np.concatenate([nt.x0[ids[0]][:3], [1.0, 0.0, 0.0, 0.0]]))
+    x0, u, v, vd, s = nt.arrays()
+    cons = [W.ConstraintSpec(W.RIGID_JOINT_6DOFS_TO_3DOFS, ids[0], ground),
+            W.ConstraintSpec(W.PRESCRIBED_BC_3DOFS, 0, ground)]
+    return W.ModelSpec("hinged_beam", x0, u, v, vd, s, [beam], cons, np.array([0.0, 0.0, -9.81]), max_iter=10,
+                       h=0.01, rho_inf=0.9, atol=1e-5, rtol=1e-3, num_steps=num_steps, meta={"ground": ground})

@@ -1,8 +1,9 @@
 """Oracle point-mass and spring elements against the reference's regression tests
 (tests/regression_tests/regression/test_masses.cpp, test_precession.cpp,
-test_spring_mass_system.cpp; inputs and expected values in tests/golden/masses_springs.json),
-plus closed forms: the spring-mass oscillator's period and a spring's force and tangent
-against a finite difference."""
+test_spring_mass_system.cpp, test_heavy_top.cpp; inputs and expected values in
+tests/golden/masses_springs.json), plus closed forms: the spring-mass oscillator's period, a
+spring's force and tangent against a finite difference, the equidistant chain's equilibrium and
+the heavy top's fixed pivot distance (rigid joint to a three-dof ground node)."""
 
 import numpy as np
 import pytest
@@ -88,3 +89,49 @@ def test_spring_force_and_tangent_closed_form(orc):
     # nothing at rest (v = vd = 0, no gravity); node 0's dofs (0..2) carry +c1 r
     np.testing.assert_allclose(b[6:9], -c1 * r, rtol=1e-13, atol=1e-13)
     np.testing.assert_allclose(b[0:3], c1 * r, rtol=1e-13, atol=1e-13)
+
+
+def heavy_top_pivot_violation(get):
+    q, x0 = get("q"), get("x0")
+    top = x0[0, :3] + q[0, :3]
+    ground = x0[1, :3] + q[1, :3]
+    # the rigid joint keeps the top's pivot offset R X0 (X0 = ground - top); the ground is held
+    # at the origin by the three-dof prescribed BC
+    X0 = x0[1, :3] - x0[0, :3]
+    return max(float(np.max(np.abs(ground - top - W.quat_rotate(q[0, 3:], X0)))), float(np.max(np.abs(ground))))
+
+
+def test_heavy_top(orc, fx):
+    d = fx["heavy_top"]
+    spec = W.heavy_top_spec(d)
+    o = OracleSystem(orc, spec)
+    for k in range(spec.num_steps):
+        assert o.step() == 0, k
+    q = o.get("q")
+    for e in d["expect_q"]:
+        c = int(e["expr"].split("[")[-1].rstrip("]"))
+        assert abs(q[0, c] - e["value"]) <= e["tol"], (e, q[0, c])
+    assert heavy_top_pivot_violation(o.get) < 1e-12
+
+
+def test_spring_mass_chain_equilibrium(orc, fx):
+    d = fx["spring_mass_chain"]
+    spec = W.spring_mass_chain_spec(d)
+    o = OracleSystem(orc, spec)
+    for _ in range(spec.num_steps):
+        assert o.step() == 0
+        assert np.max(np.abs(o.get("q")[:, 0] - d["expect_q_x"])) <= d["tol"]
+
+
+def test_spring_mass_three_dof_anchor(orc, fx):
+    """The anchor held by FixedBC3DOFs (a spring-only node: three dofs, three constraint rows)
+    moves the mass exactly as the six-dof FixedBC anchor does."""
+    d = fx["spring_mass"]
+    a = OracleSystem(orc, W.spring_mass_spec(d))
+    b = OracleSystem(orc, W.spring_mass_spec(d, W.FIXED_BC_3DOFS))
+    assert b.num_dofs == a.num_dofs - 6  # 3 dofs + 3 rows instead of 6 + 6
+    for _ in range(a.spec.num_steps):
+        assert a.step() == 0 and b.step() == 0
+    np.testing.assert_allclose(b.get("q"), a.get("q"), rtol=0, atol=1e-14)
+    e = d["expect_end"][0]
+    assert abs(b.get("q")[1, 0] - e["value"]) <= e["tol"]

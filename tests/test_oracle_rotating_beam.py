@@ -6,6 +6,7 @@ generator torque on a tilted revolute shaft, each at the test's own tolerance; p
 kinematic relations each converged joint fixes (rigid carry, root = hub * exp(pitch x), rotation
 about the revolute axis only) at 1e-12."""
 
+import numpy as np
 import pytest
 
 import rotating_beam_cases as RB
@@ -32,3 +33,17 @@ def test_rotating_beam_case(orc, fx, name):
     worst = RB.check_case(fx, name, o.get, lambda c: o.constraint(c, 2))
     inv = RB.check_invariants(fx, name, o.get)
     print(f"{name}: worst deviation from the reference {worst:.2e}, joint invariants {inv:.1e}")
+
+
+def test_hinged_beam_pendulum(orc, fx):
+    """The beam hinged at its root through a three-dof ground node (RigidJoint6DOFsTo3DOFs +
+    PrescribedBC3DOFs) swings under gravity with its root fixed in space."""
+    spec = RB.hinged_beam_spec(fx)
+    o = OracleSystem(orc, spec)
+    assert o.num_dofs == 6 * 6 + 3 + 3 + 3
+    for i in range(spec.num_steps):
+        assert o.step() == 0, i
+    q = o.get("q")
+    assert np.max(np.abs(q[0, :3])) < 1e-12
+    assert np.max(np.abs(q[spec.meta["ground"], :3])) < 1e-14
+    assert q[5, 2] < -0.1  # the tip has fallen
