@@ -28,6 +28,12 @@ namespace kyn {
 #ifndef KYN_PHASE_MARK
 #define KYN_PHASE_MARK(i)
 #endif
+// Optional spans inside a phase (profiling builds): KYN_SPAN_BEGIN(v) takes a timestamp,
+// KYN_SPAN_END(slot, v, who) adds the elapsed cycles to accumulator `slot` from thread `who`.
+#ifndef KYN_SPAN_BEGIN
+#define KYN_SPAN_BEGIN(v)
+#define KYN_SPAN_END(slot, v, who)
+#endif
 
 constexpr int kLuTile = 144;  // row bookkeeping of the block LU (n <= kMaxBlockDofs < kLuTile)
 static_assert(kMaxBlockDofs < kLuTile && kMaxBlockDofs < 192, "block LU: 3 rows per panel lane, 8-bit row keys");
@@ -38,7 +44,7 @@ constexpr int kContractThreads = 252;  // 36 components x 7 node groups
 // ---------------------------------------------------------------------------------------
 struct Lay {
     // persistent
-    int loc, rhs, xs, xs2, piv, red, cres, misc, lg_of, phys_of, lcol;
+    int loc, rhs, xs, xs2, piv, red, cres, misc, phys_of, lcol, lu_sync;
     // overlay: element view
     int nq, nv, nvd, tr, phi, dphi, w, J, qpF, qpM, phiT;
     // overlay: matrix view
@@ -66,9 +72,9 @@ __host__ __device__ inline Lay make_layout(int Nm, int Qm, int nm, int cm, bool 
     L.red = take(16 * 8);
     L.cres = take(cm * 24 * 8);  // per constraint: res, lres_b, lres_t, sys_res
     L.misc = take(16 * 4);
-    L.lg_of = take(kLuTile * 4);
-    L.phys_of = take(kLuTile * 4);
+    L.phys_of = take(2 * kLuTile * 4);
     L.lcol = take(with_matrix ? 2 * 256 * tt_size : 0);  // two panels' L11, column-major 16 x 16
+    L.lu_sync = take(16);                                  // look-ahead counter of the blocked LU
     const int overlay = o;
     L.nq = take(Nm * 7 * 8);
     L.nv = take(Nm * 6 * 8);
@@ -348,16 +354,18 @@ __device__ __forceinline__ void element_phase(const DevSys& S, int e, char* sm, 
 // Blocked LU with partial pivoting on an augmented LDS matrix [A | b] (replaces
 // dss::numeric_factorization / dss::solve, src/step/solve_system.hpp:27-31).
 //
-// Right-looking, panels of 16 columns.  Wave 0 factors a panel in registers (lane l holds
-// physical rows l, l + 64, l + 128): per column a branch-free candidate per lane, a 64-lane
-// argmax by DPP row exchanges and permlane swaps, the pivot row broadcast by readlane, then
-// the in-panel rank-1 update; no barrier inside a panel.  All waves then form the panel's U
-// rows over the trailing columns (forward substitution with the panel's unit L) and update the
-// remaining rows, one column per lane pair.  Every entry sees the same FMA sequence as in the
-// oracle's unblocked loop (a_ij -= l_ik u_kj for k ascending), so results are bit-identical.
+// Right-looking, panels of 16 columns.  Wave 0 factors a panel in registers (lane l holds the
+// rows not yet pivoted, logical c0 + l, c0 + l + 64, ...): per column a candidate per lane, a
+// 64-lane argmax by DPP row exchanges and row broadcasts (FP64: max, then the lowest logical row
+// holding it; FP32: one reduction of a packed magnitude / row key), the pivot row broadcast by
+// readlane, then the in-panel rank-1 update; no barrier inside a panel.  The other waves form
+// the panel's U rows over the trailing columns (forward substitution with the panel's unit L)
+// and update the trailing rows, the next panel's columns first (look-ahead).  Every entry sees
+// the same FMA sequence as in the oracle's unblocked loop (a_ij -= l_ik u_kj for k ascending),
+// so FP64 factors are bit-identical to the oracle's.
 //
-// Pivoting is implicit: rows never move; lg_of/phys_of map physical <-> logical rows (the
-// position the oracle's dense LU holds a row at after its swaps).  Pivot: largest |a_ik| over
+// Pivoting is implicit: rows never move; phys_of maps logical -> physical rows (the position
+// the oracle's dense LU holds a row at after its swaps).  Pivot: largest |a_ik| over
 // logical rows i >= k, lowest logical index on ties, row k kept when a_kk is NaN.  The right-
 // hand side b is column n, so elimination applies P and L^-1 to it on the way.
 // On return: L (unit) and U at physical rows, y = L^-1 P b in column n, piv as the oracle's.
@@ -365,10 +373,15 @@ __device__ __forceinline__ void element_phase(const DevSys& S, int e, char* sm, 
 constexpr int kPanel = 16;
 
 struct LuScratch {
-    int* lg_of;    // logical row of each physical row (kLuTile)
-    int* phys_of;  // physical row of each logical row (kLuTile)
+    int* phys_of;  // physical row of each logical row: two buffers of kLuTile, the mapping after
+                   // panel p in buffer p & 1 (the next panel's factorization writes the other
+                   // buffer while this panel is still being applied); after the LU, buffer 0
     void* lcol;    // two 16 x 16 column-major L11 tiles (panel p in tile p & 1), storage type
+    int* sync;     // look-ahead counter: waves 1.. add one each when the next panel's columns are done
 };
+__device__ __forceinline__ int* phys_buf(const LuScratch& W, int c0) {
+    return W.phys_of + ((c0 / kPanel) & 1) * kLuTile;
+}
 template <typename TT>
 __device__ __forceinline__ TT* panel_l11(const LuScratch& W, int c0) {
     return reinterpret_cast<TT*>(W.lcol) + ((c0 / kPanel) & 1) * kPanel * kPanel;
@@ -484,35 +497,61 @@ __device__ __forceinline__ void panel_columns(TT (&P)[NS][kPanel], int (&lg)[NS]
     if constexpr (J < kPanel) {
         if (kFull || J < w) {
             const int k = c0 + J;
-            // largest |a_rk| over eligible rows (NaN never wins), then the lowest key holding it;
-            // both reduced over the 64 lanes, so every lane sees the same pivot
-            using CT = typename Cand<TT>::type;
-            CT vs[NS];
-            CT bv = CT(-1);
-            bool dnan = false;
+            int key;
+            if constexpr (sizeof(TT) == 4) {
+                // FP32: one reduction of (|a_rk| to 15 mantissa bits, 255 - logical row): the
+                // largest magnitude, and among magnitudes equal to 2^-15 the lowest logical row
+                // (NaN never wins; no candidate or a NaN a_kk keeps row k); the physical row of
+                // the winner from a ballot
+                int bk = -1;
+                bool dnan = false;
 #pragma unroll
-            for (int s = 0; s < NS; ++s) {
-                const TT v = P[s][J];
-                vs[s] = cand_of(v, lg[s] >= k);
-                bv = vs[s] > bv ? vs[s] : bv;
-                dnan |= (lg[s] == k) & (v != v);
-            }
-            bv = wave_max(bv);
-            int bk = 0x7fffffff;
-#pragma unroll
-            for (int s = 0; s < NS; ++s) {
-                const int key = (lg[s] << 8) | (lane + 64 * s);
-                bk = ((lg[s] >= k) & (vs[s] == bv)) ? (key < bk ? key : bk) : bk;
-            }
-            int key = wave_min(bk);
-            if (__any(dnan)) {  // the oracle keeps row k when a_kk is NaN
+                for (int s = 0; s < NS; ++s) {
+                    const TT v = P[s][J];
+                    const int c = (((__float_as_int(fabsf(v)) >> 8) << 8) | (255 - lg[s]));
+                    bk = (lg[s] >= k && v == v && c > bk) ? c : bk;
+                    dnan |= (lg[s] == k) & (v != v);
+                }
+                bk = wave_max(bk);
+                const int pl0 = (bk < 0 || __any(dnan)) ? k : 255 - (bk & 255);
                 int kp = 0;
 #pragma unroll
                 for (int s = 0; s < NS; ++s) {
-                    const unsigned long long m = __ballot(lg[s] == k);
+                    const unsigned long long m = __ballot(lg[s] == pl0);
                     kp = m ? (__builtin_ctzll(m) + 64 * s) : kp;
                 }
-                key = (k << 8) | kp;
+                key = (pl0 << 8) | kp;
+            } else {
+                // largest |a_rk| over eligible rows (NaN never wins), then the lowest key holding
+                // it; both reduced over the 64 lanes, so every lane sees the same pivot
+                using CT = typename Cand<TT>::type;
+                CT vs[NS];
+                CT bv = CT(-1);
+                bool dnan = false;
+#pragma unroll
+                for (int s = 0; s < NS; ++s) {
+                    const TT v = P[s][J];
+                    vs[s] = cand_of(v, lg[s] >= k);
+                    bv = vs[s] > bv ? vs[s] : bv;
+                    dnan |= (lg[s] == k) & (v != v);
+                }
+                bv = wave_max(bv);
+                int bk = 0x7fffffff;
+#pragma unroll
+                for (int s = 0; s < NS; ++s) {
+                    const int kk = (lg[s] << 8) | (lane + 64 * s);
+                    bk = ((lg[s] >= k) & (vs[s] == bv)) ? (kk < bk ? kk : bk) : bk;
+                }
+                key = wave_min(bk);
+                if (__any(dnan)) {  // the oracle keeps row k when a_kk is NaN
+                    int kp = 0;
+#pragma unroll
+                    for (int s = 0; s < NS; ++s) {
+                        const unsigned long long m = __ballot(lg[s] == k);
+                        kp = m ? (__builtin_ctzll(m) + 64 * s) : kp;
+                    }
+                    key = (k << 8) | kp;
+                }
             }
             const int plog = key >> 8, pphys = key & 255;
             const int ol = pphys & 63, slot = pphys >> 6;
@@ -551,25 +590,28 @@ __device__ __forceinline__ void panel_columns(TT (&P)[NS][kPanel], int (&lg)[NS]
     }
 }
 
-// Factor panel columns [c0, c0 + w) in wave 0 (NS = row slots per lane, n <= 64 NS).  Row
-// bookkeeping stays in registers (lg per lane row) and is written back once at the end of the
-// panel, together with the panel's unit-lower L11 as a compact column-major tile.
+// Factor panel columns [c0, c0 + w) in wave 0 over the rows not yet pivoted: logical rows
+// c0 .. n - 1 (NS = row slots per lane, n - c0 <= 64 NS), at the physical rows of the previous
+// panel's mapping.  Row bookkeeping stays in registers (lg per lane row) and is written back once
+// at the end of the panel into this panel's mapping buffer, together with the panel's unit-lower
+// L11 as a compact column-major tile.
 template <typename TT, int NS, bool kFull>
 __device__ __forceinline__ void lu_panel_ns(TT* A, int n, int lda, int c0, int w, int* piv, const LuScratch& W) {
     if (kFull) w = kPanel;
     const int lane = threadIdx.x & 63;
+    const int* ph_in = phys_buf(W, c0 + kPanel);  // the previous panel's buffer (identity before panel 0)
+    int* ph_out = phys_buf(W, c0);
     TT P[NS][kPanel];
-    int lg[NS];
+    int lg[NS], rr[NS];
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
-        const int r = lane + 64 * s;
-        const bool ok = r < n;
-        const int rc = ok ? r : 0;  // clamped: loads are unconditional, values selected after
-        lg[s] = W.lg_of[rc];
-        lg[s] = ok ? lg[s] : -1;
+        const int li = c0 + lane + 64 * s;
+        const bool ok = li < n;
+        lg[s] = ok ? li : -1;
+        rr[s] = ph_in[ok ? li : c0];  // clamped: loads are unconditional, values selected after
 #pragma unroll
         for (int jj = 0; jj < kPanel; ++jj) {
-            P[s][jj] = A[rc * lda + c0 + (jj < w ? jj : 0)];
+            P[s][jj] = A[rr[s] * lda + c0 + (jj < w ? jj : 0)];
             pin(P[s][jj]);
             P[s][jj] = (ok && jj < w) ? P[s][jj] : TT(0);
         }
@@ -585,18 +627,17 @@ __device__ __forceinline__ void lu_panel_ns(TT* A, int n, int lda, int c0, int w
         for (int j = 0; j < kPanel; ++j)
             if (kFull || j < w) piv[c0 + j] = pl[j];
     }
+    for (int i = lane; i < c0; i += 64) ph_out[i] = ph_in[i];  // rows pivoted before this panel
     TT* Lc = panel_l11<TT>(W, c0);
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
-        const int r = lane + 64 * s;
-        if (r < n) {
+        if (lg[s] >= 0) {
 #pragma unroll
             for (int jj = 0; jj < kPanel; ++jj)
-                if (jj < w) A[r * lda + c0 + jj] = P[s][jj];
-            W.lg_of[r] = lg[s];
-            W.phys_of[lg[s]] = r;
+                if (jj < w) A[rr[s] * lda + c0 + jj] = P[s][jj];
+            ph_out[lg[s]] = rr[s];
             const int i = lg[s] - c0;  // row of L11 this lane holds, if any
-            if (i >= 0 && i < w) {
+            if (i < w) {
 #pragma unroll
                 for (int k = 0; k < kPanel; ++k) Lc[k * kPanel + i] = k < i ? P[s][k] : TT(0);
             }
@@ -605,16 +646,17 @@ __device__ __forceinline__ void lu_panel_ns(TT* A, int n, int lda, int c0, int w
 }
 template <typename TT>
 __device__ __forceinline__ void lu_panel(TT* A, int n, int lda, int c0, int w, int* piv, const LuScratch& W) {
+    const int rows = n - c0;
     if (w < kPanel) {
-        if (n <= 64)
+        if (rows <= 64)
             lu_panel_ns<TT, 1, false>(A, n, lda, c0, w, piv, W);
-        else if (n <= 128)
+        else if (rows <= 128)
             lu_panel_ns<TT, 2, false>(A, n, lda, c0, w, piv, W);
         else
             lu_panel_ns<TT, 3, false>(A, n, lda, c0, w, piv, W);
-    } else if (n <= 64) {
+    } else if (rows <= 64) {
         lu_panel_ns<TT, 1, true>(A, n, lda, c0, w, piv, W);
-    } else if (n <= 128) {
+    } else if (rows <= 128) {
         lu_panel_ns<TT, 2, true>(A, n, lda, c0, w, piv, W);
     } else {
         lu_panel_ns<TT, 3, true>(A, n, lda, c0, w, piv, W);
@@ -653,10 +695,9 @@ __device__ __forceinline__ void u_rows_group(const TT* Lc, TT* u) {
 
 // Applies panel [c0, c0 + w) to this wave's share of the columns [ja, jb): columns
 // ja + g, ja + g + gs, ... (wave g of gs).  First the panel's U rows (U12 = L11^-1 A12, lane <->
-// column, written in place into the pivot rows) and the tail rows, then the trailing rows
-// a_rj -= sum_k l_rk u_kj (k ascending; lane <-> rows lane, lane + 64 (NSA slots) with their
-// multipliers in registers, zero for rows already pivoted, which are then rewritten unchanged;
-// two columns at a time).
+// column, written in place into the pivot rows), then the trailing rows a_rj -= sum_k l_rk u_kj
+// (k ascending; lane <-> logical rows j0 + lane + 64 s, s < NSA, with their multipliers in
+// registers; two columns at a time).  Rows already pivoted are not touched.
 // Runs within one wave: its LDS writes are seen by its own later reads without a barrier.
 template <typename TT, bool kFull, int NSA>
 __device__ __forceinline__ void lu_apply_panel(TT* A, int n, int lda, int c0, int w, int ja, int jb, int g, int gs,
@@ -666,9 +707,12 @@ __device__ __forceinline__ void lu_apply_panel(TT* A, int n, int lda, int c0, in
     const int j0 = c0 + w;  // rows with logical index >= j0 are trailing rows
     const int m = ja + g < jb ? (jb - ja - g + gs - 1) / gs : 0;
     if (m == 0) return;
+    // the row mapping after this panel: the next panel's factorization (wave 0, possibly
+    // running meanwhile) permutes rows among logical j0 .. n - 1 only and writes the other buffer
+    const int* ph = phys_buf(W, c0);
     int pr[kPanel];
 #pragma unroll
-    for (int i = 0; i < kPanel; ++i) pr[i] = __builtin_amdgcn_readfirstlane(W.phys_of[c0 + (i < w ? i : 0)]);
+    for (int i = 0; i < kPanel; ++i) pr[i] = __builtin_amdgcn_readfirstlane(ph[c0 + (i < w ? i : 0)]);
     // U rows: u_i -= l_ik u_k for k ascending (each entry's FMA sequence as in the unblocked
     // loop), one column per lane; L11 from the panel's compact column-major tile, where column
     // k's entries below the diagonal are contiguous (vector loads, same address on all lanes)
@@ -688,37 +732,24 @@ __device__ __forceinline__ void lu_apply_panel(TT* A, int n, int lda, int c0, in
 #pragma unroll
         for (int i = 1; i < kPanel; ++i)
             if (kFull || i < w) A[pr[i] * lda + j] = u[i];
-        // tail rows (physical rows >= 64 NSA, at most 16): updated here, lane <-> column, with
-        // this column's U values still in registers (a lane <-> row slot for so few rows would
-        // issue a full slot of instructions for a handful of active lanes)
-        for (int r = 64 * NSA; r < n; ++r) {
-            if (W.lg_of[r] >= j0) {
-                TT* Ar = A + r * lda;
-                TT a = Ar[j];
-#pragma unroll
-                for (int i = 0; i < kPanel; ++i)
-                    if (kFull || i < w) a -= Ar[c0 + i] * u[i];
-                Ar[j] = a;
-            }
-        }
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    // trailing rows of the NSA lane <-> row slots
+    // trailing rows
     TT L[NSA][kPanel];
+    int prw[NSA];
     bool ok[NSA];
 #pragma unroll
     for (int s = 0; s < NSA; ++s) {
-        const int r = lane + 64 * s;
-        ok[s] = r < n;
-        const int rc = ok[s] ? r : 0;
-        const bool act = ok[s] && W.lg_of[rc] >= j0;
+        const int li = j0 + lane + 64 * s;
+        ok[s] = li < n;
+        prw[s] = ph[ok[s] ? li : 0];
 #pragma unroll
         for (int i = 0; i < kPanel; ++i) {
-            L[s][i] = A[rc * lda + c0 + (i < w ? i : 0)];
+            L[s][i] = A[prw[s] * lda + c0 + (i < w ? i : 0)];
             pin(L[s][i]);
-            L[s][i] = (act && (kFull || i < w)) ? L[s][i] : TT(0);
+            L[s][i] = (ok[s] && (kFull || i < w)) ? L[s][i] : TT(0);
         }
     }
     for (int t = 0; t < m; t += 2) {
@@ -740,7 +771,7 @@ __device__ __forceinline__ void lu_apply_panel(TT* A, int n, int lda, int c0, in
 #pragma unroll
         for (int s = 0; s < NSA; ++s) {
             if (ok[s]) {
-                TT* Ar = A + (lane + 64 * s) * lda;
+                TT* Ar = A + prw[s] * lda;
                 TT a0 = Ar[j];
                 TT a1 = Ar[jq];
 #pragma unroll
@@ -758,8 +789,8 @@ __device__ __forceinline__ void lu_apply_panel(TT* A, int n, int lda, int c0, in
 template <typename TT>
 __device__ __forceinline__ void lu_apply(TT* A, int n, int lda, int c0, int w, int ja, int jb, int g, int gs,
                                          const LuScratch& W) {
-    // NSA lane <-> row slots; the remaining rows (at most 16) are tail rows
-    if (n > 80) {
+    // NSA lane <-> row slots for the n - (c0 + w) <= 127 trailing rows
+    if (n - (c0 + w) > 64) {
         if (w == kPanel)
             lu_apply_panel<TT, true, 2>(A, n, lda, c0, w, ja, jb, g, gs, W);
         else
@@ -771,37 +802,58 @@ __device__ __forceinline__ void lu_apply(TT* A, int n, int lda, int c0, int w, i
     }
 }
 
-// Panels with look-ahead: all waves first apply panel p to the next panel's columns; then wave
-// 0 factors the next panel while the other waves apply panel p to the remaining columns.  The
-// other waves read the row maps while wave 0 rewrites them at the end of a panel; the entries
-// they use (physical rows of panel p's pivots, whether a row is still trailing) do not change
-// there.
+// Panels with look-ahead: all waves first apply panel p to their share of the next panel's
+// columns; the waves other than wave 0 count themselves done on an LDS counter, and wave 0
+// waits for the count (every other wave adds one per panel unconditionally, so the wait ends)
+// and factors the next panel while the others apply panel p to the remaining columns.  One
+// barrier per panel.  The other waves read row maps that wave 0 rewrites meanwhile only where
+// they do not change (panel p's pivot rows; the set of trailing rows, taken from panel p's own
+// mapping buffer).
 template <typename TT>
 __device__ __forceinline__ void lu_factor_augmented(TT* A, int n, int lda, int* piv, const LuScratch& W,
                                                     int nrhs = 1) {
+    static_assert(kThreads >= 128, "the blocked LU needs wave 0 and at least one more wave");
     const int tid = threadIdx.x, wv = tid >> 6;
     const int nw = blockDim.x >> 6;
     const int nc = n + nrhs;  // right-hand sides ride along as columns n .. n + nrhs - 1
     for (int r = tid; r < kLuTile; r += blockDim.x) {
-        W.lg_of[r] = r;
         W.phys_of[r] = r;
+        W.phys_of[kLuTile + r] = r;
     }
+    if (tid == 0) *W.sync = 0;
     __syncthreads();
     if (wv == 0) lu_panel<TT>(A, n, lda, 0, n < kPanel ? n : kPanel, piv, W);
     __syncthreads();
+    int done = 0;  // look-ahead count wave 0 waits for
     for (int c0 = 0; c0 < n; c0 += kPanel) {
         const int w = (n - c0) < kPanel ? (n - c0) : kPanel;
         const int j0 = c0 + w;
         const int wn = j0 < n ? ((n - j0) < kPanel ? (n - j0) : kPanel) : 0;  // next panel's width
-        if (wn > 0) {
-            lu_apply<TT>(A, n, lda, c0, w, j0, j0 + wn, wv, nw, W);
-            __syncthreads();
-        }
+        KYN_SPAN_BEGIN(t_sp);
         if (wv == 0) {
-            if (wn > 0) lu_panel<TT>(A, n, lda, j0, wn, piv, W);
+            if (wn > 0) {
+                lu_apply<TT>(A, n, lda, c0, w, j0, j0 + wn, 0, nw, W);  // wave 0's share of the look-ahead
+                done += nw - 1;
+                while (__hip_atomic_load(W.sync, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) < done)
+                    __builtin_amdgcn_s_sleep(1);
+                KYN_SPAN_END(0, t_sp, 0);
+                lu_panel<TT>(A, n, lda, j0, wn, piv, W);
+            }
+            KYN_SPAN_END(1, t_sp, 0);
         } else {
+            if (wn > 0) {
+                lu_apply<TT>(A, n, lda, c0, w, j0, j0 + wn, wv, nw, W);
+                if ((tid & 63) == 0)
+                    __hip_atomic_fetch_add(W.sync, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+            }
             lu_apply<TT>(A, n, lda, c0, w, j0 + wn, nc, wv - 1, nw - 1, W);
+            KYN_SPAN_END(2, t_sp, 64);
         }
+        __syncthreads();
+        KYN_SPAN_END(3, t_sp, 0);
+    }
+    if (((n - 1) / kPanel) & 1) {  // final mapping into buffer 0
+        for (int i = tid; i < n; i += blockDim.x) W.phys_of[i] = W.phys_of[kLuTile + i];
         __syncthreads();
     }
 }
@@ -870,7 +922,7 @@ __device__ __forceinline__ void forward_sweep(const TT* A, int n, int lda, const
                 for (int a = 0; a < 3; ++a) {
                     const int i = lane + 64 * a;
                     const double upd = fma(-(double)Lc[a][c], yk, r[a]);
-                    r[a] = (i > k) & (i < n) ? upd : r[a];
+                    r[a] = ((i > k) & (i < n)) ? upd : r[a];
                 }
             }
         }
@@ -889,27 +941,48 @@ __device__ __forceinline__ void diag_of(const TT* A, int n, int lda, const int* 
 }
 
 // Back substitution U x = y with y in column n of the factored augmented matrix (logical row
-// i at physical row phys_of[i]).
+// i at physical row phys_of[i]), in blocks of kBackBlock rows from the bottom: wave 0 solves
+// the diagonal block (lane <-> row, x_k to the block's lanes by readlane), then every wave
+// updates the rows above it (thread <-> row).  Each row receives r_i -= u_ik x_k for k
+// descending and x_k = r_k / u_kk exactly as in the column-oriented loop, so x is the same,
+// bit for bit; the serial part is kBackBlock steps per block instead of one step per row.
+constexpr int kBackBlock = 16;
 template <typename TT>
 __device__ __forceinline__ void back_substitute(const TT* A, int n, int lda, const int* phys_of, double* xs) {
     const int tid = threadIdx.x;
-    if (tid < 64) {
-        const int lane = tid;
-        double r[3], dg[3], dinv[3];
-        int pr[3];
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            const int i = lane + 64 * a;
-            pr[a] = i < n ? phys_of[i] : 0;
-            r[a] = i < n ? (double)A[pr[a] * lda + n] : 0.;
-        }
-        diag_of<TT>(A, n, lda, pr, dg, dinv);
-        back_sweep<TT>(A, n, lda, pr, dg, dinv, r);
-#pragma unroll
-        for (int a = 0; a < 3; ++a)
-            if (lane + 64 * a < n) xs[lane + 64 * a] = r[a];
-    }
+    for (int i = tid; i < n; i += blockDim.x) xs[i] = (double)A[phys_of[i] * lda + n];
     __syncthreads();
+    for (int kb = ((n - 1) / kBackBlock) * kBackBlock; kb >= 0; kb -= kBackBlock) {
+        const int nb = (n - kb) < kBackBlock ? (n - kb) : kBackBlock;  // rows of this block
+        if (tid < 64) {
+            const int lane = tid;
+            const bool mine = lane < nb;
+            const int pr = phys_of[kb + (mine ? lane : 0)];
+            double r = mine ? xs[kb + lane] : 0.;
+            const double d = mine ? (double)A[pr * lda + kb + lane] : 1.;
+            const double dinv = 1. / d;
+            TT u[kBackBlock];
+#pragma unroll
+            for (int c = 0; c < kBackBlock; ++c) u[c] = A[pr * lda + kb + (c < nb ? c : 0)];
+#pragma unroll
+            for (int c = kBackBlock - 1; c >= 0; --c) {
+                if (c < nb) {  // uniform
+                    const double xk = rdlane(div_by(r, d, dinv), c);
+                    const double upd = fma(-(double)u[c], xk, r);
+                    r = lane < c ? upd : (lane == c ? xk : r);
+                }
+            }
+            if (mine) xs[kb + lane] = r;
+        }
+        __syncthreads();
+        for (int i = tid; i < kb; i += blockDim.x) {
+            const TT* Ar = A + phys_of[i] * lda;
+            double r = xs[i];
+            for (int k = kb + nb - 1; k >= kb; --k) r = fma(-(double)Ar[k], xs[k], r);
+            xs[i] = r;
+        }
+        __syncthreads();
+    }
 }
 
 // Back substitution U X = Y for the right-hand sides in columns n + c, c in [0, ncols), of the
@@ -1301,8 +1374,8 @@ __device__ __forceinline__ void factor_solve_uncondition(const DevSys& S, int b,
     }
     for (int i = t; i < n; i += blockDim.x) A[i * lda + n] = (TT)rhs[i];
     __syncthreads();
-    const LuScratch W{reinterpret_cast<int*>(sm + L.lg_of), reinterpret_cast<int*>(sm + L.phys_of),
-                      reinterpret_cast<void*>(sm + L.lcol)};
+    const LuScratch W{reinterpret_cast<int*>(sm + L.phys_of), reinterpret_cast<void*>(sm + L.lcol),
+                      reinterpret_cast<int*>(sm + L.lu_sync)};
     KYN_PHASE_MARK(7);
     lu_factor_augmented<TT>(A, n, lda, piv, W);
     KYN_PHASE_MARK(8);
@@ -1815,7 +1888,7 @@ __global__ void __launch_bounds__(kThreads) k_large_assemble(DevSys S) {
 
 __host__ __device__ inline int cond_lds_bytes(int ne, int me) {
     const int lda = (ne + me + 1) | 1;
-    return align16(ne * lda * 8) + 2 * kLuTile * 4 + 2 * 256 * 8 + align16(ne * 4);
+    return align16(ne * lda * 8) + 2 * kLuTile * 4 + 2 * 256 * 8 + 16 + align16(ne * 4);
 }
 
 // rin: right-hand side vector in the layout of S.rhs (the conditioned rhs, or a refinement
@@ -1837,11 +1910,12 @@ __device__ __forceinline__ void cond_factor_body(DevSys& S, const double* rin) {
     const int lda = (ne + nrhs) | 1;
     TT* M = reinterpret_cast<TT*>(sm);
     int off = align16(ne * lda * 8);
-    int* lg = reinterpret_cast<int*>(sm + off);
-    int* ph = lg + kLuTile;
+    int* ph = reinterpret_cast<int*>(sm + off);  // two buffers
     off += 2 * kLuTile * 4;
     void* lc = sm + off;
     off += 2 * 256 * 8;
+    int* sync = reinterpret_cast<int*>(sm + off);
+    off += 16;
     int* piv = reinterpret_cast<int*>(sm + off);
     const double* rhs = rin + S.block_vec_off[b];
     for (int k = t; k < ne * (ne + nrhs); k += blockDim.x) {  // column by column: coalesced reads
@@ -1853,7 +1927,7 @@ __device__ __forceinline__ void cond_factor_body(DevSys& S, const double* rin) {
         M[i * lda + j] = (TT)v;
     }
     __syncthreads();
-    const LuScratch W{lg, ph, lc};
+    const LuScratch W{ph, lc, sync};
     lu_factor_augmented<TT>(M, ne, lda, piv, W, nrhs);
     double* Y = S.Ybuf + S.cond_Y_off[ce];
     back_substitute_cols<TT>(M, ne, lda, ph, nrhs, Y, ne);

@@ -11,12 +11,21 @@ __device__ unsigned long long kyn_phase_acc[16];
         if (threadIdx.x == 0) atomicAdd(&kyn_phase_acc[i], (unsigned long long)clock64()); \
     } while (0)
 
+__device__ unsigned long long kyn_span_acc[8];
+#define KYN_SPAN_BEGIN(v) const unsigned long long v = clock64()
+#define KYN_SPAN_END(slot, v, who)                                                              \
+    do {                                                                                        \
+        if (threadIdx.x == (who)) atomicAdd(&kyn_span_acc[slot], (unsigned long long)(clock64() - (v))); \
+    } while (0)
+
 #include "kyn_kernels.hip"
 
 extern "C" int kyn_phase_read(unsigned long long* out) {
-    return hipMemcpyFromSymbol(out, HIP_SYMBOL(kyn_phase_acc), sizeof(unsigned long long) * 16) == hipSuccess ? 0 : -1;
+    if (hipMemcpyFromSymbol(out, HIP_SYMBOL(kyn_phase_acc), sizeof(unsigned long long) * 16) != hipSuccess) return -1;
+    return hipMemcpyFromSymbol(out + 16, HIP_SYMBOL(kyn_span_acc), sizeof(unsigned long long) * 8) == hipSuccess ? 0 : -1;
 }
 extern "C" int kyn_phase_reset() {
     unsigned long long z[16] = {0};
-    return hipMemcpyToSymbol(HIP_SYMBOL(kyn_phase_acc), z, sizeof(z)) == hipSuccess ? 0 : -1;
+    if (hipMemcpyToSymbol(HIP_SYMBOL(kyn_phase_acc), z, sizeof(z)) != hipSuccess) return -1;
+    return hipMemcpyToSymbol(HIP_SYMBOL(kyn_span_acc), z, sizeof(unsigned long long) * 8) == hipSuccess ? 0 : -1;
 }

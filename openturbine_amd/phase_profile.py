@@ -41,7 +41,7 @@ def main():
     lib.L.kyn_phase_reset()
     s.step_many(args.steps, use_graph=False)
     it1, _ = s.totals()
-    acc = (C.c_ulonglong * 16)()
+    acc = (C.c_ulonglong * 24)()
     lib.L.kyn_phase_read(acc)
     a = [int(v) for v in acc]
     blocks_per_model = np.array([len(sp.beams) for sp in specs])
@@ -54,6 +54,10 @@ def main():
         total += c
         print(f"  {name:32s} {c:10.0f} cycles")
     print(f"  {'total per block-iteration':32s} {total:10.0f} cycles")
+    spans = ["LU wait for look-ahead (wave 0)", "LU wait + panel (wave 0)", "LU look-ahead + trailing (wave 1)",
+             "LU panel steps incl. barrier"]
+    for i, name in enumerate(spans):
+        print(f"    {name:34s} {a[16 + i] / count:10.0f} cycles")
 
 
 if __name__ == "__main__":
