@@ -38,6 +38,10 @@ namespace kyn {
 constexpr int kLuTile = 144;  // row bookkeeping of the block LU (n <= kMaxBlockDofs < kLuTile)
 static_assert(kMaxBlockDofs < kLuTile && kMaxBlockDofs < 192, "block LU: 3 rows per panel lane, 8-bit row keys");
 constexpr int kContractThreads = 252;  // 36 components x 7 node groups
+// {phi_i(q), phi'_i(q)} pairs per QP, rows padded with zeros to 22: one QP's pairs are whole
+// 16-byte vectors (11 for FP32, 22 for FP64)
+constexpr int kPhiStride = 44;
+static_assert(kPhiStride >= 2 * kMaxElemNodes && kPhiStride % 4 == 0, "phi pair stride");
 
 // ---------------------------------------------------------------------------------------
 // LDS layout (identical on host and device)
@@ -86,7 +90,7 @@ __host__ __device__ inline Lay make_layout(int Nm, int Qm, int nm, int cm, bool 
     L.J = take(Qm * 8);
     L.qpF = take(Qm * 24 * 8);
     L.qpM = take(Qm * 6 * 36 * tt_size);
-    L.phiT = take(Qm * Nm * 2 * tt_size);  // {phi_i(q), phi'_i(q)} pairs, QP-major, tangent precision
+    L.phiT = take(Qm * kPhiStride * tt_size);  // {phi_i(q), phi'_i(q)} pairs, QP-major, tangent precision
     const int elem_end = o;
     L.lda = (nm + 1) | 1;  // [A | b]; odd row stride: row-per-lane accesses hit distinct banks
     L.A = overlay;
@@ -164,13 +168,17 @@ __device__ __forceinline__ void element_phase(const DevSys& S, int e, char* sm, 
         }
         loc[k] = kFused ? S.node_local[node] : 0;
     }
-    for (int k = tid; k < N * Q; k += blockDim.x) {
-        const double p = S.shape_interp[soff + k], dp = S.shape_deriv[soff + k];
-        phi[k] = p;
-        dphi[k] = dp;
+    for (int k = tid; k < (kPhiStride / 2) * Q; k += blockDim.x) {
         const int i = k / Q, q = k - Q * (k / Q);
-        phiT[(q * N + i) * 2] = (TT)p;
-        phiT[(q * N + i) * 2 + 1] = (TT)dp;
+        double p = 0., dp = 0.;
+        if (i < N) {
+            p = S.shape_interp[soff + k];
+            dp = S.shape_deriv[soff + k];
+            phi[k] = p;
+            dphi[k] = dp;
+        }
+        phiT[q * kPhiStride + 2 * i] = (TT)p;
+        phiT[q * kPhiStride + 2 * i + 1] = (TT)dp;
     }
     for (int q = tid; q < Q; q += blockDim.x) {
         wq[q] = S.qp_weight[qoff + q];
@@ -253,7 +261,8 @@ __device__ __forceinline__ void element_phase(const DevSys& S, int e, char* sm, 
     // Thread t < 252 owns component cc = t % 36 = (r, s) and nodes j = t / 36 + 7 b (b < 3);
     // per QP it forms its own R0_q[j][cc], R1_q[j][cc] from row r of the QP matrices and
     // column s of T_j, then accumulates S_ij[cc] += phi_i R0 + phi'_i R1 for every row i
-    // (phi pairs are LDS broadcasts).  Each R entry is formed by exactly one thread.
+    // (the QP's phi pairs: broadcast vector loads, all rows at once).  Each R entry is formed by
+    // exactly one thread.
     const int cc = tid % 36, jg = tid / 36;
     const bool active = tid < kContractThreads;
     const int r_ = cc / 6, s_ = cc - 6 * (cc / 6);
@@ -275,6 +284,7 @@ __device__ __forceinline__ void element_phase(const DevSys& S, int e, char* sm, 
     for (int i = 0; i < kMaxElemNodes; ++i)
 #pragma unroll
         for (int b = 0; b < 3; ++b) acc[i][b] = TT(0);
+    const int Nu = __builtin_amdgcn_readfirstlane(N);  // uniform: a scalar branch per row group
     for (int q = 0; q < Q; ++q) {
         const TT* M6 = qpM + (size_t)q * 6 * 36 + r_ * 6;
         TT ra[3], rp[3], ro[3], rc[3];
@@ -286,26 +296,45 @@ __device__ __forceinline__ void element_phase(const DevSys& S, int e, char* sm, 
             rc[u] = M6[180 + col0 + u];
         }
         const TT rm = M6[72 + s_];
-        const TT* ph = phiT + (size_t)q * N * 2;
+        const TT* ph = phiT + (size_t)q * kPhiStride;
+        TT pj[3], dj[3];
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+            pj[b] = ph[jc[b] * 2];
+            dj[b] = ph[jc[b] * 2 + 1];
+        }
+        // every row's pair for this QP in registers (whole vectors, one wait), then the rows
+        constexpr int VW = 16 / (int)sizeof(TT);
+        typedef TT vec_t __attribute__((ext_vector_type(16 / sizeof(TT))));
+        TT phv[kPhiStride];
+#pragma unroll
+        for (int c = 0; c < kPhiStride / VW; ++c) {
+            const vec_t v = *reinterpret_cast<const vec_t*>(ph + c * VW);
+#pragma unroll
+            for (int e = 0; e < VW; ++e) phv[c * VW + e] = v[e];
+        }
         TT R0[3], R1[3];
 #pragma unroll
         for (int b = 0; b < 3; ++b) {
-            const TT pj = ph[jc[b] * 2], dj = ph[jc[b] * 2 + 1];
-            TT v0 = pj * rm, v1 = TT(0);
+            TT v0 = pj[b] * rm, v1 = TT(0);
 #pragma unroll
             for (int u = 0; u < 3; ++u) {
-                v0 += (pj * ra[u] + dj * rp[u]) * tc[b][u];
-                v1 += (pj * ro[u] + dj * rc[u]) * tc[b][u];
+                v0 += (pj[b] * ra[u] + dj[b] * rp[u]) * tc[b][u];
+                v1 += (pj[b] * ro[u] + dj[b] * rc[u]) * tc[b][u];
             }
             R0[b] = v0;
             R1[b] = v1;
         }
+        // rows in groups of three behind a scalar branch on N (rows past N are skipped)
 #pragma unroll
-        for (int i = 0; i < kMaxElemNodes; ++i) {
-            if (i < N) {
-                const TT pi = ph[i * 2], di = ph[i * 2 + 1];
+        for (int i0 = 0; i0 < kMaxElemNodes; i0 += 3) {
+            if (i0 < Nu) {
 #pragma unroll
-                for (int b = 0; b < 3; ++b) acc[i][b] += pi * R0[b] + di * R1[b];
+                for (int i = i0; i < i0 + 3 && i < kMaxElemNodes; ++i) {
+                    const TT pi = phv[i * 2], di = phv[i * 2 + 1];
+#pragma unroll
+                    for (int b = 0; b < 3; ++b) acc[i][b] += pi * R0[b] + di * R1[b];
+                }
             }
         }
     }
