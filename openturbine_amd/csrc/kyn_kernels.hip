@@ -345,7 +345,11 @@ __device__ __forceinline__ void element_phase(const DevSys& S, int e, char* sm, 
     if (kFused) {
         // the overlay is free after the barrier above: zero the matrix view, then write
         TT* A = reinterpret_cast<TT*>(sm + L.A);
-        for (int k = tid; k < n * L.lda; k += blockDim.x) A[k] = TT(0);
+        {   // 16-byte stores; the matrix view is 16-byte aligned and reserved to a 16-byte multiple
+            uint4* A4 = reinterpret_cast<uint4*>(sm + L.A);
+            const int nv = (n * L.lda * (int)sizeof(TT) + 15) / 16;
+            for (int k = tid; k < nv; k += blockDim.x) A4[k] = make_uint4(0u, 0u, 0u, 0u);
+        }
         __syncthreads();
         const TT cond = (TT)S.p.conditioner;
         if (active) {
@@ -1005,9 +1009,17 @@ __device__ __forceinline__ void back_substitute(const TT* A, int n, int lda, con
         }
         __syncthreads();
         for (int i = tid; i < kb; i += blockDim.x) {
-            const TT* Ar = A + phys_of[i] * lda;
+            const TT* Ar = A + phys_of[i] * lda + kb;
+            TT u[kBackBlock];
+            double xk[kBackBlock];
+#pragma unroll
+            for (int c = 0; c < kBackBlock; ++c) {  // loads first (clamped to the block), one wait
+                u[c] = Ar[c < nb ? c : 0];
+                xk[c] = xs[kb + (c < nb ? c : 0)];
+            }
             double r = xs[i];
-            for (int k = kb + nb - 1; k >= kb; --k) r = fma(-(double)Ar[k], xs[k], r);
+#pragma unroll
+            for (int c = kBackBlock - 1; c >= 0; --c) r = c < nb ? fma(-(double)u[c], xk[c], r) : r;
             xs[i] = r;
         }
         __syncthreads();
