@@ -187,10 +187,12 @@ __device__ __forceinline__ void element_phase(const DevSys& S, int e, char* sm, 
     __syncthreads();
     KYN_PHASE_MARK(1);
 
-    // ---- QP values: two lanes per QP (inertial | stiffness)
-    for (int t = tid; t < 2 * Q; t += blockDim.x) {
-        const int q = t >> 1;
-        const int half = t & 1;
+    // ---- QP values: wave 0 the inertial terms, wave 1 the stiffness terms, lane <-> QP (Q <= 64:
+    // each wave runs one of the two code paths once, and the two waves run side by side)
+    static_assert(kMaxElemQps <= 64, "one lane per QP");
+    if (tid < 128 && (tid & 63) < Q) {
+        const int q = tid & 63;
+        const int half = tid >> 6;
         const double w = wq[q], J = Jq[q];
         const double* r0 = S.qp_r0 + (size_t)(qoff + q) * 4;
         TT* M6 = qpM + (size_t)q * 6 * 36;  // A1, A2, Mi, P, O, C
