@@ -152,14 +152,20 @@ def test_small_ensemble_fused_fp64_and_fp32(kyn, orc, refine):
     g64 = ota.KynSystem(kyn, specs, exec_mode=ota.EXEC_FUSED)
     g32 = ota.KynSystem(kyn, specs, exec_mode=ota.EXEC_FUSED, precision=ota.TANGENT_FP32, refine_sweeps=refine)
     oracles = [OracleSystem(orc, sp) for sp in specs]
+    iter_diffs = 0
     for _ in range(5):
         assert g64.step() == 0
         assert g32.step() == 0
         for o in oracles:
             assert o.step() == 0
         _, it64 = g64.status()
+        _, it32 = g32.status()
         for k, o in enumerate(oracles):
             assert it64[k] == len(o.convergence())
+        iter_diffs += int(np.sum(np.asarray(it32) != np.asarray(it64)))
+    # the FP32 tangent is an inexact Jacobian, yet the Newton path stays the FP64 one: iteration
+    # counts equal in all but rare model-steps (1 of 120 over 24 rotors x 5 steps)
+    assert iter_diffs <= 2, iter_diffs
     q64, q32 = g64.get("q"), g32.get("q")
     for k, o in enumerate(oracles):
         sl = g64.model_slice(k)
