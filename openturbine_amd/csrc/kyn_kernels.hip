@@ -976,55 +976,74 @@ __device__ __forceinline__ void diag_of(const TT* A, int n, int lda, const int* 
 }
 
 // Back substitution U x = y with y in column n of the factored augmented matrix (logical row
-// i at physical row phys_of[i]), in blocks of kBackBlock rows from the bottom: wave 0 solves
-// the diagonal block (lane <-> row, x_k to the block's lanes by readlane), then every wave
-// updates the rows above it (thread <-> row).  Each row receives r_i -= u_ik x_k for k
-// descending and x_k = r_k / u_kk exactly as in the column-oriented loop, so x is the same,
-// bit for bit; the serial part is kBackBlock steps per block instead of one step per row.
+// i at physical row phys_of[i]), in blocks of kBackBlock rows from the bottom.  Once a block's
+// x is known, wave 0 updates the rows of the block above with it and solves that block (lane <->
+// row, x_k to the block's lanes by readlane), while the other waves update all rows further up
+// (thread <-> row); one barrier per block.  Each row receives r_i -= u_ik x_k for k descending
+// and x_k = r_k / u_kk exactly as in the column-oriented loop, so x is the same, bit for bit.
 constexpr int kBackBlock = 16;
 template <typename TT>
+__device__ __forceinline__ double back_block_solve(const TT* A, int lda, const int* phys_of, int kb, int nb,
+                                                   double r, int lane) {
+    // rows kb .. kb + nb - 1 on lanes 0 .. nb - 1, r holding each row's value after the blocks below
+    const bool mine = lane < nb;
+    const int pr = phys_of[kb + (mine ? lane : 0)];
+    const double d = mine ? (double)A[pr * lda + kb + lane] : 1.;
+    const double dinv = 1. / d;
+    TT u[kBackBlock];
+#pragma unroll
+    for (int c = 0; c < kBackBlock; ++c) u[c] = A[pr * lda + kb + (c < nb ? c : 0)];
+#pragma unroll
+    for (int c = kBackBlock - 1; c >= 0; --c) {
+        if (c < nb) {  // uniform
+            const double xk = rdlane(div_by(r, d, dinv), c);
+            const double upd = fma(-(double)u[c], xk, r);
+            r = lane < c ? upd : (lane == c ? xk : r);
+        }
+    }
+    return r;
+}
+// row i -= its entries in block columns kb .. kb + nb - 1 times x, columns descending
+template <typename TT>
+__device__ __forceinline__ double back_row_update(const TT* A, int lda, const int* phys_of, int i, int kb, int nb,
+                                                  const double* xs, double r) {
+    const TT* Ar = A + phys_of[i] * lda + kb;
+    TT u[kBackBlock];
+    double xk[kBackBlock];
+#pragma unroll
+    for (int c = 0; c < kBackBlock; ++c) {  // loads first (clamped to the block), one wait
+        u[c] = Ar[c < nb ? c : 0];
+        xk[c] = xs[kb + (c < nb ? c : 0)];
+    }
+#pragma unroll
+    for (int c = kBackBlock - 1; c >= 0; --c) r = c < nb ? fma(-(double)u[c], xk[c], r) : r;
+    return r;
+}
+template <typename TT>
 __device__ __forceinline__ void back_substitute(const TT* A, int n, int lda, const int* phys_of, double* xs) {
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63;
     for (int i = tid; i < n; i += blockDim.x) xs[i] = (double)A[phys_of[i] * lda + n];
     __syncthreads();
-    for (int kb = ((n - 1) / kBackBlock) * kBackBlock; kb >= 0; kb -= kBackBlock) {
-        const int nb = (n - kb) < kBackBlock ? (n - kb) : kBackBlock;  // rows of this block
+    int kb = ((n - 1) / kBackBlock) * kBackBlock;
+    int nb = n - kb;  // the bottom block, possibly partial
+    if (tid < 64) {
+        const double r = back_block_solve<TT>(A, lda, phys_of, kb, nb, lane < nb ? xs[kb + lane] : 0., lane);
+        if (lane < nb) xs[kb + lane] = r;
+    }
+    __syncthreads();
+    for (; kb > 0; kb -= kBackBlock) {
+        const int kp = kb - kBackBlock;  // the block above (full)
         if (tid < 64) {
-            const int lane = tid;
-            const bool mine = lane < nb;
-            const int pr = phys_of[kb + (mine ? lane : 0)];
-            double r = mine ? xs[kb + lane] : 0.;
-            const double d = mine ? (double)A[pr * lda + kb + lane] : 1.;
-            const double dinv = 1. / d;
-            TT u[kBackBlock];
-#pragma unroll
-            for (int c = 0; c < kBackBlock; ++c) u[c] = A[pr * lda + kb + (c < nb ? c : 0)];
-#pragma unroll
-            for (int c = kBackBlock - 1; c >= 0; --c) {
-                if (c < nb) {  // uniform
-                    const double xk = rdlane(div_by(r, d, dinv), c);
-                    const double upd = fma(-(double)u[c], xk, r);
-                    r = lane < c ? upd : (lane == c ? xk : r);
-                }
-            }
-            if (mine) xs[kb + lane] = r;
+            double r = 0.;
+            if (lane < kBackBlock) r = back_row_update<TT>(A, lda, phys_of, kp + lane, kb, nb, xs, xs[kp + lane]);
+            r = back_block_solve<TT>(A, lda, phys_of, kp, kBackBlock, r, lane);
+            if (lane < kBackBlock) xs[kp + lane] = r;
+        } else {
+            for (int i = tid - 64; i < kp; i += blockDim.x - 64)
+                xs[i] = back_row_update<TT>(A, lda, phys_of, i, kb, nb, xs, xs[i]);
         }
         __syncthreads();
-        for (int i = tid; i < kb; i += blockDim.x) {
-            const TT* Ar = A + phys_of[i] * lda + kb;
-            TT u[kBackBlock];
-            double xk[kBackBlock];
-#pragma unroll
-            for (int c = 0; c < kBackBlock; ++c) {  // loads first (clamped to the block), one wait
-                u[c] = Ar[c < nb ? c : 0];
-                xk[c] = xs[kb + (c < nb ? c : 0)];
-            }
-            double r = xs[i];
-#pragma unroll
-            for (int c = kBackBlock - 1; c >= 0; --c) r = c < nb ? fma(-(double)u[c], xk[c], r) : r;
-            xs[i] = r;
-        }
-        __syncthreads();
+        nb = kBackBlock;
     }
 }
 
