@@ -8,7 +8,10 @@ increment criterion (atol 1e-10, rtol 1e-6).  Inputs are synthetic (SplitMix64 s
 20261202) and resident in HBM (device input schedule) before timing starts.
 
 Timed legs (all on the one GPU):
-  value            fused FP32-tangent form, one hipGraph-replayed kernel per step
+  value            fused FP32-tangent form with one FP64 iterative-refinement sweep per solve
+                   (--refine; the setting whose final state meets 1e-9 against FP64), one
+                   hipGraph-replayed kernel per step
+  fp32_without_refinement  the same without refinement (faster, 5e-9 off FP64 after 22 steps)
   fused_fp64       fused form, FP64 tangent
   plain_fp64       the reference's launch structure (one kernel per stage, host reads the
                    convergence error every Newton iteration), FP64
@@ -16,6 +19,9 @@ Roofline: the fused FP32 step kernel; algorithmic HBM bytes per launch, 8(67N + 
 per element and Newton iteration (s = 4), summed over the iterations actually taken, over its
 HIP-event-timed duration; flop rates (contraction 144 N^2 Q; with LU 2/3 n^3 and solves 2 n^2)
 as extra fields; measured traffic from the committed counter summary.
+final_state_fp32_vs_fp64 (extra field): the FP32-tangent leg's final q against the FP64 leg's
+after the same inputs and steps, relative to max |q| (whole ensemble and worst single model);
+the FP64 fused form equals the oracle bit for bit on the parity cases.
 cpu_baseline: the CPU oracle (same algorithm, dense pivoted LU instead of KLU) on the same
 ensemble (all rotors, 4 steps by default), OpenMP over rotors on the host cores.
 structure (extra field): the large single structure (tower 8 + shaft 8 + 3 x 16 blade elements
@@ -78,10 +84,12 @@ def contraction_flops(specs):
     return np.array([sum(144.0 * len(b.nodes) ** 2 * len(b.quad) for b in sp.beams) for sp in specs])
 
 
-def measured_traffic():
-    """HBM bytes per launch of the FP32 step kernel from the committed counter summary
-    (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs, copy-calibrated), if present."""
-    path = ROOT / "profiles" / "k_fused_step_f32_ensemble4096_traffic.txt"
+def measured_traffic(refine):
+    """HBM bytes per launch of the FP32 step kernel from the committed counter summary for this
+    refinement setting (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs, copy-calibrated), if present."""
+    name = f"k_fused_step_f32_refine{refine}_ensemble4096_traffic.txt" if refine else \
+        "k_fused_step_f32_ensemble4096_traffic.txt"
+    path = ROOT / "profiles" / name
     if not path.exists():
         return None
     for line in path.read_text().splitlines():
@@ -180,7 +188,9 @@ def main():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-plain", action="store_true")
     ap.add_argument("--skip-fp64", action="store_true")
-    ap.add_argument("--refine", type=int, default=0, help="FP64 refinement sweeps of the FP32-tangent solve")
+    ap.add_argument("--refine", type=int, default=1,
+                    help="FP64 refinement sweeps of the FP32-tangent solve (one meets the 1e-9 final-state bar)")
+    ap.add_argument("--skip-unrefined", action="store_true", help="skip the FP32 leg without refinement")
     ap.add_argument("--structure-steps", type=int, default=10, help="steps of the large-structure legs (0: skip)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run one device copy of BYTES (counter calibration) and exit")
@@ -212,7 +222,18 @@ def main():
                   warmup=args.warmup, steps=args.steps, refine=args.refine)
     log(f"  {args.steps} steps in {r32['wall_s']:.3f}s, fails {r32['fails']}, mean iters/model/step "
         f"{r32['iters'].mean() / args.steps:.2f}, step kernel {r32['kernel_ms']:.3f} ms")
+    q32 = r32["system"].get("q")
+    offs = np.append(np.asarray(r32["system"].node_off, dtype=np.int64), r32["system"].n)
     del r32["system"]
+
+    r0 = None
+    if args.refine > 0 and not args.skip_unrefined:
+        log("leg: fused FP32 tangent without refinement (hipGraph)")
+        r0 = run_leg(lib, specs, sched, exec_mode=ota.EXEC_FUSED, precision=ota.TANGENT_FP32,
+                     warmup=args.warmup, steps=args.steps, refine=0)
+        log(f"  {args.steps} steps in {r0['wall_s']:.3f}s, fails {r0['fails']}, step kernel {r0['kernel_ms']:.3f} ms")
+        q0 = r0["system"].get("q")
+        del r0["system"]
 
     r64 = None
     if not args.skip_fp64:
@@ -220,6 +241,7 @@ def main():
         r64 = run_leg(lib, specs, sched, exec_mode=ota.EXEC_FUSED, precision=ota.TANGENT_FP64,
                       warmup=args.warmup, steps=args.steps)
         log(f"  {args.steps} steps in {r64['wall_s']:.3f}s, fails {r64['fails']}")
+        q64 = r64["system"].get("q")
         del r64["system"]
 
     rp = None
@@ -235,6 +257,26 @@ def main():
         log("leg: large structure (plain form, condensed solve)")
         structure = structure_legs(lib, args.structure_steps)
 
+    # final state of the FP32-tangent leg against the FP64 leg (same inputs, same warmup + steps):
+    # max |dq| over the ensemble relative to max |q|, and the worst model's own relative deviation
+    def state_error(qa, qb):
+        d = np.max(np.abs(qa - qb), axis=1)
+        m = np.max(np.abs(qb), axis=1)
+        per_model = np.maximum.reduceat(d, offs[:-1]) / np.maximum.reduceat(m, offs[:-1])
+        return {"q_rel": float(np.max(d) / np.max(m)), "q_rel_worst_model": float(np.max(per_model)),
+                "after_steps": args.warmup + args.steps}
+
+    state_err = state_error(q32, q64) if r64 else None
+    if state_err:
+        log(f"  FP32-tangent final q vs FP64: {state_err['q_rel']:.2e} (worst model {state_err['q_rel_worst_model']:.2e})")
+    unrefined = None
+    if r0:
+        unrefined = {"steps_per_s": round(args.steps / r0["wall_s"], 4), "kernel_ms": round(r0["kernel_ms"], 4),
+                     "final_state_vs_fp64": state_error(q0, q64) if r64 else None}
+        if r64:
+            e = unrefined["final_state_vs_fp64"]
+            log(f"  without refinement: final q vs FP64 {e['q_rel']:.2e} (worst model {e['q_rel_worst_model']:.2e})")
+
     value = args.steps / r32["wall_s"]
     ms_per_step = 1000.0 * r32["wall_s"] / args.steps
     # roofline of the fused FP32 step kernel (HIP events around each graph launch of the timed
@@ -246,7 +288,7 @@ def main():
     mfma_flops_launch = float(np.sum(iters_per_step * contraction_flops(specs)))
     kern_s = (r32["kernel_ms"] if r32["kernel_ms"] > 0 else ms_per_step) / 1000.0
     achieved = bytes_launch / kern_s / 1e9
-    traffic = measured_traffic()
+    traffic = measured_traffic(args.refine)
     result = {
         "metric": METRIC,
         "value": round(value, 4),
@@ -269,6 +311,8 @@ def main():
         "newton_iters_per_model_step": {"fp32": round(float(r32["iters"].mean()) / args.steps, 3),
                                          "fp64": round(float(r64["iters"].mean()) / args.steps, 3) if r64 else None},
         "failed_model_steps": {"fp32": r32["fails"], "fp64": r64["fails"] if r64 else None},
+        "final_state_fp32_vs_fp64": state_err,
+        "fp32_without_refinement": unrefined,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": 8000.0, "unit": "GB/s",
                      "frac": round(achieved / 8000.0, 5),
                      "traffic": traffic["bytes_per_launch"] if traffic else None,

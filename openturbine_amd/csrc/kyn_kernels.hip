@@ -1070,30 +1070,23 @@ __device__ __forceinline__ void back_substitute_cols(const TT* A, int n, int lda
 }
 
 // Triangular solves with the LDS factors (logical row i at physical row phys_of[i]), FP64
-// accumulation, by wave 0.  b: LDS, pivoted here; x -> xs.
+// accumulation, by wave 0.  b: LDS, original row order, read through phys_of (not modified);
+// x -> xs.
 template <typename TT>
-__device__ __forceinline__ void lu_solve(const TT* A, int n, int lda, const int* piv, const int* phys_of, double* b,
-                                         double* xs) {
+__device__ __forceinline__ void lu_solve(const TT* A, int n, int lda, const int* phys_of, const double* b, double* xs) {
     const int tid = threadIdx.x;
-    if (tid == 0)
-        for (int k = 0; k < n; ++k) {
-            const int p = piv[k];
-            if (p != k) {
-                const double t = b[k];
-                b[k] = b[p];
-                b[p] = t;
-            }
-        }
-    __syncthreads();
     if (tid < 64) {
         const int lane = tid;
         double r[3], dg[3], dinv[3];
         int pr[3];
+        // P b: the factorization's row interchanges (k <-> piv[k], k ascending) composed on the
+        // identity are phys_of, so entry i of the interchanged b is b[phys_of[i]]: one gather
+        // instead of n sequential swaps
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
             const int i = lane + 64 * a;
             pr[a] = i < n ? phys_of[i] : 0;
-            r[a] = i < n ? b[i] : 0.;
+            r[a] = i < n ? b[pr[a]] : 0.;
         }
         diag_of<TT>(A, n, lda, pr, dg, dinv);
         forward_sweep<TT>(A, n, lda, pr, r);
@@ -1447,12 +1440,13 @@ __device__ __forceinline__ void factor_solve_uncondition(const DevSys& S, int b,
         for (int i = t; i < n; i += blockDim.x) {
             double r = S.rhs[off + i];
             const TT* col = Ac + i;
-#pragma unroll 4
+            // 16 loads of the column-major copy in flight per wait; j ascending as before
+#pragma unroll 16
             for (int j = 0; j < n; ++j) r -= (double)col[(size_t)j * n] * xs[j];
             rhs[i] = r;
         }
         __syncthreads();
-        lu_solve<TT>(A, n, lda, piv, W.phys_of, rhs, xs2);
+        lu_solve<TT>(A, n, lda, W.phys_of, rhs, xs2);
         for (int i = t; i < n; i += blockDim.x) xs[i] += xs2[i];
         __syncthreads();
     }
