@@ -276,6 +276,8 @@ struct Builder {
         block_con_begin, block_con_end, block_cons, block_node_begin, block_node_end, block_nodes,
         block_vec_off;
     std::vector<int64_t> block_A_off;
+    std::vector<int64_t> block_C_off;  // refinement copy (fused form), padded row stride
+    int64_t C_total = 0;
     std::vector<int> node_block, node_local, node_pin, block_npin;
     std::vector<int> elem_N, elem_Q, elem_node_off, elem_nodes, elem_qp_off, elem_shape_off;
     std::vector<int64_t> elem_mat_off;
@@ -946,6 +948,8 @@ kyn_system kyn_system_create(const kyn_model* models, size_t num_models, const k
         // matrix, library LU; the others keep the per-block LDS factorization
         std::vector<int> block_large(B.block_n.size(), 0);
         int64_t A_small = 0, A_big = 0;
+        B.block_C_off.assign(B.block_n.size(), 0);
+        B.C_total = 0;
         for (size_t b = 0; b < B.block_n.size(); ++b) {
             const int n = B.block_n[b];
             if (n > kyn::kMaxBlockDofs) {
@@ -964,6 +968,8 @@ kyn_system kyn_system_create(const kyn_model* models, size_t num_models, const k
                 nm = std::max(nm, n);
                 B.block_A_off[b] = A_small;
                 A_small += static_cast<int64_t>(n) * (n + 1);
+                B.block_C_off[b] = B.C_total;  // stays a multiple of 4 elements
+                B.C_total += static_cast<int64_t>(n) * ((n + 3) & ~3);
             }
         }
         B.A_total = A_small;
@@ -1047,6 +1053,7 @@ kyn_system kyn_system_create(const kyn_model* models, size_t num_models, const k
         S.block_node_end = Ar.up(B.block_node_end);
         S.block_nodes = Ar.up(B.block_nodes);
         S.block_A_off = Ar.up(B.block_A_off);
+        S.block_C_off = Ar.up(B.block_C_off);
         S.block_large = Ar.up(block_large);
         {
             std::vector<int64_t> big_off(B.block_n.size(), 0);
@@ -1294,7 +1301,7 @@ kyn_system kyn_system_create(const kyn_model* models, size_t num_models, const k
         const size_t tt = S.precision ? sizeof(float) : sizeof(double);
         S.A = sys->exec_mode == KYN_EXEC_PLAIN ? Ar.zeros<double>(static_cast<size_t>(B.A_total)) : nullptr;
         S.Acopy = (S.refine_sweeps > 0 && sys->exec_mode == KYN_EXEC_FUSED)
-                      ? static_cast<void*>(Ar.zeros<char>(static_cast<size_t>(B.A_total) * tt))
+                      ? static_cast<void*>(Ar.zeros<char>(static_cast<size_t>(B.C_total) * tt))
                       : nullptr;
         S.rhs = Ar.zeros<double>(B.vec_total);
         S.xsol = Ar.zeros<double>(B.vec_total);

@@ -1020,10 +1020,13 @@ __device__ __forceinline__ double back_row_update(const TT* A, int lda, const in
     return r;
 }
 template <typename TT>
-__device__ __forceinline__ void back_substitute(const TT* A, int n, int lda, const int* phys_of, double* xs) {
+__device__ __forceinline__ void back_substitute(const TT* A, int n, int lda, const int* phys_of, double* xs,
+                                                bool y_in_matrix = true) {
     const int tid = threadIdx.x, lane = tid & 63;
-    for (int i = tid; i < n; i += blockDim.x) xs[i] = (double)A[phys_of[i] * lda + n];
-    __syncthreads();
+    if (y_in_matrix) {  // else the caller has written y (FP64, logical order) to xs and synchronized
+        for (int i = tid; i < n; i += blockDim.x) xs[i] = (double)A[phys_of[i] * lda + n];
+        __syncthreads();
+    }
     int kb = ((n - 1) / kBackBlock) * kBackBlock;
     int nb = n - kb;  // the bottom block, possibly partial
     if (tid < 64) {
@@ -1070,14 +1073,16 @@ __device__ __forceinline__ void back_substitute_cols(const TT* A, int n, int lda
 }
 
 // Triangular solves with the LDS factors (logical row i at physical row phys_of[i]), FP64
-// accumulation, by wave 0.  b: LDS, original row order, read through phys_of (not modified);
-// x -> xs.
+// accumulation: forward substitution by wave 0, then the blocked back substitution of the
+// first solve (all waves; the same values as a column-oriented sweep).  b: LDS, original row
+// order, read through phys_of (not modified); x -> xs.
 template <typename TT>
 __device__ __forceinline__ void lu_solve(const TT* A, int n, int lda, const int* phys_of, const double* b, double* xs) {
     const int tid = threadIdx.x;
+    KYN_SPAN_BEGIN(t_fw);
     if (tid < 64) {
         const int lane = tid;
-        double r[3], dg[3], dinv[3];
+        double r[3];
         int pr[3];
         // P b: the factorization's row interchanges (k <-> piv[k], k ascending) composed on the
         // identity are phys_of, so entry i of the interchanged b is b[phys_of[i]]: one gather
@@ -1088,14 +1093,14 @@ __device__ __forceinline__ void lu_solve(const TT* A, int n, int lda, const int*
             pr[a] = i < n ? phys_of[i] : 0;
             r[a] = i < n ? b[pr[a]] : 0.;
         }
-        diag_of<TT>(A, n, lda, pr, dg, dinv);
         forward_sweep<TT>(A, n, lda, pr, r);
-        back_sweep<TT>(A, n, lda, pr, dg, dinv, r);
 #pragma unroll
         for (int a = 0; a < 3; ++a)
             if (lane + 64 * a < n) xs[lane + 64 * a] = r[a];
     }
     __syncthreads();
+    KYN_SPAN_END(6, t_fw, 0);
+    back_substitute<TT>(A, n, lda, phys_of, xs, false);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -1424,8 +1429,12 @@ __device__ __forceinline__ void factor_solve_uncondition(const DevSys& S, int b,
                                          double* rhs, double* xs, double* xs2, char* sm, const Lay& L) {
     const int n = S.block_n[b], nsys = S.block_nsys[b], off = S.block_vec_off[b];
     const int t = threadIdx.x;
-    if (Ac != nullptr) {  // column-major copy: the refinement residual reads it coalesced
-        for (int k = t; k < n * n; k += blockDim.x) Ac[k] = A[(k % n) * lda + (k / n)];
+    const int ldc = (n + 3) & ~3;  // row stride of the copy: whole 16-byte rows
+    if (Ac != nullptr) {  // row-major copy, zero-padded rows: the residual reads 16-byte vectors
+        for (int k = t; k < n * ldc; k += blockDim.x) {
+            const int i = k / ldc, j = k - ldc * (k / ldc);
+            Ac[k] = j < n ? A[i * lda + j] : TT(0);
+        }
     }
     for (int i = t; i < n; i += blockDim.x) A[i * lda + n] = (TT)rhs[i];
     __syncthreads();
@@ -1437,16 +1446,26 @@ __device__ __forceinline__ void factor_solve_uncondition(const DevSys& S, int b,
     back_substitute<TT>(A, n, lda, W.phys_of, xs);
     KYN_PHASE_MARK(9);
     for (int sweep = 0; Ac != nullptr && sweep < S.refine_sweeps; ++sweep) {
+        KYN_SPAN_BEGIN(t_rf);
+        typedef TT vec4_t __attribute__((ext_vector_type(4)));
         for (int i = t; i < n; i += blockDim.x) {
             double r = S.rhs[off + i];
-            const TT* col = Ac + i;
-            // 16 loads of the column-major copy in flight per wait; j ascending as before
-#pragma unroll 16
-            for (int j = 0; j < n; ++j) r -= (double)col[(size_t)j * n] * xs[j];
+            const vec4_t* row = reinterpret_cast<const vec4_t*>(Ac + (size_t)i * ldc);
+            // thread <-> row, four entries per load, j ascending
+#pragma unroll 8
+            for (int j4 = 0; j4 < ldc / 4; ++j4) {
+                const vec4_t v = row[j4];
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+                    if (4 * j4 + e < n) r -= (double)v[e] * xs[4 * j4 + e];
+            }
             rhs[i] = r;
         }
         __syncthreads();
+        KYN_SPAN_END(4, t_rf, 0);
+        KYN_SPAN_BEGIN(t_rs);
         lu_solve<TT>(A, n, lda, W.phys_of, rhs, xs2);
+        KYN_SPAN_END(5, t_rs, 0);
         for (int i = t; i < n; i += blockDim.x) xs[i] += xs2[i];
         __syncthreads();
     }
@@ -1508,8 +1527,8 @@ __device__ __forceinline__ void fused_step_body(const DevSys& S, int Nm, int Qm,
             block_constraints<TT>(S, b, sm, L);
             condition_rhs(S, b, rhs);
             KYN_PHASE_MARK(6);
-            // unfactored copy for refinement: compact rows (stride n) at the block's own offset
-            TT* Ac = (S.Acopy != nullptr && S.refine_sweeps > 0) ? reinterpret_cast<TT*>(S.Acopy) + S.block_A_off[b]
+            // unfactored copy for refinement: row-major, row stride (n + 3) & ~3, at the block's own offset
+            TT* Ac = (S.Acopy != nullptr && S.refine_sweeps > 0) ? reinterpret_cast<TT*>(S.Acopy) + S.block_C_off[b]
                                                                  : nullptr;
             factor_solve_uncondition<TT>(S, b, A, Ac, L.lda, piv, rhs, xs, xs2, sm, L);
             KYN_PHASE_MARK(10);

@@ -77,6 +77,8 @@ struct DevSys {
     const int *block_node_begin, *block_node_end;  // into block_nodes (ascending)
     const int* block_nodes;
     const int64_t* block_A_off;   // into A (elements of the storage type)
+    const int64_t* block_C_off;   // into Acopy: the refinement's row-major copy of the unfactored
+                                  // block, row stride (n + 3) & ~3 (whole 16-byte rows, 16-byte aligned)
     const int* block_vec_off;     // into rhs / xsol
 
     // ---- node -> block-local dof

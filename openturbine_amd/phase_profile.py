@@ -55,9 +55,11 @@ def main():
         print(f"  {name:32s} {c:10.0f} cycles")
     print(f"  {'total per block-iteration':32s} {total:10.0f} cycles")
     spans = ["LU wait for look-ahead (wave 0)", "LU wait + panel (wave 0)", "LU look-ahead + trailing (wave 1)",
-             "LU panel steps incl. barrier"]
+             "LU panel steps incl. barrier", "refinement residual (HBM copy)", "refinement solve",
+             "refinement forward sweep (wave 0)"]
     for i, name in enumerate(spans):
-        print(f"    {name:34s} {a[16 + i] / count:10.0f} cycles")
+        if i < 4 or a[16 + i]:
+            print(f"    {name:34s} {a[16 + i] / count:10.0f} cycles")
 
 
 if __name__ == "__main__":
