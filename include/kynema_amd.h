@@ -81,7 +81,9 @@ typedef struct {
 typedef struct {
     int exec_mode;     /* KYN_EXEC_PLAIN or KYN_EXEC_FUSED */
     int precision;     /* KYN_TANGENT_FP64 or KYN_TANGENT_FP32 */
-    int refine_sweeps; /* FP64 iterative-refinement sweeps of the FP32 solve (0..8) */
+    int refine_sweeps; /* FP64 iterative-refinement sweeps of the FP32 solve (0..8); on the
+                          4096-rotor ensemble one sweep keeps the final state within 1e-9 of the
+                          FP64 form after 22 steps (8e-10), none within 5e-9 */
 } kyn_exec_options;
 
 /* Last error message of the calling thread (empty if none). */
