@@ -491,9 +491,8 @@ StepParameters::StepParameters(bool dyn, size_t mi, double h_, double rho, doubl
 // populate_element_views.hpp, calculate_jacobian.hpp, constraints/constraints.hpp,
 // dof_management/*.hpp, solver/solver.hpp
 // ---------------------------------------------------------------------------------------
-namespace {
+// src/state/update_global_position.hpp:24-45
 void UpdateGlobalPositionNode(const double* q, const double* x0, double* x) {
-    // src/state/update_global_position.hpp:24-45
     x[0] = x0[0] + q[0];
     x[1] = x0[1] + q[1];
     x[2] = x0[2] + q[2];
@@ -501,6 +500,7 @@ void UpdateGlobalPositionNode(const double* q, const double* x0, double* x) {
     for (int k = 0; k < 4; ++k) x[3 + k] = r[k];
 }
 
+namespace {
 size_t uf_find(std::vector<size_t>& p, size_t i) {
     while (p[i] != i) {
         p[i] = p[p[i]];
@@ -772,7 +772,6 @@ System CreateSystem(const ModelIn& model) {
 // ---------------------------------------------------------------------------------------
 // Step pieces
 // ---------------------------------------------------------------------------------------
-namespace {
 // src/state/calculate_displacement.hpp:25-50
 void CalculateDisplacementNode(double h, const double* qd, const double* qp, double* q) {
     for (int c = 0; c < 3; ++c) q[c] = qp[c] + h * qd[c];
@@ -780,6 +779,7 @@ void CalculateDisplacementNode(double h, const double* qd, const double* qp, dou
     const Quat r = QuaternionCompose(dq, {qp[3], qp[4], qp[5], qp[6]});
     for (int c = 0; c < 4; ++c) q[3 + c] = r[c];
 }
+namespace {
 double& Aentry(System& s, size_t grow, size_t gcol) {
     const size_t b = s.gdof_block[grow];
     auto& blk = s.blocks[b];
@@ -788,17 +788,22 @@ double& Aentry(System& s, size_t grow, size_t gcol) {
 }  // namespace
 
 // src/step/predict_next_state.hpp:14-47 with src/state/calculate_next_state.hpp:26-37
+// src/state/calculate_next_state.hpp:26-37, one node
+void CalculateNextStateNode(double h, double alpha_f, double alpha_m, double beta, double gamma, double* qd,
+                            double* v, double* vd, double* a) {
+    for (int c = 0; c < 6; ++c) {
+        const double v_p = v[c], vd_p = vd[c], a_p = a[c];
+        vd[c] = 0.;
+        a[c] = (alpha_f * vd_p - alpha_m * a_p) / (1. - alpha_m);
+        v[c] = v_p + h * (1. - gamma) * a_p + gamma * h * a[c];
+        qd[c] = v_p + (0.5 - beta) * h * a_p + beta * h * a[c];
+    }
+}
 void PredictNextState(const StepParameters& p, System& s) {
     s.q_prev = s.q;
     for (size_t i = 0; i < s.num_nodes; ++i)
-        for (int c = 0; c < 6; ++c) {
-            const size_t k = i * 6 + c;
-            const double v_p = s.v[k], vd_p = s.vd[k], a_p = s.a[k];
-            s.vd[k] = 0.;
-            s.a[k] = (p.alpha_f * vd_p - p.alpha_m * a_p) / (1. - p.alpha_m);
-            s.v[k] = v_p + p.h * (1. - p.gamma) * a_p + p.gamma * p.h * s.a[k];
-            s.q_delta[k] = v_p + (0.5 - p.beta) * p.h * a_p + p.beta * p.h * s.a[k];
-        }
+        CalculateNextStateNode(p.h, p.alpha_f, p.alpha_m, p.beta, p.gamma, &s.q_delta[i * 6], &s.v[i * 6],
+                               &s.vd[i * 6], &s.a[i * 6]);
     for (size_t i = 0; i < s.num_nodes; ++i)
         CalculateDisplacementNode(p.h, &s.q_delta[i * 6], &s.q_prev[i * 7], &s.q[i * 7]);
     s.time_step++;
@@ -908,46 +913,83 @@ void CalculateQuadraturePointValues(double beta_prime, double gamma_prime, const
             }
         }
     }
-    // integrate_residual_vector.hpp:32-50
+    IntegrateResidualVector(N, Q, e.weight.data(), e.jacobian.data(), e.shape_interp.data(), e.shape_deriv.data(),
+                            Fc.data(), Fd.data(), Fi.data(), e.qp_Fe.data(), Fg.data(), e.node_FX.data(),
+                            e.residual.data());
+    std::vector<double> K(N * N * 36), M(N * N * 36);
+    IntegrateStiffnessMatrix(N, Q, e.weight.data(), e.jacobian.data(), e.shape_interp.data(), e.shape_deriv.data(),
+                             Kuu.data(), Puu.data(), Cuu.data(), Ouu.data(), Quu.data(), K.data());
+    IntegrateInertiaMatrix(N, Q, e.weight.data(), e.jacobian.data(), e.shape_interp.data(), beta_prime, gamma_prime,
+                           Muu.data(), Guu.data(), M.data());
+    std::vector<const double*> T(N);
+    for (size_t j = 0; j < N; ++j) T[j] = &s.tangent[e.node_idx[j] * 36];
+    CalculateSystemMatrix(N, M.data(), K.data(), T.data(), e.sysmat.data());
+}
+
+// src/system/beams/integrate_residual_vector.hpp:32-50: R_i = sum_q w (phi'_i Fc + J phi_i (Fd + Fi - Fe - Fg)) - FX_i
+void IntegrateResidualVector(size_t N, size_t Q, const double* weight, const double* jacobian, const double* phi,
+                             const double* dphi, const double* Fc, const double* Fd, const double* Fi,
+                             const double* Fe, const double* Fg, const double* node_FX, double* residual) {
     for (size_t k = 0; k < N; ++k) {
         std::array<double, 6> loc{};
         for (size_t q = 0; q < Q; ++q) {
-            const double w = e.weight[q];
-            const double cc = w * e.shape_deriv[k * Q + q];
-            const double cd = w * e.jacobian[q] * e.shape_interp[k * Q + q];
+            const double w = weight[q];
+            const double cc = w * dphi[k * Q + q];
+            const double cd = w * jacobian[q] * phi[k * Q + q];
             for (int c = 0; c < 6; ++c)
-                loc[c] += cc * Fc[q * 6 + c] +
-                          cd * (Fd[q * 6 + c] + Fi[q * 6 + c] - e.qp_Fe[q * 6 + c] - Fg[q * 6 + c]);
+                loc[c] += cc * Fc[q * 6 + c] + cd * (Fd[q * 6 + c] + Fi[q * 6 + c] - Fe[q * 6 + c] - Fg[q * 6 + c]);
         }
-        for (int c = 0; c < 6; ++c) e.residual[k * 6 + c] = loc[c] - e.node_FX[k * 6 + c];
+        for (int c = 0; c < 6; ++c) residual[k * 6 + c] = loc[c] - node_FX[k * 6 + c];
     }
-    // integrate_stiffness_matrix.hpp:34-96, integrate_inertia_matrix.hpp:31-78,
-    // calculate_system_matrix.hpp:24-53
+}
+// src/system/beams/integrate_stiffness_matrix.hpp:34-96: K_ij = sum_q phi_i phi_j wJ (Kuu + Quu)
+// + phi_i phi'_j w Puu + phi'_i phi'_j (w/J) Cuu + phi'_i phi_j w Ouu, out [N][N][36]
+void IntegrateStiffnessMatrix(size_t N, size_t Q, const double* weight, const double* jacobian, const double* phi,
+                              const double* dphi, const Mat6* Kuu, const Mat6* Puu, const Mat6* Cuu, const Mat6* Ouu,
+                              const Mat6* Quu, double* out) {
     for (size_t i = 0; i < N; ++i)
         for (size_t j = 0; j < N; ++j) {
-            Mat6 Ks{}, Mi{};
+            Mat6 Ks{};
             for (size_t q = 0; q < Q; ++q) {
-                const double w = e.weight[q], J = e.jacobian[q];
-                const double p1 = e.shape_interp[i * Q + q], p2 = e.shape_interp[j * Q + q];
-                const double d1 = e.shape_deriv[i * Q + q], d2 = e.shape_deriv[j * Q + q];
+                const double w = weight[q], J = jacobian[q];
+                const double p1 = phi[i * Q + q], p2 = phi[j * Q + q];
+                const double d1 = dphi[i * Q + q], d2 = dphi[j * Q + q];
                 const double K = (p1 * p2) * (w * J);
                 const double P = (p1 * d2) * w;
                 const double C = (d1 * d2) * (w / J);
                 const double O = (d1 * p2) * w;
-                const double coeff = p1 * p2 * w * J;
-                for (int c = 0; c < 36; ++c) {
-                    Ks[c] = Ks[c] + K * (Kuu[q][c] + Quu[q][c]) + P * Puu[q][c] + C * Cuu[q][c] +
-                            O * Ouu[q][c];
-                    Mi[c] = Mi[c] + coeff * (beta_prime * Muu[q][c] + gamma_prime * Guu[q][c]);
-                }
+                for (int c = 0; c < 36; ++c)
+                    Ks[c] = Ks[c] + K * (Kuu[q][c] + Quu[q][c]) + P * Puu[q][c] + C * Cuu[q][c] + O * Ouu[q][c];
             }
-            const double* T = &s.tangent[e.node_idx[j] * 36];
-            double* S = &e.sysmat[(i * N + j) * 36];
+            std::copy(Ks.begin(), Ks.end(), out + (i * N + j) * 36);
+        }
+}
+// src/system/beams/integrate_inertia_matrix.hpp:31-78: M_ij = sum_q phi_i phi_j wJ (beta' Muu + gamma' Guu)
+void IntegrateInertiaMatrix(size_t N, size_t Q, const double* weight, const double* jacobian, const double* phi,
+                            double beta_prime, double gamma_prime, const Mat6* Muu, const Mat6* Guu, double* out) {
+    for (size_t i = 0; i < N; ++i)
+        for (size_t j = 0; j < N; ++j) {
+            Mat6 Mi{};
+            for (size_t q = 0; q < Q; ++q) {
+                const double w = weight[q], J = jacobian[q];
+                const double coeff = phi[i * Q + q] * phi[j * Q + q] * w * J;
+                for (int c = 0; c < 36; ++c) Mi[c] = Mi[c] + coeff * (beta_prime * Muu[q][c] + gamma_prime * Guu[q][c]);
+            }
+            std::copy(Mi.begin(), Mi.end(), out + (i * N + j) * 36);
+        }
+}
+// src/system/beams/calculate_system_matrix.hpp:24-53: S_ij = K_ij T_j + M_ij (T_j: node j's stored tangent)
+void CalculateSystemMatrix(size_t N, const double* M, const double* K, const double* const* T, double* S) {
+    for (size_t i = 0; i < N; ++i)
+        for (size_t j = 0; j < N; ++j) {
+            const double* Kij = K + (i * N + j) * 36;
+            const double* Mij = M + (i * N + j) * 36;
+            double* Sij = S + (i * N + j) * 36;
             for (int r = 0; r < 6; ++r)
                 for (int c = 0; c < 6; ++c) {
                     double acc = 0.;
-                    for (int k = 0; k < 6; ++k) acc += Ks[r * 6 + k] * T[k * 6 + c];
-                    S[r * 6 + c] = acc + Mi[r * 6 + c];
+                    for (int k = 0; k < 6; ++k) acc += Kij[r * 6 + k] * T[j][k * 6 + c];
+                    Sij[r * 6 + c] = acc + Mij[r * 6 + c];
                 }
         }
 }
@@ -1299,17 +1341,28 @@ double CalculateConvergenceError(const StepParameters& p, const System& s) {
     return std::sqrt((sys + con) / static_cast<double>(s.num_dofs));
 }
 // src/step/update_state_prediction.hpp:25-68, src/state/update_{dynamic,static}_prediction.hpp
+// src/state/update_dynamic_prediction.hpp:34-44, one node with `nd` active dofs
+void UpdateDynamicPredictionNode(double h, double beta_prime, double gamma_prime, const double* x, size_t nd,
+                                 double* qd, double* v, double* vd) {
+    for (size_t c = 0; c < nd; ++c) {
+        const double dx = x[c];
+        qd[c] += dx / h;
+        v[c] += gamma_prime * dx;
+        vd[c] += beta_prime * dx;
+    }
+}
+// src/state/update_static_prediction.hpp:12-38, one node with `nd` active dofs
+void UpdateStaticPredictionNode(double h, const double* x, size_t nd, double* qd) {
+    for (size_t c = 0; c < nd; ++c) qd[c] += x[c] / h;
+}
 void UpdateStatePrediction(const StepParameters& p, System& s) {
     for (size_t i = 0; i < s.num_nodes; ++i) {
         const size_t nd = s.active_dofs[i], d0 = s.dof_map[i];
-        for (size_t c = 0; c < nd; ++c) {
-            const double dx = s.xsol[d0 + c];
-            s.q_delta[i * 6 + c] += dx / p.h;
-            if (p.is_dynamic_solve) {
-                s.v[i * 6 + c] += p.gamma_prime * dx;
-                s.vd[i * 6 + c] += p.beta_prime * dx;
-            }
-        }
+        if (p.is_dynamic_solve)
+            UpdateDynamicPredictionNode(p.h, p.beta_prime, p.gamma_prime, &s.xsol[d0], nd, &s.q_delta[i * 6],
+                                        &s.v[i * 6], &s.vd[i * 6]);
+        else
+            UpdateStaticPredictionNode(p.h, &s.xsol[d0], nd, &s.q_delta[i * 6]);
     }
     for (size_t i = 0; i < s.num_nodes; ++i)
         CalculateDisplacementNode(p.h, &s.q_delta[i * 6], &s.q_prev[i * 7], &s.q[i * 7]);
@@ -1320,8 +1373,12 @@ void UpdateConstraintPrediction(System& s) {
         for (size_t r = 0; r < c.row_count; ++r) c.lambda[r] += s.xsol[s.num_system_dofs + c.row_first + r];
 }
 // src/step/step.hpp:85-121
+// src/state/update_algorithmic_acceleration.hpp:18-23, one node
+void UpdateAlgorithmicAccelerationNode(double alpha_f, double alpha_m, const double* vd, double* a) {
+    for (int c = 0; c < 6; ++c) a[c] += (1. - alpha_f) / (1. - alpha_m) * vd[c];
+}
 void StepEpilogue(const StepParameters& p, System& s) {
-    for (size_t k = 0; k < s.num_nodes * 6; ++k) s.a[k] += (1. - p.alpha_f) / (1. - p.alpha_m) * s.vd[k];
+    for (size_t i = 0; i < s.num_nodes; ++i) UpdateAlgorithmicAccelerationNode(p.alpha_f, p.alpha_m, &s.vd[i * 6], &s.a[i * 6]);
     for (size_t i = 0; i < s.num_nodes; ++i) UpdateGlobalPositionNode(&s.q[i * 7], &s.x0[i * 7], &s.x[i * 7]);
     // calculate_constraint_output.hpp, calculate_revolute_joint_output.hpp
     for (auto& c : s.cons) {

@@ -78,6 +78,23 @@ Mat6 CalculateInertiaStiffnessMatrix(double m, const Vec3& u_ddot, const Vec3& o
                                      const Vec3& omega_dot, const Vec3& eta, const Mat3& rho,
                                      const Mat3& omega_tilde, const Mat3& omega_dot_tilde);
 Mat6 CalculateTangentOperatorT(double h, const double* q_delta6);  // stores T^T
+void IntegrateResidualVector(size_t N, size_t Q, const double* weight, const double* jacobian, const double* phi,
+                             const double* dphi, const double* Fc, const double* Fd, const double* Fi,
+                             const double* Fe, const double* Fg, const double* node_FX, double* residual);
+void IntegrateStiffnessMatrix(size_t N, size_t Q, const double* weight, const double* jacobian, const double* phi,
+                              const double* dphi, const Mat6* Kuu, const Mat6* Puu, const Mat6* Cuu, const Mat6* Ouu,
+                              const Mat6* Quu, double* out);
+void IntegrateInertiaMatrix(size_t N, size_t Q, const double* weight, const double* jacobian, const double* phi,
+                            double beta_prime, double gamma_prime, const Mat6* Muu, const Mat6* Guu, double* out);
+void CalculateSystemMatrix(size_t N, const double* M, const double* K, const double* const* T, double* S);
+void CalculateNextStateNode(double h, double alpha_f, double alpha_m, double beta, double gamma, double* qd,
+                            double* v, double* vd, double* a);
+void CalculateDisplacementNode(double h, const double* qd, const double* qp, double* q);
+void UpdateDynamicPredictionNode(double h, double beta_prime, double gamma_prime, const double* x, size_t nd,
+                                 double* qd, double* v, double* vd);
+void UpdateStaticPredictionNode(double h, const double* x, size_t nd, double* qd);
+void UpdateAlgorithmicAccelerationNode(double alpha_f, double alpha_m, const double* vd, double* a);
+void UpdateGlobalPositionNode(const double* q, const double* x0, double* x);
 
 // ---------------------------------------------------------------------------------------
 // Model inputs (src/model/model.hpp, node.hpp; src/elements/beams/beam_element.hpp)

@@ -20,6 +20,24 @@ struct OrcSystem {
 };
 }  // namespace
 
+namespace {
+Mat6 m6(const double* a) {
+    Mat6 m{};
+    std::copy(a, a + 36, m.begin());
+    return m;
+}
+Mat3 m3(const double* a) {
+    Mat3 m{};
+    std::copy(a, a + 9, m.begin());
+    return m;
+}
+Vec3 v3(const double* a) { return {a[0], a[1], a[2]}; }
+template <typename A>
+void put(const A& a, double* out) {
+    std::copy(a.begin(), a.end(), out);
+}
+}  // namespace
+
 extern "C" {
 
 const char* orc_last_error() { return g_err.c_str(); }
@@ -284,46 +302,97 @@ void orc_fn_rotate_section(const double* q, const double* C, double* out) {
     const auto r = RotateSectionMatrix({q[0], q[1], q[2], q[3]}, c);
     std::copy(r.begin(), r.end(), out);
 }
-void orc_fn_ouu_puu_quu(const double* C, const double* x0pupSS, const double* Mt, const double* Nt,
-                        double* O, double* P, double* Q) {
-    Mat6 c{};
-    Mat3 x{}, m{}, nn{};
-    std::copy(C, C + 36, c.begin());
-    std::copy(x0pupSS, x0pupSS + 9, x.begin());
-    std::copy(Mt, Mt + 9, m.begin());
-    std::copy(Nt, Nt + 9, nn.begin());
-    const auto o = CalculateOuu(c, x, m, nn);
-    const auto p = CalculatePuu(c, x, nn);
-    const auto q = CalculateQuu(c, x, nn);
-    std::copy(o.begin(), o.end(), O);
-    std::copy(p.begin(), p.end(), P);
-    std::copy(q.begin(), q.end(), Q);
+// beams/calculate_{Ouu,Puu,Quu}.hpp
+void orc_fn_ouu(const double* C, const double* x0pupSS, const double* Mt, const double* Nt, double* out) {
+    put(CalculateOuu(m6(C), m3(x0pupSS), m3(Mt), m3(Nt)), out);
+}
+void orc_fn_puu(const double* C, const double* x0pupSS, const double* Nt, double* out) {
+    put(CalculatePuu(m6(C), m3(x0pupSS), m3(Nt)), out);
+}
+void orc_fn_quu(const double* C, const double* x0pupSS, const double* Nt, double* out) {
+    put(CalculateQuu(m6(C), m3(x0pupSS), m3(Nt)), out);
+}
+// beams/calculate_force_FC.hpp, calculate_force_FD.hpp, calculate_temporary_variables.hpp
+void orc_fn_fc(const double* C, const double* strain, double* out) {
+    put(CalculateForceFC(m6(C), {strain[0], strain[1], strain[2], strain[3], strain[4], strain[5]}), out);
 }
 void orc_fn_fd(const double* x0pupSS, const double* FC, double* out) {
-    Mat3 x{};
-    std::copy(x0pupSS, x0pupSS + 9, x.begin());
-    const auto r = CalculateForceFD(x, {FC[0], FC[1], FC[2], FC[3], FC[4], FC[5]});
-    std::copy(r.begin(), r.end(), out);
+    put(CalculateForceFD(m3(x0pupSS), {FC[0], FC[1], FC[2], FC[3], FC[4], FC[5]}), out);
 }
-// inertial pieces: in = m, u_ddot[3], omega[3], omega_dot[3], eta[3], rho[9]
-void orc_fn_inertial(const double* in, double* FI, double* G, double* K) {
-    const double m = in[0];
-    const Vec3 ud{in[1], in[2], in[3]}, w{in[4], in[5], in[6]}, wd{in[7], in[8], in[9]},
-        eta{in[10], in[11], in[12]};
-    Mat3 rho{};
-    std::copy(in + 13, in + 22, rho.begin());
-    const Mat3 et = VecTilde(eta), wt = VecTilde(w), wdt = VecTilde(wd);
-    const auto f = CalculateInertialForce(m, ud, w, wd, eta, et, rho, wt, wdt);
-    const auto g = CalculateGyroscopicMatrix(m, w, eta, rho, wt);
-    const auto k = CalculateInertiaStiffnessMatrix(m, ud, w, wd, eta, rho, wt, wdt);
-    std::copy(f.begin(), f.end(), FI);
-    std::copy(g.begin(), g.end(), G);
-    std::copy(k.begin(), k.end(), K);
+void orc_fn_temporary_variables(const double* x0p, const double* up, double* out) {
+    put(CalculateTemporaryVariables(v3(x0p), v3(up)), out);
 }
-void orc_fn_gravity(double m, const double* g, const double* eta, double* out) {
-    const auto r = CalculateGravityForce(m, {g[0], g[1], g[2]}, VecTilde({eta[0], eta[1], eta[2]}));
-    std::copy(r.begin(), r.end(), out);
+// masses/calculate_mass_matrix_components.hpp, calculate_{inertial,gravity}_force.hpp,
+// calculate_gyroscopic_matrix.hpp, calculate_inertia_stiffness_matrix.hpp: the tilde matrices
+// are inputs of their own, as in the reference functors
+void orc_fn_eta_rho(const double* Muu, double* eta, double* rho) {
+    put(CalculateEta(m6(Muu)), eta);
+    put(CalculateRho(m6(Muu)), rho);
 }
+void orc_fn_inertial_force(double m, const double* ud, const double* w, const double* wd, const double* eta,
+                           const double* eta_tilde, const double* rho, const double* w_tilde, const double* wd_tilde,
+                           double* out) {
+    put(CalculateInertialForce(m, v3(ud), v3(w), v3(wd), v3(eta), m3(eta_tilde), m3(rho), m3(w_tilde), m3(wd_tilde)),
+        out);
+}
+void orc_fn_gravity_force(double m, const double* g, const double* eta_tilde, double* out) {
+    put(CalculateGravityForce(m, v3(g), m3(eta_tilde)), out);
+}
+void orc_fn_gyroscopic_matrix(double m, const double* w, const double* eta, const double* rho, const double* w_tilde,
+                              double* out) {
+    put(CalculateGyroscopicMatrix(m, v3(w), v3(eta), m3(rho), m3(w_tilde)), out);
+}
+void orc_fn_inertia_stiffness_matrix(double m, const double* ud, const double* w, const double* wd, const double* eta,
+                                     const double* rho, const double* w_tilde, const double* wd_tilde, double* out) {
+    put(CalculateInertiaStiffnessMatrix(m, v3(ud), v3(w), v3(wd), v3(eta), m3(rho), m3(w_tilde), m3(wd_tilde)), out);
+}
+// beams/integrate_{residual_vector,stiffness_matrix,inertia_matrix}.hpp: one element, N nodes,
+// Q points; shape tables [N][Q], QP matrices [Q][36], QP forces [Q][6]
+void orc_fn_integrate_residual(size_t N, size_t Q, const double* w, const double* J, const double* phi,
+                               const double* dphi, const double* Fc, const double* Fd, const double* Fi,
+                               const double* Fe, const double* Fg, const double* FX, double* out) {
+    IntegrateResidualVector(N, Q, w, J, phi, dphi, Fc, Fd, Fi, Fe, Fg, FX, out);
+}
+void orc_fn_integrate_stiffness(size_t N, size_t Q, const double* w, const double* J, const double* phi,
+                                const double* dphi, const double* Kuu, const double* Puu, const double* Cuu,
+                                const double* Ouu, const double* Quu, double* out) {
+    std::vector<Mat6> K(Q), P(Q), C(Q), O(Q), Qm(Q);
+    for (size_t q = 0; q < Q; ++q) {
+        K[q] = m6(Kuu + 36 * q);
+        P[q] = m6(Puu + 36 * q);
+        C[q] = m6(Cuu + 36 * q);
+        O[q] = m6(Ouu + 36 * q);
+        Qm[q] = m6(Quu + 36 * q);
+    }
+    IntegrateStiffnessMatrix(N, Q, w, J, phi, dphi, K.data(), P.data(), C.data(), O.data(), Qm.data(), out);
+}
+void orc_fn_integrate_inertia(size_t N, size_t Q, const double* w, const double* J, const double* phi,
+                              double beta_prime, double gamma_prime, const double* Muu, const double* Guu,
+                              double* out) {
+    std::vector<Mat6> M(Q), G(Q);
+    for (size_t q = 0; q < Q; ++q) {
+        M[q] = m6(Muu + 36 * q);
+        G[q] = m6(Guu + 36 * q);
+    }
+    IntegrateInertiaMatrix(N, Q, w, J, phi, beta_prime, gamma_prime, M.data(), G.data(), out);
+}
+// state/*.hpp, one node
+void orc_fn_next_state(double h, double alpha_f, double alpha_m, double beta, double gamma, double* qd, double* v,
+                       double* vd, double* a) {
+    CalculateNextStateNode(h, alpha_f, alpha_m, beta, gamma, qd, v, vd, a);
+}
+void orc_fn_displacement(double h, const double* qd, const double* q_prev, double* q) {
+    CalculateDisplacementNode(h, qd, q_prev, q);
+}
+void orc_fn_dynamic_prediction(double h, double beta_prime, double gamma_prime, const double* x, double* qd, double* v,
+                               double* vd) {
+    UpdateDynamicPredictionNode(h, beta_prime, gamma_prime, x, 6, qd, v, vd);
+}
+void orc_fn_static_prediction(double h, const double* x, double* qd) { UpdateStaticPredictionNode(h, x, 6, qd); }
+void orc_fn_algorithmic_acceleration(double alpha_f, double alpha_m, const double* vd, double* a) {
+    UpdateAlgorithmicAccelerationNode(alpha_f, alpha_m, vd, a);
+}
+void orc_fn_global_position(const double* q, const double* x0, double* x) { UpdateGlobalPositionNode(q, x0, x); }
 void orc_fn_tangent_operator(double h, const double* qd, double* out) {
     const auto t = CalculateTangentOperatorT(h, qd);
     std::copy(t.begin(), t.end(), out);

@@ -12,7 +12,14 @@ Sources (paths relative to the reference root):
   tests/regression_tests/regression/test_rotor.cpp
   tests/regression_tests/regression/iea15_rotor_data.hpp    -> iea15_rotor.json
   tests/unit_tests/system/beams/test_calculate_strain.cpp   -> functor_known_answers.json
-  tests/unit_tests/system/masses/*.cpp                      -> functor_known_answers.json
+  tests/unit_tests/system/beams/test_calculate_{Ouu,Puu,Quu,force_FC,force_FD,
+    temporary_variables}.cpp,
+  tests/unit_tests/system/masses/*.cpp,
+  tests/unit_tests/state/test_{calculate_next_state,calculate_displacement,
+    update_dynamic_prediction,update_static_prediction,update_algorithmic_acceleration,
+    update_global_position}.cpp                             -> functor_known_answers.json
+  tests/unit_tests/system/beams/test_integrate_{stiffness_matrix,inertia_matrix,
+    residual_vector}.cpp                                    -> integrate_known_answers.json
   tests/regression_tests/regression/test_masses.cpp,
   tests/regression_tests/regression/test_precession.cpp,
   tests/regression_tests/regression/test_spring_mass_system.cpp,
@@ -37,6 +44,35 @@ HERE = Path(__file__).resolve().parent
 
 def numbers(text):
     return [float(t) for t in NUM.findall(text)]
+
+
+def arith(expr):
+    """Value of a numeric literal expression (+ - * / and parentheses only, e.g. `1. / 3.`)."""
+    import ast
+
+    def ev(n):
+        if isinstance(n, ast.Expression):
+            return ev(n.body)
+        if isinstance(n, ast.Constant) and isinstance(n.value, (int, float)):
+            return float(n.value)
+        if isinstance(n, ast.UnaryOp) and isinstance(n.op, (ast.USub, ast.UAdd)):
+            return -ev(n.operand) if isinstance(n.op, ast.USub) else ev(n.operand)
+        if isinstance(n, ast.BinOp) and isinstance(n.op, (ast.Add, ast.Sub, ast.Mult, ast.Div)):
+            a, b = ev(n.left), ev(n.right)
+            return {ast.Add: a + b, ast.Sub: a - b, ast.Mult: a * b, ast.Div: a / b}[type(n.op)]
+        raise ValueError("not a numeric literal expression: " + expr)
+
+    e = re.sub(r"(\d)\.(?![\d])", r"\1.0", expr.strip())  # C++ `1.` -> `1.0`
+    e = re.sub(r"(?<=\d)[uUlLfF]+\b", "", e)
+    return ev(ast.parse(e, mode="eval"))
+
+
+def values(braced):
+    """Element values of a `{a, b, ...}` initializer whose elements may be literal expressions."""
+    inner = braced.strip()
+    if inner.startswith("{"):
+        inner = balanced(inner, 0)
+    return [arith(t) for t in split_top(inner) if t.strip()]
 
 
 def balanced(text, start):
@@ -194,18 +230,157 @@ def iea15(root):
     }
 
 
+FUNCTOR_TESTS = {
+    # key: reference unit test (relative path) -- one functor call on named inputs each
+    "strain": "tests/unit_tests/system/beams/test_calculate_strain.cpp",
+    "Ouu": "tests/unit_tests/system/beams/test_calculate_Ouu.cpp",
+    "Puu": "tests/unit_tests/system/beams/test_calculate_Puu.cpp",
+    "Quu": "tests/unit_tests/system/beams/test_calculate_Quu.cpp",
+    "force_FC": "tests/unit_tests/system/beams/test_calculate_force_FC.cpp",
+    "force_FD": "tests/unit_tests/system/beams/test_calculate_force_FD.cpp",
+    "temporary_variables": "tests/unit_tests/system/beams/test_calculate_temporary_variables.cpp",
+    "gravity_force": "tests/unit_tests/system/masses/test_calculate_gravity_force.cpp",
+    "gyroscopic_matrix": "tests/unit_tests/system/masses/test_calculate_gyroscopic_matrix.cpp",
+    "inertia_stiffness_matrix": "tests/unit_tests/system/masses/test_calculate_inertia_stiffness_matrix.cpp",
+    "inertial_forces": "tests/unit_tests/system/masses/test_calculate_inertial_forces.cpp",
+    "mass_matrix_components": "tests/unit_tests/system/masses/test_calculate_mass_matrix_components.cpp",
+    "rotate_section_matrix": "tests/unit_tests/system/masses/test_rotate_section_matrix.cpp",
+    "next_state": "tests/unit_tests/state/test_calculate_next_state.cpp",
+    "displacement": "tests/unit_tests/state/test_calculate_displacement.cpp",
+    "dynamic_prediction": "tests/unit_tests/state/test_update_dynamic_prediction.cpp",
+    "static_prediction": "tests/unit_tests/state/test_update_static_prediction.cpp",
+    "algorithmic_acceleration": "tests/unit_tests/state/test_update_algorithmic_acceleration.cpp",
+    "global_position": "tests/unit_tests/state/test_update_global_position.cpp",
+}
+
+
+def functor_case(text):
+    """Named inputs (CreateView<...>("name", std::array{...})), scalar constants
+    (`auto|double name = value;`) and expected outputs (`name_exact_data = std::array{...}`)."""
+    inputs = {}
+    for m in re.finditer(r'Create(?:Mutable|Left)?View<[^>]*>\(\s*"(\w+)",\s*std::array(?:<[^>]*>)?\s*\{', text):
+        try:
+            inputs[m.group(1)] = values(balanced(text, m.end() - 1))
+        except (ValueError, SyntaxError):  # a non-numeric table (e.g. freedom signatures)
+            continue
+    scalars = {m.group(1): float(m.group(2))
+               for m in re.finditer(r"(?:constexpr auto|const double|constexpr double)\s+(\w+)\s*=\s*(-?[\d.eE+-]+)\s*;", text)}
+    expected = {}
+    for m in re.finditer(r"(\w+)_exact_data\s*=\s*std::array(?:<[^>]*>)?\s*\{", text):
+        expected[m.group(1)] = values(balanced(text, m.end() - 1))
+    return {"inputs": inputs, "scalars": scalars, "expected": expected}
+
+
 def functor_known_answers(root):
     out = {}
-    t = strip_comments((root / "tests/unit_tests/system/beams/test_calculate_strain.cpp").read_text())
-    out["strain"] = {
-        "x0_prime": numbers(re.search(r'"x0_prime", std::array\{([^}]*)\}', t).group(1)),
-        "u_prime": numbers(re.search(r'"u_prime", std::array\{([^}]*)\}', t).group(1)),
-        "r": numbers(re.search(r'"r", std::array\{([^}]*)\}', t).group(1)),
-        "r_prime": numbers(re.search(r'"r_prime", std::array\{([^}]*)\}', t).group(1)),
-        "expected": numbers(re.search(r"strain_exact_data = std::array\{([^}]*)\}", t).group(1)),
-        "source": "tests/unit_tests/system/beams/test_calculate_strain.cpp",
-    }
+    for key, rel in FUNCTOR_TESTS.items():
+        case = functor_case(strip_comments((root / rel).read_text()))
+        case["source"] = rel
+        out[key] = case
     return out
+
+
+INTEGRATE_TESTS = {
+    "integrate_stiffness": "tests/unit_tests/system/beams/test_integrate_stiffness_matrix.cpp",
+    "integrate_inertia": "tests/unit_tests/system/beams/test_integrate_inertia_matrix.cpp",
+    "integrate_residual": "tests/unit_tests/system/beams/test_integrate_residual_vector.cpp",
+}
+
+
+def statements(body):
+    """Top-level `const auto NAME = ...;` and `constexpr auto NAME = ...;` statements of a body."""
+    out = {}
+    for m in re.finditer(r"(?:const|constexpr)\s+auto\s+(\w+)\s*=", body):
+        depth, i = 0, m.end()
+        while i < len(body):
+            ch = body[i]
+            if ch in "({[":
+                depth += 1
+            elif ch in ")}]":
+                depth -= 1
+            elif ch == ";" and depth == 0:
+                break
+            i += 1
+        out[m.group(1)] = body[m.end():i].strip()
+    return out
+
+
+def view_data(expr, src):
+    """Numbers a CreateView / CreateLeftView statement fills its view with (None: zero view)."""
+    if not re.match(r"Create(?:Left|Mutable)?View", expr):
+        return None
+    args = split_top(balanced(expr, expr.index("(")))
+    data = ",".join(args[1:]).strip()  # template argument lists hold commas too
+    call = re.fullmatch(r"(\w+)\(\)", data)
+    if call:  # a constexpr data function
+        m = re.search(r"\b" + call.group(1) + r"\(\)\s*\{", src)
+        body = balanced(src, m.end() - 1).split("return", 1)[1]
+        return values(body[body.index("{"):body.rindex("}") + 1])
+    return values(data[data.index("{"):])
+
+
+def scalar_value(expr):
+    m = re.fullmatch(r"size_t\{(\d+)U?\}", expr)
+    if m:
+        return float(m.group(1))
+    m = re.fullmatch(r"-?[\d.]+(?:[eE][-+]?\d+)?", expr)
+    return float(expr) if m else None
+
+
+def integrate_cases(src, rel):
+    """One entry per TEST: the integrator functor, its positional arguments by name, the arrays
+    and scalars those names hold (a helper's parameters share the caller's names) and the
+    expected output."""
+    cases = {}
+    for m in re.finditer(r"TEST\((\w+),\s*(\w+)\)\s*\{", src):
+        body = balanced(src, m.end() - 1)
+        call = re.fullmatch(r"\s*(\w+)\(\);\s*", body)
+        if call:
+            body = function_body(src, call.group(1))
+        helper_tol = None
+        helper = re.search(r"\b(TestIntegrate\w+)\(\s*qp_\w+", body)
+        if helper:
+            hm = re.search(r"void\s+" + helper.group(1) + r"\s*\(", src)
+            params = balanced(src, hm.end() - 1)
+            t = re.search(r"tolerance\s*=\s*([\d.eE+-]+)", params)
+            helper_tol = float(t.group(1)) if t else None
+            call_args = split_top(balanced(body, helper.end() - len(helper.group(0)) + len(helper.group(1))))
+            if len(call_args) > 6:
+                helper_tol = float(call_args[6])
+            hbody = balanced(src, src.index("{", hm.end() + len(params)))
+            body = body + "\n" + hbody
+        st = statements(body)
+        calls = list(re.finditer(r"beams::Integrate(\w+)Element<[^>]*>\s*\{", body))
+        fm = calls[0]
+        # one functor call per element (element index first); the expected output holds the
+        # elements' results in that order
+        args = [[a.strip() for a in split_top(balanced(body, c.end() - 1))] for c in calls]
+        arrays, scalars = {}, {}
+        for name, expr in st.items():
+            if name.endswith("_data") or name.startswith("exact") or "exact" in name:
+                continue
+            sv = scalar_value(expr)
+            if sv is not None:
+                scalars[name] = sv
+            else:
+                d = view_data(expr, src)
+                if d is not None:
+                    arrays[name] = d
+        exact = [v for k, v in st.items() if re.search(r"exact\w*_data$", k)]
+        cases[m.group(2)] = {
+            "functor": "Integrate" + fm.group(1) + "Element",
+            "calls": args,
+            "arrays": arrays,
+            "scalars": scalars,
+            "expected": values(exact[0][exact[0].index("{"):]),
+            "tolerance": helper_tol if helper_tol is not None else 1e-15,
+            "source": rel,
+        }
+    return cases
+
+
+def integrate_known_answers(root):
+    return {key: integrate_cases(strip_comments((root / rel).read_text()), rel) for key, rel in INTEGRATE_TESTS.items()}
 
 
 def near_values(body, pattern):
@@ -461,6 +636,7 @@ def main():
     (HERE / "cantilever_sine.json").write_text(json.dumps(cantilever(root), indent=1))
     (HERE / "iea15_rotor.json").write_text(json.dumps(iea15(root)))
     (HERE / "functor_known_answers.json").write_text(json.dumps(functor_known_answers(root), indent=1))
+    (HERE / "integrate_known_answers.json").write_text(json.dumps(integrate_known_answers(root), indent=1))
     (HERE / "masses_springs.json").write_text(json.dumps(masses_springs(root), indent=1))
     print("wrote fixtures to", HERE)
 

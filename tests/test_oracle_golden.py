@@ -86,7 +86,7 @@ def test_strain_known_answer(lib):
 
     k = load("functor_known_answers.json")["strain"]
     dp = C.POINTER(C.c_double)
-    arrs = [np.array(k[n], dtype=np.float64) for n in ("x0_prime", "u_prime", "r", "r_prime")]
+    arrs = [np.array(k["inputs"][n], dtype=np.float64) for n in ("x0_prime", "u_prime", "r", "r_prime")]
     out = np.zeros(6)
     lib.L.orc_fn_strain(*[a.ctypes.data_as(dp) for a in arrs], out.ctypes.data_as(dp))
-    np.testing.assert_allclose(out, k["expected"], atol=1e-15)
+    np.testing.assert_allclose(out, k["expected"]["strain"], atol=1e-15)
