@@ -491,6 +491,19 @@ StepParameters::StepParameters(bool dyn, size_t mi, double h_, double rho, doubl
 // populate_element_views.hpp, calculate_jacobian.hpp, constraints/constraints.hpp,
 // dof_management/*.hpp, solver/solver.hpp
 // ---------------------------------------------------------------------------------------
+// src/elements/beams/calculate_jacobian.hpp:35-76: x0' = sum_k phi'_k x0_k, J = |x0'|, unit
+// tangent x0' / J per QP; node_x0 [N][7], shape derivatives [N][Q]
+void CalculateJacobian(size_t N, size_t Q, const double* node_x0, const double* dphi, double* J, double* x0p) {
+    for (size_t q = 0; q < Q; ++q) {
+        std::array<double, 3> d{};
+        for (size_t k = 0; k < N; ++k)
+            for (int c = 0; c < 3; ++c) d[c] += node_x0[k * 7 + c] * dphi[k * Q + q];
+        const double j = std::sqrt(std::pow(d[0], 2.) + std::pow(d[1], 2.) + std::pow(d[2], 2.));
+        J[q] = j;
+        for (int c = 0; c < 3; ++c) x0p[q * 3 + c] = d[c] / j;
+    }
+}
+
 // src/state/update_global_position.hpp:24-45
 void UpdateGlobalPositionNode(const double* q, const double* x0, double* x) {
     x[0] = x0[0] + q[0];
@@ -579,17 +592,17 @@ System CreateSystem(const ModelIn& model) {
         e.qp_x0_prime.assign(Q * 3, 0.);
         e.qp_r0.assign(Q * 4, 0.);
         e.jacobian.assign(Q, 0.);
+        std::vector<double> node_x0(N * 7);
+        for (size_t k = 0; k < N; ++k)
+            std::copy(&s.x0[bi.nodes[k] * 7], &s.x0[bi.nodes[k] * 7] + 7, node_x0.begin() + static_cast<long>(k * 7));
+        CalculateJacobian(N, Q, node_x0.data(), e.shape_deriv.data(), e.jacobian.data(), e.qp_x0_prime.data());
         for (size_t q = 0; q < Q; ++q) {
-            std::array<double, 3> p{}, d{};
+            std::array<double, 3> p{};
             std::array<double, 4> r{};
             for (size_t k = 0; k < N; ++k) {
-                const double* x0 = &s.x0[bi.nodes[k] * 7];
+                const double* x0 = &node_x0[k * 7];
                 const double phi = e.shape_interp[k * Q + q];
-                const double dphi = e.shape_deriv[k * Q + q];
-                for (int c = 0; c < 3; ++c) {
-                    p[c] += x0[c] * phi;
-                    d[c] += x0[c] * dphi;
-                }
+                for (int c = 0; c < 3; ++c) p[c] += x0[c] * phi;
                 for (int c = 0; c < 4; ++c) r[c] += x0[3 + c] * phi;
             }
             const double len = std::sqrt(std::pow(r[0], 2) + std::pow(r[1], 2) +
@@ -597,12 +610,7 @@ System CreateSystem(const ModelIn& model) {
             if (len == 0.) r = {1., 0., 0., 0.};
             else
                 for (int c = 0; c < 4; ++c) r[c] /= len;
-            const double J = std::sqrt(std::pow(d[0], 2.) + std::pow(d[1], 2.) + std::pow(d[2], 2.));
-            e.jacobian[q] = J;
-            for (int c = 0; c < 3; ++c) {
-                e.qp_x0[q * 3 + c] = p[c];
-                e.qp_x0_prime[q * 3 + c] = d[c] / J;
-            }
+            for (int c = 0; c < 3; ++c) e.qp_x0[q * 3 + c] = p[c];
             for (int c = 0; c < 4; ++c) e.qp_r0[q * 4 + c] = r[c];
         }
         e.qp_Fe.assign(Q * 6, 0.);

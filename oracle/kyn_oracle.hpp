@@ -95,6 +95,7 @@ void UpdateDynamicPredictionNode(double h, double beta_prime, double gamma_prime
 void UpdateStaticPredictionNode(double h, const double* x, size_t nd, double* qd);
 void UpdateAlgorithmicAccelerationNode(double alpha_f, double alpha_m, const double* vd, double* a);
 void UpdateGlobalPositionNode(const double* q, const double* x0, double* x);
+void CalculateJacobian(size_t N, size_t Q, const double* node_x0, const double* dphi, double* J, double* x0p);
 
 // ---------------------------------------------------------------------------------------
 // Model inputs (src/model/model.hpp, node.hpp; src/elements/beams/beam_element.hpp)

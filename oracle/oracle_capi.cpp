@@ -392,6 +392,9 @@ void orc_fn_static_prediction(double h, const double* x, double* qd) { UpdateSta
 void orc_fn_algorithmic_acceleration(double alpha_f, double alpha_m, const double* vd, double* a) {
     UpdateAlgorithmicAccelerationNode(alpha_f, alpha_m, vd, a);
 }
+void orc_fn_jacobian(size_t N, size_t Q, const double* node_x0, const double* dphi, double* J, double* x0p) {
+    CalculateJacobian(N, Q, node_x0, dphi, J, x0p);
+}
 void orc_fn_global_position(const double* q, const double* x0, double* x) { UpdateGlobalPositionNode(q, x0, x); }
 void orc_fn_tangent_operator(double h, const double* qd, double* out) {
     const auto t = CalculateTangentOperatorT(h, qd);

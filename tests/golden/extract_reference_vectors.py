@@ -20,6 +20,7 @@ Sources (paths relative to the reference root):
     update_global_position}.cpp                             -> functor_known_answers.json
   tests/unit_tests/system/beams/test_integrate_{stiffness_matrix,inertia_matrix,
     residual_vector}.cpp                                    -> integrate_known_answers.json
+  tests/unit_tests/system/beams/test_curved_beam.hpp         -> curved_beam.json
   tests/regression_tests/regression/test_masses.cpp,
   tests/regression_tests/regression/test_precession.cpp,
   tests/regression_tests/regression/test_spring_mass_system.cpp,
@@ -46,11 +47,14 @@ def numbers(text):
     return [float(t) for t in NUM.findall(text)]
 
 
-def arith(expr):
-    """Value of a numeric literal expression (+ - * / and parentheses only, e.g. `1. / 3.`)."""
+def arith(expr, env=None):
+    """Value of a numeric literal expression (+ - * / and parentheses, e.g. `1. / 3.`; `NAME[i]`
+    for an element of an array parsed earlier, given in `env`)."""
     import ast
 
     def ev(n):
+        if isinstance(n, ast.Subscript) and env is not None and isinstance(n.value, ast.Name) and n.value.id in env:
+            return float(env[n.value.id][int(ev(n.slice))])
         if isinstance(n, ast.Expression):
             return ev(n.body)
         if isinstance(n, ast.Constant) and isinstance(n.value, (int, float)):
@@ -383,6 +387,30 @@ def integrate_known_answers(root):
     return {key: integrate_cases(strip_comments((root / rel).read_text()), rel) for key, rel in INTEGRATE_TESTS.items()}
 
 
+def flat_values(braced, env=None):
+    """Flattened element values of a possibly nested `{{...}, {...}}` initializer."""
+    inner = balanced(braced.strip(), 0)
+    out = []
+    for t in split_top(inner):
+        t = t.strip()
+        if not t:
+            continue
+        out.extend(flat_values(t, env) if t.startswith("{") else [arith(t, env)])
+    return out
+
+
+def curved_beam(root):
+    """Every constexpr data array of the curved-beam unit tests (a 3-node, 7-point element whose
+    expected values come from the reference's Mathematica notebooks), by name."""
+    rel = "tests/unit_tests/system/beams/test_curved_beam.hpp"
+    text = strip_comments((root / rel).read_text())
+    out = {"source": rel}
+    for m in re.finditer(r"constexpr\s+(?:std::array<[^=]*>|auto)\s+(\w+)\s*=\s*(?:std::array(?:<[^>]*>)?)?\s*\{", text):
+        out[m.group(1)] = flat_values(text[m.end() - 1:], out)
+    out["kGLLNodes"] = values(re.search(r"kGLLNodes(\{[^}]*\})", text).group(1))
+    return out
+
+
 def near_values(body, pattern):
     """EXPECT_NEAR(<expr matching pattern>, value, tol) -> [(expr, value, tol)], in order."""
     out = []
@@ -637,6 +665,7 @@ def main():
     (HERE / "iea15_rotor.json").write_text(json.dumps(iea15(root)))
     (HERE / "functor_known_answers.json").write_text(json.dumps(functor_known_answers(root), indent=1))
     (HERE / "integrate_known_answers.json").write_text(json.dumps(integrate_known_answers(root), indent=1))
+    (HERE / "curved_beam.json").write_text(json.dumps(curved_beam(root), indent=1))
     (HERE / "masses_springs.json").write_text(json.dumps(masses_springs(root), indent=1))
     print("wrote fixtures to", HERE)
 

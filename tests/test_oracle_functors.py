@@ -256,3 +256,59 @@ def test_dense_lu_against_numpy(lib):
         rc = f(C.c_size_t(n), ptr(Ac), ptr(bc))
         assert rc == 0
         np.testing.assert_allclose(bc, np.linalg.solve(A, b), rtol=1e-10, atol=1e-12)
+
+
+# ---- curved beam (reference unit tests test_curved_beam.cpp with test_curved_beam.hpp: a 3-node
+#      second-order element on GLL nodes, 7-point Gauss rule, values from the reference's
+#      Mathematica notebooks), at the reference's tolerances
+CB = load("curved_beam.json")
+CB_N, CB_Q = 3, 7
+
+
+def test_curved_beam_shape_functions_and_jacobian(lib):
+    f = lib.L.orc_fn_lagrange
+    f.restype = None
+    xs = arr(CB["kGLLNodes"])
+    phi, dphi = np.zeros((CB_Q, CB_N)), np.zeros((CB_Q, CB_N))
+    for q, x in enumerate(CB["kGaussQuadraturePoints"]):
+        w, dw = np.zeros(CB_N), np.zeros(CB_N)
+        f(C.c_double(x), C.c_size_t(CB_N), ptr(xs), ptr(w), ptr(dw))
+        phi[q], dphi[q] = w, dw
+    check(phi.ravel(), CB["kExpectedInterpWeights"], 1e-12)
+    check(dphi.ravel(), CB["kExpectedDerivWeights"], 1e-12)
+    J, x0p = np.zeros(CB_Q), np.zeros(CB_Q * 3)
+    call(lib, "jacobian", CB_N, CB_Q, arr(CB["kCurvedBeamNodes_data"]), arr(CB["kDerivWeightsFlat"][: CB_N * CB_Q]), J, x0p)
+    check(J, CB["kExpectedJacobians"], 1e-12)
+    np.testing.assert_allclose(np.linalg.norm(x0p.reshape(CB_Q, 3), axis=1), 1.0, atol=1e-12)
+
+
+def test_curved_beam_section_terms(lib):
+    out6, out36 = np.zeros(6), np.zeros(36)
+    call(lib, "fc", arr(CB["kCurvedBeamCuu_data"]), arr(CB["kCurvedBeamStrain_data"]), out6)
+    check(out6, CB["kExpectedFc_data"], 1e-6)
+    call(lib, "fd", arr(CB["kX0pupSS_data"]), arr(CB["kExpectedFc_data"]), out6)
+    check(out6, CB["kExpectedFd_data"], 1e-6)
+    call(lib, "rotate_section", arr(CB["xr_data"]), arr(CB["Cstar_data"]), out36)
+    check(out36, CB["kExpectedCuu_data"], 1e-6)
+    Cuu, xs = arr(CB["kCurvedBeamCuu_data"]), arr(CB["kX0pupSS_data"])
+    Mt, Nt = arr(CB["kCurvedBeamM_tilde_data"]), arr(CB["kCurvedBeamN_tilde_data"])
+    call(lib, "ouu", Cuu, xs, Mt, Nt, out36)
+    check(out36, CB["kExpectedOuu_data"], 1e-6)
+    call(lib, "puu", Cuu, xs, Nt, out36)
+    check(out36, CB["kExpectedPuu_data"], 1e-6)
+    call(lib, "quu", Cuu, xs, Nt, out36)
+    check(out36, CB["kExpectedQuu_data"], 1e-6)
+
+
+def test_curved_beam_residual_and_stiffness_integration(lib):
+    w, J = arr(CB["kGaussQuadratureWeights"]), arr(CB["kExpectedJacobians"])
+    phi, dphi = arr(CB["kInterpWeightsFlat"][: CB_N * CB_Q]), arr(CB["kDerivWeightsFlat"][: CB_N * CB_Q])
+    zero6 = np.zeros(CB_Q * 6)
+    res = np.zeros(CB_N * 6)
+    call(lib, "integrate_residual", CB_N, CB_Q, w, J, phi, dphi, arr(CB["kFc"]), arr(CB["kFd"]), arr(CB["kFi"]),
+         zero6, zero6, np.zeros(CB_N * 6), res)
+    check(res, CB["kExpectedResidualVector"], 1e-8)
+    K = np.zeros(CB_N * CB_N * 36)
+    call(lib, "integrate_stiffness", CB_N, CB_Q, w, J, phi, dphi, arr(CB["kKuu"]), arr(CB["kPuu"]), arr(CB["kCuu"]),
+         arr(CB["kOuu"]), arr(CB["kQuu"]), K)
+    check(K, CB["kExpectedStiffnessMatrix_data"], 1e-7)
