@@ -780,6 +780,120 @@ System CreateSystem(const ModelIn& model) {
 // ---------------------------------------------------------------------------------------
 // Step pieces
 // ---------------------------------------------------------------------------------------
+// ---------------------------------------------------------------------------------------
+// CRS graph (src/solver/create_full_matrix.hpp:19-74, compute_row_ptrs.hpp:15-71,
+// compute_col_inds.hpp:14-61, compute_system_row_entries.hpp:74-102,
+// compute_system_col_inds.hpp:90-131, compute_constraints_row_entries.hpp:25-35,
+// compute_constraints_col_inds.hpp:49-59, scan_row_entries.hpp)
+// ---------------------------------------------------------------------------------------
+namespace {
+bool ElementContainsNode(const CrsGraphIn& g, size_t e, size_t node) {
+    for (size_t k : g.element_nodes[e])
+        if (k == node) return true;
+    return false;
+}
+bool ConstraintContainsNode(const CrsGraphIn& g, size_t c, size_t dof_index) {
+    return (dof_index == g.base_node_freedom_table[c][0] && g.base_active_dofs[c] != 0) ||
+           (dof_index == g.target_node_freedom_table[c][0] && g.target_active_dofs[c] != 0);
+}
+}  // namespace
+
+std::vector<size_t> ComputeRowPtrs(const CrsGraphIn& g) {
+    std::vector<size_t> entries(g.num_dofs, 0);
+    for (size_t node = 0; node < g.active_dofs.size(); ++node) {
+        const size_t nd = g.active_dofs[node], dof = g.node_freedom_map_table[node];
+        size_t row = nd;
+        for (size_t e = 0; e < g.element_nodes.size(); ++e) {
+            if (!ElementContainsNode(g, e, node)) continue;
+            size_t in_elem = 0;
+            for (size_t k : g.element_nodes[e]) in_elem += g.active_dofs[k];
+            row += in_elem - nd;
+        }
+        for (size_t c = 0; c < g.row_range.size(); ++c)
+            if (ConstraintContainsNode(g, c, dof)) row += g.row_range[c].second - g.row_range[c].first;
+        for (size_t k = 0; k < nd; ++k) entries[dof + k] = row;
+    }
+    for (size_t c = 0; c < g.row_range.size(); ++c)
+        for (size_t r = g.row_range[c].first; r < g.row_range[c].second; ++r)
+            entries[g.num_system_dofs + r] = g.base_active_dofs[c] + g.target_active_dofs[c];
+    std::vector<size_t> row_ptrs(g.num_dofs + 1, 0);
+    for (size_t i = 0; i < g.num_dofs; ++i) row_ptrs[i + 1] = row_ptrs[i] + entries[i];
+    return row_ptrs;
+}
+
+std::vector<size_t> ComputeColInds(const CrsGraphIn& g, const std::vector<size_t>& row_ptrs) {
+    std::vector<size_t> col(row_ptrs.back(), 0);
+    for (size_t node = 0; node < g.active_dofs.size(); ++node) {
+        const size_t nd = g.active_dofs[node], dof = g.node_freedom_map_table[node];
+        std::array<size_t, 6> cur{};
+        for (size_t c1 = 0; c1 < nd; ++c1) {
+            size_t idx = row_ptrs[dof + c1];
+            for (size_t c2 = 0; c2 < nd; ++c2) col[idx++] = dof + c2;
+            cur[c1] = nd;
+        }
+        for (size_t e = 0; e < g.element_nodes.size(); ++e) {
+            if (!ElementContainsNode(g, e, node)) continue;
+            for (size_t c = 0; c < nd; ++c) {
+                size_t idx = row_ptrs[dof + c] + cur[c];
+                for (size_t k : g.element_nodes[e]) {
+                    if (k == node) continue;
+                    for (size_t m = 0; m < g.active_dofs[k]; ++m) col[idx++] = g.node_freedom_map_table[k] + m;
+                }
+                cur[c] = idx - row_ptrs[dof + c];
+            }
+        }
+        for (size_t cn = 0; cn < g.row_range.size(); ++cn) {
+            if (!ConstraintContainsNode(g, cn, dof)) continue;
+            for (size_t c = 0; c < nd; ++c) {
+                size_t idx = row_ptrs[dof + c] + cur[c];
+                for (size_t r = g.row_range[cn].first; r < g.row_range[cn].second; ++r) col[idx++] = g.num_system_dofs + r;
+                cur[c] = idx - row_ptrs[dof + c];
+            }
+        }
+    }
+    for (size_t cn = 0; cn < g.row_range.size(); ++cn)
+        for (size_t r = g.row_range[cn].first; r < g.row_range[cn].second; ++r) {
+            size_t idx = row_ptrs[r + g.num_system_dofs];
+            for (size_t k = 0; k < g.target_active_dofs[cn]; ++k) col[idx++] = g.target_node_freedom_table[cn][k];
+            for (size_t k = 0; k < g.base_active_dofs[cn]; ++k) col[idx++] = g.base_node_freedom_table[cn][k];
+        }
+    return col;
+}
+
+// KokkosSparse::sort_crs_graph (create_full_matrix.hpp:62): each row's columns ascending
+void SortCrsGraph(const std::vector<size_t>& row_ptrs, std::vector<size_t>& col_inds) {
+    for (size_t r = 0; r + 1 < row_ptrs.size(); ++r)
+        std::sort(col_inds.begin() + static_cast<long>(row_ptrs[r]), col_inds.begin() + static_cast<long>(row_ptrs[r + 1]));
+}
+
+// The graph inputs of a system: Elements' node_state_indices (beams, masses, springs), the
+// constraints' base / target freedom tables and row ranges (solver.hpp / constraints.hpp)
+CrsGraphIn CrsGraphInput(const System& s) {
+    CrsGraphIn g;
+    g.num_system_dofs = s.num_system_dofs;
+    g.num_dofs = s.num_dofs;
+    g.active_dofs = s.active_dofs;
+    g.node_freedom_map_table = s.dof_map;
+    for (const auto& e : s.beams) g.element_nodes.emplace_back(e.node_idx.begin(), e.node_idx.end());
+    for (const auto& m : s.masses) g.element_nodes.push_back({m.node});
+    for (const auto& sp : s.springs) g.element_nodes.push_back({sp.n1, sp.n2});
+    for (const auto& c : s.cons) {
+        const size_t nb = static_cast<size_t>(count_active_dofs(c.base_sig));
+        const size_t nt = static_cast<size_t>(count_active_dofs(c.target_sig));
+        g.base_active_dofs.push_back(nb);
+        g.target_active_dofs.push_back(nt);
+        std::array<size_t, 6> bt{}, tt{};
+        for (size_t k = 0; k < 6; ++k) {
+            bt[k] = c.base_dof + k;
+            tt[k] = c.target_dof + k;
+        }
+        g.base_node_freedom_table.push_back(bt);
+        g.target_node_freedom_table.push_back(tt);
+        g.row_range.emplace_back(c.row_first, c.row_first + c.row_count);
+    }
+    return g;
+}
+
 // src/state/calculate_displacement.hpp:25-50
 void CalculateDisplacementNode(double h, const double* qd, const double* qp, double* q) {
     for (int c = 0; c < 3; ++c) q[c] = qp[c] + h * qd[c];

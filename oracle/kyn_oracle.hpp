@@ -249,6 +249,22 @@ struct System {
 
 System CreateSystem(const ModelIn& model);
 
+// CRS graph of the full matrix (src/solver/create_full_matrix.hpp:19-74): the inputs the reference
+// passes to ComputeRowPtrs / ComputeColInds, per node, element (beams, then masses, then springs,
+// as in Elements) and constraint
+struct CrsGraphIn {
+    size_t num_system_dofs{}, num_dofs{};
+    std::vector<size_t> active_dofs, node_freedom_map_table;
+    std::vector<std::vector<size_t>> element_nodes;
+    std::vector<size_t> base_active_dofs, target_active_dofs;
+    std::vector<std::array<size_t, 6>> base_node_freedom_table, target_node_freedom_table;
+    std::vector<std::pair<size_t, size_t>> row_range;
+};
+CrsGraphIn CrsGraphInput(const System& s);
+std::vector<size_t> ComputeRowPtrs(const CrsGraphIn& g);
+std::vector<size_t> ComputeColInds(const CrsGraphIn& g, const std::vector<size_t>& row_ptrs);
+void SortCrsGraph(const std::vector<size_t>& row_ptrs, std::vector<size_t>& col_inds);
+
 // Step pieces (src/step/*.hpp)
 void PredictNextState(const StepParameters& p, System& s);
 void ResetConstraints(System& s);

@@ -395,6 +395,60 @@ void orc_fn_algorithmic_acceleration(double alpha_f, double alpha_m, const doubl
 void orc_fn_jacobian(size_t N, size_t Q, const double* node_x0, const double* dphi, double* J, double* x0p) {
     CalculateJacobian(N, Q, node_x0, dphi, J, x0p);
 }
+// solver/compute_row_ptrs.hpp, compute_col_inds.hpp, create_full_matrix.hpp: the CRS graph from the
+// reference's inputs (node_state_indices [n_elem][max_nodes], freedom tables [n_cons][6], row
+// ranges [n_cons][2]); row_ptrs_in (may be null) replaces the computed row pointers for the column
+// indices, as the reference's ComputeColInds takes them; sort != 0 sorts each row.  Returns nnz.
+size_t orc_fn_crs_graph(size_t num_system_dofs, size_t num_dofs, size_t n_nodes, const size_t* active,
+                        const size_t* nfmt, size_t n_elem, const size_t* nodes_per_elem, size_t max_nodes,
+                        const size_t* nsi, size_t n_cons, const size_t* base_active, const size_t* target_active,
+                        const size_t* base_nft, const size_t* target_nft, const size_t* row_range,
+                        const size_t* row_ptrs_in, int sort, size_t* row_ptrs_out, size_t* col_inds_out,
+                        size_t col_cap) {
+    CrsGraphIn g;
+    g.num_system_dofs = num_system_dofs;
+    g.num_dofs = num_dofs;
+    g.active_dofs.assign(active, active + n_nodes);
+    g.node_freedom_map_table.assign(nfmt, nfmt + n_nodes);
+    for (size_t e = 0; e < n_elem; ++e) g.element_nodes.emplace_back(nsi + e * max_nodes, nsi + e * max_nodes + nodes_per_elem[e]);
+    for (size_t c = 0; c < n_cons; ++c) {
+        g.base_active_dofs.push_back(base_active[c]);
+        g.target_active_dofs.push_back(target_active[c]);
+        std::array<size_t, 6> b{}, t{};
+        std::copy(base_nft + 6 * c, base_nft + 6 * c + 6, b.begin());
+        std::copy(target_nft + 6 * c, target_nft + 6 * c + 6, t.begin());
+        g.base_node_freedom_table.push_back(b);
+        g.target_node_freedom_table.push_back(t);
+        g.row_range.emplace_back(row_range[2 * c], row_range[2 * c + 1]);
+    }
+    auto rp = row_ptrs_in ? std::vector<size_t>(row_ptrs_in, row_ptrs_in + num_dofs + 1) : ComputeRowPtrs(g);
+    auto ci = ComputeColInds(g, rp);
+    if (sort) SortCrsGraph(rp, ci);
+    std::copy(rp.begin(), rp.end(), row_ptrs_out);
+    std::copy(ci.begin(), ci.begin() + static_cast<long>(std::min(col_cap, ci.size())), col_inds_out);
+    return ci.size();
+}
+// The assembled system matrix (before conditioning and solving: after orc_debug_assemble) in the
+// global dof order, dense n x n row-major, from the independent blocks
+void orc_get_matrix(void* h, double* out) {
+    const auto& s = static_cast<OrcSystem*>(h)->sys;
+    const size_t n = s.num_dofs;
+    std::fill(out, out + n * n, 0.);
+    for (const auto& blk : s.blocks)
+        for (size_t i = 0; i < blk.n; ++i)
+            for (size_t j = 0; j < blk.n; ++j) out[blk.gdof[i] * n + blk.gdof[j]] = blk.A[i * blk.n + j];
+}
+// The full matrix's sorted CRS graph of a system (CreateFullMatrix): returns nnz; outputs may be null
+size_t orc_crs_graph(void* h, size_t* row_ptrs, size_t* col_inds, size_t cap) {
+    const auto& s = static_cast<OrcSystem*>(h)->sys;
+    const auto g = CrsGraphInput(s);
+    auto rp = ComputeRowPtrs(g);
+    auto ci = ComputeColInds(g, rp);
+    SortCrsGraph(rp, ci);
+    if (row_ptrs) std::copy(rp.begin(), rp.end(), row_ptrs);
+    if (col_inds) std::copy(ci.begin(), ci.begin() + static_cast<long>(std::min(cap, ci.size())), col_inds);
+    return ci.size();
+}
 void orc_fn_global_position(const double* q, const double* x0, double* x) { UpdateGlobalPositionNode(q, x0, x); }
 void orc_fn_tangent_operator(double h, const double* qd, double* out) {
     const auto t = CalculateTangentOperatorT(h, qd);

@@ -21,6 +21,7 @@ Sources (paths relative to the reference root):
   tests/unit_tests/system/beams/test_integrate_{stiffness_matrix,inertia_matrix,
     residual_vector}.cpp                                    -> integrate_known_answers.json
   tests/unit_tests/system/beams/test_curved_beam.hpp         -> curved_beam.json
+  tests/unit_tests/solver/test_compute_{row_ptrs,col_inds}.cpp -> crs_known_answers.json
   tests/regression_tests/regression/test_masses.cpp,
   tests/regression_tests/regression/test_precession.cpp,
   tests/regression_tests/regression/test_spring_mass_system.cpp,
@@ -55,6 +56,11 @@ def arith(expr, env=None):
     def ev(n):
         if isinstance(n, ast.Subscript) and env is not None and isinstance(n.value, ast.Name) and n.value.id in env:
             return float(env[n.value.id][int(ev(n.slice))])
+        if isinstance(n, ast.Call) and env is not None and isinstance(n.func, ast.Name) and n.func.id in env \
+                and len(n.args) == 1:  # a view read `name(i)`
+            return float(env[n.func.id][int(ev(n.args[0]))])
+        if isinstance(n, ast.Name) and env is not None and n.id in env and not isinstance(env[n.id], list):
+            return float(env[n.id])
         if isinstance(n, ast.Expression):
             return ev(n.body)
         if isinstance(n, ast.Constant) and isinstance(n.value, (int, float)):
@@ -63,7 +69,11 @@ def arith(expr, env=None):
             return -ev(n.operand) if isinstance(n.op, ast.USub) else ev(n.operand)
         if isinstance(n, ast.BinOp) and isinstance(n.op, (ast.Add, ast.Sub, ast.Mult, ast.Div)):
             a, b = ev(n.left), ev(n.right)
-            return {ast.Add: a + b, ast.Sub: a - b, ast.Mult: a * b, ast.Div: a / b}[type(n.op)]
+            if isinstance(n.op, ast.Add):
+                return a + b
+            if isinstance(n.op, ast.Sub):
+                return a - b
+            return a * b if isinstance(n.op, ast.Mult) else a / b
         raise ValueError("not a numeric literal expression: " + expr)
 
     e = re.sub(r"(\d)\.(?![\d])", r"\1.0", expr.strip())  # C++ `1.` -> `1.0`
@@ -411,6 +421,81 @@ def curved_beam(root):
     return out
 
 
+CRS_TESTS = {
+    "row_ptrs": "tests/unit_tests/solver/test_compute_row_ptrs.cpp",
+    "col_inds": "tests/unit_tests/solver/test_compute_col_inds.cpp",
+}
+
+
+def expectation_loops(body, env):
+    """Evaluates the checks after the call under test: nested `for (auto v : std::views::iota(a,
+    b)) { ... }` loops, `auto v = expr;` locals and `EXPECT_EQ(name(index), value);` -> a list of
+    [name, index, value] (the loops run over literal ranges; expressions are arithmetic on the
+    test's constants, loop variables and input arrays)."""
+    checks = []
+
+    def run(text, env):
+        i = 0
+        while i < len(text):
+            m = re.compile(r"\s*for\s*\(\s*auto\s+(\w+)\s*:\s*std::views::iota\(").match(text, i)
+            if m:
+                rng = split_top(balanced(text, m.end() - 1))
+                lo, hi = int(arith(rng[0], env)), int(arith(rng[1], env))
+                j = text.index("{", m.end())
+                inner = balanced(text, j)
+                for v in range(lo, hi):
+                    run(inner, dict(env, **{m.group(1): v}))
+                i = j + len(inner) + 2
+                continue
+            m = re.compile(r"\s*(?:const\s+)?auto\s+(\w+)\s*=\s*([^;]+);").match(text, i)
+            if m:
+                try:
+                    env = dict(env, **{m.group(1): arith(m.group(2), env)})
+                except (ValueError, SyntaxError, KeyError):  # a mirror view or other non-number
+                    pass
+                i = m.end()
+                continue
+            m = re.compile(r"\s*EXPECT_EQ\(\s*(\w+)\(").match(text, i)
+            if m:
+                args = split_top(balanced(text, text.index("(", m.start() + m.group(0).index("EXPECT_EQ"))))
+                idx = args[0][args[0].index("(") + 1: args[0].rindex(")")]
+                checks.append([m.group(1), int(arith(idx, env)), arith(args[1], env)])
+                i = text.index(";", m.end()) + 1
+                continue
+            i += 1
+
+    run(body, env)
+    return checks
+
+
+def crs_cases(src, rel):
+    out = {}
+    for m in re.finditer(r"TEST\((\w+),\s*(\w+)\)\s*\{", src):
+        body = balanced(src, m.end() - 1)
+        env, arrays = {}, {}
+        for c in re.finditer(r"constexpr auto (\w+)\s*=\s*([^;]+);", body):
+            env[c.group(1)] = arith(c.group(2), env)
+        for c in re.finditer(r'CreateView<(?:[^<>]|<[^<>]*>)*>\(\s*"(\w+)",', body):
+            expr = balanced(body, c.end() - len(c.group(0)) + c.group(0).index("("))
+            data = ",".join(split_top(expr)[1:])
+            if "make_pair" in data:
+                arrays[c.group(1)] = [[arith(a), arith(b)] for a, b in re.findall(r"make_pair\(([^,]+),\s*([^)]+)\)", data)]
+            else:
+                arrays[c.group(1)] = values(data[data.index("{"):])
+        tail = body[body.index("::invoke("):]
+        tail = tail[tail.index(";") + 1:]
+        lookup = dict(env)
+        if "row_ptrs" in arrays:
+            lookup["row_ptrs_host"] = arrays["row_ptrs"]
+        out[m.group(2)] = {"scalars": env, "arrays": arrays, "sorted": "sort_crs_graph" in body,
+                           "checks": expectation_loops(tail, lookup), "source": rel}
+    return out
+
+
+def crs_known_answers(root):
+    return {key: crs_cases(strip_comments((root / rel).read_text()), rel) for key, rel in CRS_TESTS.items()}
+
+
 def near_values(body, pattern):
     """EXPECT_NEAR(<expr matching pattern>, value, tol) -> [(expr, value, tol)], in order."""
     out = []
@@ -666,6 +751,7 @@ def main():
     (HERE / "functor_known_answers.json").write_text(json.dumps(functor_known_answers(root), indent=1))
     (HERE / "integrate_known_answers.json").write_text(json.dumps(integrate_known_answers(root), indent=1))
     (HERE / "curved_beam.json").write_text(json.dumps(curved_beam(root), indent=1))
+    (HERE / "crs_known_answers.json").write_text(json.dumps(crs_known_answers(root), indent=1))
     (HERE / "masses_springs.json").write_text(json.dumps(masses_springs(root), indent=1))
     print("wrote fixtures to", HERE)
 

@@ -197,6 +197,13 @@ class OracleSystem:
         self.lib.L.orc_get_constraint(self.h, cid, which, out.ctypes.data_as(_dp))
         return out
 
+    def system_matrix(self):
+        """The assembled system matrix (after debug_assemble), global dof order."""
+        out = np.zeros((self.num_dofs, self.num_dofs))
+        self.lib.L.orc_get_matrix.argtypes = [C.c_void_p, _dp]
+        self.lib.L.orc_get_matrix(self.h, out.ctypes.data_as(_dp))
+        return out
+
     def debug_assemble(self, inputs=None):
         if inputs:
             self.apply_inputs(inputs)
