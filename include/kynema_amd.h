@@ -81,9 +81,11 @@ typedef struct {
 typedef struct {
     int exec_mode;     /* KYN_EXEC_PLAIN or KYN_EXEC_FUSED */
     int precision;     /* KYN_TANGENT_FP64 or KYN_TANGENT_FP32 */
-    int refine_sweeps; /* FP64 iterative-refinement sweeps of the FP32 solve (0..8); on the
-                          4096-rotor ensemble one sweep keeps the final state within 1e-9 of the
-                          FP64 form after 22 steps (8e-10), none within 5e-9 */
+    int refine_sweeps; /* FP64 iterative-refinement sweeps of the FP32 solve (0..8) */
+    double refine_err_bound; /* fused form: refine only in Newton iterations whose previous
+                                convergence error is below this bound (<= 0: every iteration);
+                                the increments contract ~20x per iteration, so the last two
+                                iterations, which set the final state, stay refined */
 } kyn_exec_options;
 
 /* Last error message of the calling thread (empty if none). */
@@ -201,6 +203,11 @@ int kyn_read_restart(kyn_system s, size_t model, const unsigned char* in, size_t
 /* ---- element-level access (tests): first assembly of a step without solving
  * (the sequence of test_solver.cpp:97-112), and the element terms it produced ---- */
 int kyn_debug_assemble(kyn_system s);
+/* Bound of the one wait between waves (the blocked LU's look-ahead counter): after `polls`
+ * polls (default 1 << 20, ~28 ms) the waiting wave gives up, the kernel still ends and the
+ * step returns KYN_ERR_HIP; polls < 0 gives up at the first wait (tests of that error path).
+ * No counterpart in the reference (its LU runs on the host). */
+int kyn_debug_set_wait_limit(kyn_system s, int polls);
 int kyn_get_element_terms(kyn_system s, size_t element, double* residual /* [N][6] */,
                           double* system_matrix /* [N][N][36] */);
 

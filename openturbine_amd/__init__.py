@@ -19,6 +19,9 @@ LIB_PATH = PKG / "libkynema_amd.so"
 
 KYN_OK = 0
 KYN_NOT_CONVERGED = 1
+KYN_ERR_INVALID = -1
+KYN_ERR_HIP = -2
+KYN_ERR_UNSUPPORTED = -3
 EXEC_PLAIN = 0
 EXEC_FUSED = 1
 TANGENT_FP64 = 0
@@ -38,7 +41,8 @@ class StepParams(C.Structure):
 
 
 class ExecOptions(C.Structure):
-    _fields_ = [("exec_mode", C.c_int), ("precision", C.c_int), ("refine_sweeps", C.c_int)]
+    _fields_ = [("exec_mode", C.c_int), ("precision", C.c_int), ("refine_sweeps", C.c_int),
+                ("refine_err_bound", C.c_double)]
 
 
 def build(force: bool = True) -> None:
@@ -167,6 +171,7 @@ class KynLib:
         L.kyn_write_restart.argtypes = [C.c_void_p, C.c_size_t, C.POINTER(C.c_ubyte)]
         L.kyn_read_restart.argtypes = [C.c_void_p, C.c_size_t, C.POINTER(C.c_ubyte), C.c_size_t]
         L.kyn_debug_assemble.argtypes = [C.c_void_p]
+        L.kyn_debug_set_wait_limit.argtypes = [C.c_void_p, C.c_int]
         L.kyn_get_element_terms.argtypes = [C.c_void_p, C.c_size_t, _dp, _dp]
         L.kyn_get_motion.argtypes = [C.c_void_p, _dp]
         L.kyn_model_num_nodes.argtypes = [C.c_void_p]
@@ -259,7 +264,7 @@ class KynBladeInterface:
         sol.absolute_error_tolerance = float(solution.get("atol", 1e-5))
         sol.relative_error_tolerance = float(solution.get("rtol", 1e-3))
         b, self._keep = beam
-        opts = ExecOptions(exec_mode, precision, refine_sweeps)
+        opts = ExecOptions(exec_mode, precision, refine_sweeps, 0.0)
         self.h = lib.L.kyn_blade_interface_create(C.byref(sol), C.byref(b), C.byref(opts))
         if not self.h:
             raise RuntimeError("kyn_blade_interface_create failed: " + lib.error())
@@ -348,7 +353,7 @@ class KynSystem:
     """One system advancing one or more ModelSpecs (each keeps its own Step semantics)."""
 
     def __init__(self, lib: KynLib, specs, *, exec_mode=EXEC_FUSED, precision=TANGENT_FP64, refine_sweeps=0,
-                 max_iter=None, atol=None, rtol=None):
+                 refine_err_bound=0.0, max_iter=None, atol=None, rtol=None):
         from .workloads import build_model
 
         if not isinstance(specs, (list, tuple)):
@@ -360,7 +365,7 @@ class KynSystem:
         arr = (C.c_void_p * len(models))(*models)
         params = StepParams(int(s0.is_dynamic), int(max_iter or s0.max_iter), s0.h, s0.rho_inf,
                             s0.atol if atol is None else atol, s0.rtol if rtol is None else rtol)
-        opts = ExecOptions(exec_mode, precision, refine_sweeps)
+        opts = ExecOptions(exec_mode, precision, refine_sweeps, float(refine_err_bound))
         self.h = lib.L.kyn_system_create(arr, len(models), C.byref(params), C.byref(opts))
         for m in models:
             lib.model_destroy(m)

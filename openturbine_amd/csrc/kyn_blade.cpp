@@ -453,7 +453,7 @@ kyn_blade_interface kyn_blade_interface_create(const kyn_solution_input* solutio
         bi->built = build_beam(m, *blade);
         const kyn_step_params p{solution->dynamic_solve, solution->max_iter, solution->time_step, solution->rho_inf,
                                 solution->absolute_error_tolerance, solution->relative_error_tolerance};
-        const kyn_exec_options def{KYN_EXEC_FUSED, KYN_TANGENT_FP64, 0};
+        const kyn_exec_options def{KYN_EXEC_FUSED, KYN_TANGENT_FP64, 0, 0.};
         bi->sys = kyn_system_create(&m, 1, &p, options ? options : &def);
         if (!bi->sys) throw BladeError(KYN_ERR_INVALID, std::string("kyn_system_create failed: ") + kyn_last_error());
         bi->n = kyn_system_num_nodes(bi->sys);

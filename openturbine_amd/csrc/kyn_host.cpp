@@ -279,6 +279,7 @@ struct Builder {
     std::vector<int64_t> block_C_off;  // refinement copy (fused form), padded row stride
     int64_t C_total = 0;
     std::vector<int> node_block, node_local, node_pin, block_npin;
+    std::vector<int> block_nopiv;  // FP32 tangent: the block factors without row interchanges
     std::vector<int> elem_N, elem_Q, elem_node_off, elem_nodes, elem_qp_off, elem_shape_off;
     std::vector<int64_t> elem_mat_off;
     std::vector<double> qp_weight, qp_jac, qp_Mstar, qp_Cstar, qp_x0p, qp_r0, shape_interp, shape_deriv;
@@ -295,7 +296,7 @@ struct Builder {
     int64_t mat_total = 0;
 };
 
-void add_model(Builder& B, const kyn_model_s& mo) {
+void add_model(Builder& B, const kyn_model_s& mo, bool dynamic) {
     const int node0 = static_cast<int>(B.node_block.size());
     const int con0 = static_cast<int>(B.con_type.size());
     const int elem0 = static_cast<int>(B.elem_N.size());
@@ -434,6 +435,16 @@ void add_model(Builder& B, const kyn_model_s& mo) {
             l += 6;
         }
         B.block_npin.push_back(npin);
+        {   // every node on a beam element of the block (mass and stiffness on all six dofs), no
+            // translations-only node, no point masses or springs, dynamic solve (the mass term
+            // keeps the node rows nonsingular): the FP32 factorization may skip row interchanges
+            std::vector<char> on_beam(n, 0);
+            for (int ek : belems[bi])
+                for (auto nid : mo.beams[static_cast<size_t>(ek)].nodes) on_beam[nid] = 1;
+            bool ok = dynamic && npin == 0 && bmasses[bi].empty() && bsprings[bi].empty();
+            for (int nd : bnodes[bi]) ok = ok && on_beam[static_cast<size_t>(nd)];
+            B.block_nopiv.push_back(ok ? 1 : 0);
+        }
         B.block_node_end.push_back(static_cast<int>(B.block_nodes.size()));
         const int bnsys = l;
         B.block_con_begin.push_back(static_cast<int>(B.block_cons.size()));
@@ -747,6 +758,18 @@ void solve_large_blocks(kyn_system_s* s) {
     }
 }
 
+// Device error flags raised by a kernel that gave up a bounded wait: reported (and cleared)
+// after the step, as an error of the step.
+void check_device_errors(kyn_system_s* s) {
+    int flags = 0;
+    hip_check(hipMemcpy(&flags, s->S.dev_error, sizeof(int), hipMemcpyDeviceToHost), "D2H");
+    if (flags == 0) return;
+    const int zero = 0;
+    hip_check(hipMemcpy(s->S.dev_error, &zero, sizeof(int), hipMemcpyHostToDevice), "H2D");
+    throw KynError(KYN_ERR_HIP, "a wave of the blocked LU gave up waiting on its look-ahead counter "
+                                "(bound " + std::to_string(s->S.lu_wait_limit) + " polls); the step's results are invalid");
+}
+
 int run_step(kyn_system_s* s, bool solve = true) {
     auto& S = s->S;
     hip_check(hipEventRecord(s->ev0, s->stream), "event");
@@ -799,6 +822,7 @@ int run_step(kyn_system_s* s, bool solve = true) {
         hip_check(hipEventElapsedTime(&kms, s->ek0, s->ek1), "elapsed");
     }
     s->last_kernel_ms = kms;
+    check_device_errors(s);
     if (!solve) return KYN_OK;
     std::vector<int> st(s->num_models);
     hip_check(hipMemcpy(st.data(), S.model_status, st.size() * sizeof(int), hipMemcpyDeviceToHost), "D2H");
@@ -912,7 +936,7 @@ kyn_system kyn_system_create(const kyn_model* models, size_t num_models, const k
         hip_check(hipGetDeviceCount(&dev_count), "hipGetDeviceCount");
         if (dev_count < 1) throw KynError(KYN_ERR_HIP, "no HIP device");
         Builder B;
-        for (size_t k = 0; k < num_models; ++k) add_model(B, *models[k]);
+        for (size_t k = 0; k < num_models; ++k) add_model(B, *models[k], params->is_dynamic_solve != 0);
         auto& S = sys->S;
         // ---- step parameters (step_parameters.hpp:37-52)
         const double rho = params->rho_inf, h = params->h;
@@ -933,6 +957,7 @@ kyn_system kyn_system_create(const kyn_model* models, size_t num_models, const k
         sys->exec_mode = options ? options->exec_mode : KYN_EXEC_FUSED;
         S.precision = options ? options->precision : KYN_TANGENT_FP64;
         S.refine_sweeps = options ? options->refine_sweeps : 0;
+        S.refine_err_bound = options ? options->refine_err_bound : 0.;
         if (S.refine_sweeps < 0 || S.refine_sweeps > 8) throw KynError(KYN_ERR_INVALID, "refine_sweeps in [0, 8]");
 
         // ---- sizes and eligibility
@@ -1237,6 +1262,7 @@ kyn_system kyn_system_create(const kyn_model* models, size_t num_models, const k
         S.node_local = Ar.up(B.node_local);
         S.node_pin = Ar.up(B.node_pin);
         S.block_npin = Ar.up(B.block_npin);
+        S.block_nopiv = Ar.up(B.block_nopiv);
         S.elem_N = Ar.up(B.elem_N);
         S.elem_Q = Ar.up(B.elem_Q);
         S.elem_node_off = Ar.up(B.elem_node_off);
@@ -1314,6 +1340,8 @@ kyn_system kyn_system_create(const kyn_model* models, size_t num_models, const k
         S.model_fail_total = Ar.zeros<int>(num_models);
         S.any_active = Ar.zeros<int>(1);
         S.sched_slot = Ar.zeros<int>(1);
+        S.dev_error = Ar.zeros<int>(1);
+        S.lu_wait_limit = kyn::kLuWaitLimitDefault;
         S.sched_steps = 0;
 
         // ---- host copies of the layout
@@ -1446,6 +1474,7 @@ int kyn_step_many(kyn_system s, size_t num_steps, int use_graph) {
         for (auto& e : ev) (void)hipEventDestroy(e);
         s->last_step_ms = total;
         s->last_kernel_ms = kern / static_cast<float>(num_steps);
+        check_device_errors(s);
         std::vector<int> fails(s->num_models);
         hip_check(hipMemcpy(fails.data(), S.model_fail_total, fails.size() * sizeof(int), hipMemcpyDeviceToHost), "D2H");
         for (int f : fails)
@@ -1753,9 +1782,11 @@ int kyn_write_restart(kyn_system s, size_t model, unsigned char* out) {
 int kyn_read_restart(kyn_system s, size_t model, const unsigned char* in, size_t size) {
     return guarded([&] {
         if (!s || !in || model >= s->num_models) throw KynError(KYN_ERR_INVALID, "bad model");
+        if (size < sizeof(size_t)) throw KynError(KYN_ERR_INVALID, "restart image too short for its node count");
         size_t n = 0;
         std::memcpy(&n, in, sizeof(size_t));
         const size_t nm = static_cast<size_t>(s->model_node_end[model] - s->model_node_begin[model]);
+        // read_state_from_file.hpp:28-30 (std::length_error there)
         if (n != nm) throw KynError(KYN_ERR_INVALID, "Number of system nodes in file is not the same as in model");
         if (size < kyn_restart_size(s, model)) throw KynError(KYN_ERR_INVALID, "restart image too short");
         const size_t n0 = static_cast<size_t>(s->model_node_begin[model]);
@@ -1767,6 +1798,15 @@ int kyn_read_restart(kyn_system s, size_t model, const unsigned char* in, size_t
             hip_check(hipMemcpy(p + n0 * w, in + off, n * w * sizeof(double), hipMemcpyHostToDevice), "H2D");
             off += n * w * sizeof(double);
         }
+        return KYN_OK;
+    });
+}
+
+int kyn_debug_set_wait_limit(kyn_system s, int polls) {
+    return guarded([&] {
+        if (!s) throw KynError(KYN_ERR_INVALID, "null system");
+        s->S.lu_wait_limit = polls;
+        s->graph_ready = false;  // the captured step holds the kernel arguments
         return KYN_OK;
     });
 }

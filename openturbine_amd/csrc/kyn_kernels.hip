@@ -12,9 +12,11 @@
 // calculate_system_matrix.hpp): S_ij = sum_q phi_i R0_q[j] + phi'_i R1_q[j] where
 //   R0_q[j] = (phi_j wJ(Kuu+Quu) + phi'_j w Puu) T_j + phi_j wJ(beta' Muu + gamma' Guu)
 //   R1_q[j] = (phi_j w Ouu + phi'_j (w/J) Cuu) T_j
-// i.e. an (N x 2Q) x (2Q x 36N) product.  Each of 252 threads owns one 6x6 component c and
-// the nodes j = jg, jg+7, jg+14, forms its own R entries per QP in registers and accumulates
-// all rows i (a 21 x 3 register tile); no LDS staging of R, no barrier inside the QP loop.
+// i.e. an (N x 2Q) x (2Q x 36N) product.  FP64 tangent: each of 252 threads owns one 6x6
+// component c and the nodes j = jg, jg+7, jg+14, forms its own R entries per QP in registers
+// and accumulates all rows i (a 21 x 3 register tile); no LDS staging of R, no barrier inside
+// the QP loop.  FP32 tangent: v_mfma_f32_16x16x4f32 tiles of the same product, every lane
+// forming the one R entry of its B operand (contract_mfma_f32).
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -41,6 +43,7 @@ constexpr int kContractThreads = 252;  // 36 components x 7 node groups
 // {phi_i(q), phi'_i(q)} pairs per QP, rows padded with zeros to 22: one QP's pairs are whole
 // 16-byte vectors (11 for FP32, 22 for FP64)
 constexpr int kPhiStride = 44;
+typedef float f32x4 __attribute__((ext_vector_type(4)));
 static_assert(kPhiStride >= 2 * kMaxElemNodes && kPhiStride % 4 == 0, "phi pair stride");
 
 // ---------------------------------------------------------------------------------------
@@ -119,12 +122,196 @@ template <typename T>
 __device__ __forceinline__ T tabs(T v) { return v < T(0) ? -v : v; }
 
 // ---------------------------------------------------------------------------------------
+// Node-pair contraction of the FP32 tangent on the matrix cores.
+//
+// S_ij[r][s] = sum_q phi_i(q) R0_q[j][r][s] + phi'_i(q) R1_q[j][r][s] is the product of
+// L (N x 2Q: phi_i(q), phi'_i(q)) and R (2Q x 36N), formed as v_mfma_f32_16x16x4f32 tiles: rows
+// i in one or two 16-row tiles, k = 4 ks + lk covering QPs 2 ks and 2 ks + 1 (lane group lk: QP
+// 2 ks + lk / 2, R0 for even lk, R1 for odd), 16 columns per tile.  Every lane forms the one R
+// entry its B operand needs, from its QP's matrices (LDS) and node j's tangent block.  Columns
+// are ordered c = h EP + e with h = 0 for the translational columns s < 3 (T_j is the identity
+// there) and h = 1 for the rotational ones (R carries T_j), e = 18 j + 3 r + s' (s = 3 h + s'),
+// EP = 16 TH, TH = ceil(18 N / 16) tiles per half.
+//
+// Wave w owns tiles w, w + 4, ... (tile ids are wave-uniform scalars, so the branches on them
+// are scalar branches) and runs the QP loop inside each tile: a lane's column (j, r, s) and node
+// j's tangent column are fixed for the whole loop, which per QP pair loads the lane's A entry,
+// node j's shape pair and the QP-matrix entries of its R, forms R with 2 (h = 0) or 8 (h = 1)
+// FMAs and issues one or two MFMAs.  The accumulators stay in registers until every wave is
+// done with the element view, which the matrix view overlays.
+// ---------------------------------------------------------------------------------------
+constexpr int kMfmaTilesPerWave = 12;  // 2 halves x ceil(18 x 21 / 16) tiles over 4 waves
+static_assert(2 * ((18 * kMaxElemNodes + 15) / 16) <= 4 * kMfmaTilesPerWave, "tiles per wave");
+
+// R entry of a translational column (T_j = I): R0 = phi_j (Mi + Kuu + Quu)[r][s] + phi'_j
+// Puu[r][s], R1 = phi_j Ouu[r][s] + phi'_j Cuu[r][s].  M6 points at row r of the QP's matrices
+// (Kuu, Quu, Mi, Puu, Ouu, Cuu at offsets 0, 36, 72, 108, 144, 180).  Every lane loads the same
+// set of entries and selects (no divergent branch around the loads).
+__device__ __forceinline__ float r_entry_trans(const float* M6, bool t, int sp, float pj, float dj) {
+    const float a = M6[(t ? 144 : 72) + sp];
+    const float y = M6[(t ? 180 : 108) + sp];
+    const float k = M6[sp] + M6[36 + sp];
+    const float x = t ? a : a + k;
+    return __builtin_fmaf(pj, x, dj * y);
+}
+
+// R entry of a rotational column s = 3 + s': R0 = phi_j (Mi[r][s] + sum_u (Kuu + Quu)[r][3+u]
+// T[u][s']) + phi'_j sum_u Puu[r][3+u] T[u][s'], R1 = sum_u (phi_j Ouu + phi'_j Cuu)[r][3+u]
+// T[u][s'], with (t0, t1, t2) = column s' of T_j.
+__device__ __forceinline__ float r_entry_rot(const float* M6, bool t, int sp, float pj, float dj, float t0, float t1,
+                                             float t2) {
+    const float* X = M6 + (t ? 147 : 3);
+    const float* Y = M6 + (t ? 183 : 111);
+    const float a0 = X[0], a1 = X[1], a2 = X[2];
+    const float q0 = M6[39], q1 = M6[40], q2 = M6[41];
+    const float y0 = Y[0], y1 = Y[1], y2 = Y[2];
+    const float m = M6[75 + sp];
+    const float x0 = t ? a0 : a0 + q0, x1 = t ? a1 : a1 + q1, x2 = t ? a2 : a2 + q2;
+    const float z = t ? 0.f : m;
+    const float za = __builtin_fmaf(x2, t2, __builtin_fmaf(x1, t1, __builtin_fmaf(x0, t0, z)));
+    const float zb = __builtin_fmaf(y2, t2, __builtin_fmaf(y1, t1, y0 * t0));
+    return __builtin_fmaf(pj, za, dj * zb);
+}
+
+template <bool kFused>
+__device__ __forceinline__ void contract_mfma_f32(const DevSys& S, char* sm, const Lay& L, int n, int N, int Q,
+                                                  const int* loc, double* mat_out) {
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int Nu = __builtin_amdgcn_readfirstlane(N), Qu = __builtin_amdgcn_readfirstlane(Q);
+    const int li = lane & 15, lk = lane >> 4, qo = lk >> 1;
+    const bool t = (lk & 1) != 0;
+    const float* qpM = reinterpret_cast<const float*>(sm + L.qpM);
+    const float* phiT = reinterpret_cast<const float*>(sm + L.phiT);
+    const double* tr = reinterpret_cast<const double*>(sm + L.tr);
+    const int TH = (18 * Nu + 15) >> 4;
+    const int ntiles = 2 * TH;
+    const bool two_rows = Nu > 16;
+    const int npairs = Qu >> 1;  // QP pairs with both QPs present; an odd Q adds one with QP Q - 1 only
+    // A operand: row li (and 16 + li) of L at column k: phi (t = 0) or phi' (t = 1) of the lane's QP
+    const int ia = 2 * li + (t ? 1 : 0);
+    const bool row2 = li < kPhiStride / 2 - 16;  // rows 16 + li present in the zero-padded phi table
+    const int ib = row2 ? ia + 32 : ia;
+    // the lane's QP within a pair, and the odd-Q tail (QP Q - 1 for both lane halves, A zero for qo = 1)
+    const float* M6q = qpM + qo * 216;
+    const float* phq = phiT + qo * kPhiStride;
+    const int qt = Qu - 1;
+    const bool tail = (Qu & 1) != 0;
+    const bool tail_valid = qo == 0;
+    f32x4 acc[kMfmaTilesPerWave][2];
+#pragma unroll
+    for (int s = 0; s < kMfmaTilesPerWave; ++s)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) acc[s][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+#pragma unroll
+    for (int s = 0; s < kMfmaTilesPerWave; ++s) {
+        const int T = wv + 4 * s;
+        if (T < ntiles) {
+            const int h = T >= TH ? 1 : 0;
+            const int ecol = 16 * (T - h * TH) + li;
+            // padded columns (ecol >= 18 N) form the entry of the last column; discarded at write-out
+            const int ec = ecol < 18 * Nu ? ecol : 18 * Nu - 1;
+            const int j = ec / 18, rem = ec - 18 * j, r_ = rem / 3, sp = rem - 3 * r_;
+            f32x4 c0 = acc[s][0], c1 = acc[s][1];
+            if (h == 0) {
+                for (int ks = 0; ks < npairs; ++ks) {
+                    const float* ph = phq + ks * (2 * kPhiStride);
+                    const float R = r_entry_trans(M6q + ks * 432 + r_ * 6, t, sp, ph[2 * j], ph[2 * j + 1]);
+                    const float a1 = ph[ib];
+                    c0 = __builtin_amdgcn_mfma_f32_16x16x4f32(ph[ia], R, c0, 0, 0, 0);
+                    if (two_rows) c1 = __builtin_amdgcn_mfma_f32_16x16x4f32(row2 ? a1 : 0.f, R, c1, 0, 0, 0);
+                }
+                if (tail) {
+                    const float* ph = phiT + qt * kPhiStride;
+                    const float R = r_entry_trans(qpM + qt * 216 + r_ * 6, t, sp, ph[2 * j], ph[2 * j + 1]);
+                    const float a0 = ph[ia], a1 = ph[ib];
+                    c0 = __builtin_amdgcn_mfma_f32_16x16x4f32(tail_valid ? a0 : 0.f, R, c0, 0, 0, 0);
+                    if (two_rows)
+                        c1 = __builtin_amdgcn_mfma_f32_16x16x4f32(tail_valid && row2 ? a1 : 0.f, R, c1, 0, 0, 0);
+                }
+            } else {
+                const double* Tj = tr + j * 9 + sp;  // column s' of node j's rotational tangent block
+                const float t0 = (float)Tj[0], t1 = (float)Tj[3], t2 = (float)Tj[6];
+                for (int ks = 0; ks < npairs; ++ks) {
+                    const float* ph = phq + ks * (2 * kPhiStride);
+                    const float R = r_entry_rot(M6q + ks * 432 + r_ * 6, t, sp, ph[2 * j], ph[2 * j + 1], t0, t1, t2);
+                    const float a1 = ph[ib];
+                    c0 = __builtin_amdgcn_mfma_f32_16x16x4f32(ph[ia], R, c0, 0, 0, 0);
+                    if (two_rows) c1 = __builtin_amdgcn_mfma_f32_16x16x4f32(row2 ? a1 : 0.f, R, c1, 0, 0, 0);
+                }
+                if (tail) {
+                    const float* ph = phiT + qt * kPhiStride;
+                    const float R = r_entry_rot(qpM + qt * 216 + r_ * 6, t, sp, ph[2 * j], ph[2 * j + 1], t0, t1, t2);
+                    const float a0 = ph[ia], a1 = ph[ib];
+                    c0 = __builtin_amdgcn_mfma_f32_16x16x4f32(tail_valid ? a0 : 0.f, R, c0, 0, 0, 0);
+                    if (two_rows)
+                        c1 = __builtin_amdgcn_mfma_f32_16x16x4f32(tail_valid && row2 ? a1 : 0.f, R, c1, 0, 0, 0);
+                }
+            }
+            acc[s][0] = c0;
+            acc[s][1] = c1;
+        }
+    }
+    __syncthreads();  // every wave done with the element view
+    KYN_PHASE_MARK(4);
+    // ---- write out: lane holds S[i = 16 b + 4 lk + r][column of its tile]
+    float* A = reinterpret_cast<float*>(sm + L.A);
+    const float cond = (float)S.p.conditioner;
+    int rowoff[2][4];  // block-local row of node i (fused) or i (otherwise); -1 past N
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int i = 16 * b + 4 * lk + r;
+            rowoff[b][r] = i < Nu ? (kFused ? loc[i] : i) : -1;
+        }
+    if (kFused) {
+        uint4* A4 = reinterpret_cast<uint4*>(sm + L.A);
+        const int nvec = (n * L.lda * (int)sizeof(float) + 15) / 16;
+        for (int k = tid; k < nvec; k += blockDim.x) A4[k] = make_uint4(0u, 0u, 0u, 0u);
+        __syncthreads();
+    }
+#pragma unroll
+    for (int s = 0; s < kMfmaTilesPerWave; ++s) {
+        const int T = wv + 4 * s;
+        if (T < ntiles) {
+            const int h = T >= TH ? 1 : 0;
+            const int ecol = 16 * (T - h * TH) + li;
+            if (ecol < 18 * Nu) {
+                const int j = ecol / 18, rem = ecol - 18 * j, r_ = rem / 3, col = 3 * h + rem - 3 * r_;
+                if (kFused) {
+                    const int c = loc[j] + col;
+#pragma unroll
+                    for (int b = 0; b < 2; ++b)
+#pragma unroll
+                        for (int r = 0; r < 4; ++r)
+                            if (rowoff[b][r] >= 0) A[(rowoff[b][r] + r_) * L.lda + c] = acc[s][b][r] * cond;
+                } else {
+                    double* out = mat_out + (size_t)j * 36 + r_ * 6 + col;
+#pragma unroll
+                    for (int b = 0; b < 2; ++b)
+#pragma unroll
+                        for (int r = 0; r < 4; ++r)
+                            if (rowoff[b][r] >= 0) out[(size_t)rowoff[b][r] * Nu * 36] = (double)acc[s][b][r];
+                }
+            }
+        }
+    }
+    if (kFused) __syncthreads();
+}
+
+// ---------------------------------------------------------------------------------------
 // Element phase: gather, QP values, residual, contraction.
 //   kFused: residual goes into the block rhs (-f + R) and the tangent into LDS matrix A
-//           (times the conditioner, at block-local dofs); otherwise into elem_res/elem_mat.
+//           (times the conditioner, at block-local dofs); otherwise into res_out ([N][6]) and
+//           mat_out ([N][N][36]).
+//   kOwnTangent: the nodes' rotational tangent blocks are formed here from q_delta
+//           (UpdateTangentOperator folded in) rather than read from tan_rot.
 // ---------------------------------------------------------------------------------------
-template <typename TT, bool kFused>
-__device__ __forceinline__ void element_phase(const DevSys& S, int e, char* sm, const Lay& L, int n) {
+template <typename TT, bool kFused, bool kOwnTangent = kFused>
+__device__ __forceinline__ void element_phase(const DevSys& S, int e, char* sm, const Lay& L, int n,
+                                              double* res_out = nullptr, double* mat_out = nullptr) {
     const int tid = threadIdx.x;
     const int N = S.elem_N[e], Q = S.elem_Q[e];
     const int noff = S.elem_node_off[e];
@@ -156,7 +343,7 @@ __device__ __forceinline__ void element_phase(const DevSys& S, int e, char* sm, 
             nv[k * 6 + c] = S.v[node * 6 + c];
             nvd[k * 6 + c] = S.vd[node * 6 + c];
         }
-        if (kFused) {
+        if (kOwnTangent) {
             // UpdateTangentOperator (update_tangent_operator.hpp:13-26) folded in
             double qd[6];
 #pragma unroll
@@ -254,130 +441,135 @@ __device__ __forceinline__ void element_phase(const DevSys& S, int e, char* sm, 
             const int node = S.elem_nodes[noff + k];
             rhs[loc[k] + c] = -S.f[node * 6 + c] + R;
         } else {
-            S.elem_res[(size_t)(noff + k) * 6 + c] = R;
+            res_out[k * 6 + c] = R;
         }
     }
 
     KYN_PHASE_MARK(3);
-    // ---- contraction, one pass over the QPs, no barrier
-    // Thread t < 252 owns component cc = t % 36 = (r, s) and nodes j = t / 36 + 7 b (b < 3);
-    // per QP it forms its own R0_q[j][cc], R1_q[j][cc] from row r of the QP matrices and
-    // column s of T_j, then accumulates S_ij[cc] += phi_i R0 + phi'_i R1 for every row i
-    // (the QP's phi pairs: broadcast vector loads, all rows at once).  Each R entry is formed by
-    // exactly one thread.
-    const int cc = tid % 36, jg = tid / 36;
-    const bool active = tid < kContractThreads;
-    const int r_ = cc / 6, s_ = cc - 6 * (cc / 6);
-    const bool rot = s_ >= 3;
-    const int col0 = rot ? 3 : 0;  // columns of A/P/O/C combined with T_j's column (s - 3)
-    TT tc[3][3];
-    int jc[3];
-#pragma unroll
-    for (int b = 0; b < 3; ++b) {
-        const int j = jg + 7 * b;
-        const bool ok = active && j < N;
-        jc[b] = ok ? j : 0;
-#pragma unroll
-        for (int u = 0; u < 3; ++u)
-            tc[b][u] = !ok ? TT(0) : (rot ? (TT)tr[j * 9 + u * 3 + (s_ - 3)] : (u == s_ ? TT(1) : TT(0)));
-    }
-    TT acc[kMaxElemNodes][3];
-#pragma unroll
-    for (int i = 0; i < kMaxElemNodes; ++i)
-#pragma unroll
-        for (int b = 0; b < 3; ++b) acc[i][b] = TT(0);
-    const int Nu = __builtin_amdgcn_readfirstlane(N);  // uniform: a scalar branch per row group
-    for (int q = 0; q < Q; ++q) {
-        const TT* M6 = qpM + (size_t)q * 6 * 36 + r_ * 6;
-        TT ra[3], rp[3], ro[3], rc[3];
-#pragma unroll
-        for (int u = 0; u < 3; ++u) {
-            ra[u] = M6[col0 + u] + M6[36 + col0 + u];
-            rp[u] = M6[108 + col0 + u];
-            ro[u] = M6[144 + col0 + u];
-            rc[u] = M6[180 + col0 + u];
-        }
-        const TT rm = M6[72 + s_];
-        const TT* ph = phiT + (size_t)q * kPhiStride;
-        TT pj[3], dj[3];
+    if constexpr (sizeof(TT) == 4) {
+        // FP32 tangent: the contraction on the matrix cores
+        contract_mfma_f32<kFused>(S, sm, L, n, N, Q, loc, mat_out);
+    } else {
+        // ---- contraction, one pass over the QPs, no barrier
+        // Thread t < 252 owns component cc = t % 36 = (r, s) and nodes j = t / 36 + 7 b (b < 3);
+        // per QP it forms its own R0_q[j][cc], R1_q[j][cc] from row r of the QP matrices and
+        // column s of T_j, then accumulates S_ij[cc] += phi_i R0 + phi'_i R1 for every row i
+        // (the QP's phi pairs: broadcast vector loads, all rows at once).  Each R entry is formed by
+        // exactly one thread.
+        const int cc = tid % 36, jg = tid / 36;
+        const bool active = tid < kContractThreads;
+        const int r_ = cc / 6, s_ = cc - 6 * (cc / 6);
+        const bool rot = s_ >= 3;
+        const int col0 = rot ? 3 : 0;  // columns of A/P/O/C combined with T_j's column (s - 3)
+        TT tc[3][3];
+        int jc[3];
 #pragma unroll
         for (int b = 0; b < 3; ++b) {
-            pj[b] = ph[jc[b] * 2];
-            dj[b] = ph[jc[b] * 2 + 1];
+            const int j = jg + 7 * b;
+            const bool ok = active && j < N;
+            jc[b] = ok ? j : 0;
+#pragma unroll
+            for (int u = 0; u < 3; ++u)
+                tc[b][u] = !ok ? TT(0) : (rot ? (TT)tr[j * 9 + u * 3 + (s_ - 3)] : (u == s_ ? TT(1) : TT(0)));
         }
-        // every row's pair for this QP in registers (whole vectors, one wait), then the rows
-        constexpr int VW = 16 / (int)sizeof(TT);
-        typedef TT vec_t __attribute__((ext_vector_type(16 / sizeof(TT))));
-        TT phv[kPhiStride];
+        TT acc[kMaxElemNodes][3];
 #pragma unroll
-        for (int c = 0; c < kPhiStride / VW; ++c) {
-            const vec_t v = *reinterpret_cast<const vec_t*>(ph + c * VW);
+        for (int i = 0; i < kMaxElemNodes; ++i)
 #pragma unroll
-            for (int e = 0; e < VW; ++e) phv[c * VW + e] = v[e];
-        }
-        TT R0[3], R1[3];
-#pragma unroll
-        for (int b = 0; b < 3; ++b) {
-            TT v0 = pj[b] * rm, v1 = TT(0);
+            for (int b = 0; b < 3; ++b) acc[i][b] = TT(0);
+        const int Nu = __builtin_amdgcn_readfirstlane(N);  // uniform: a scalar branch per row group
+        for (int q = 0; q < Q; ++q) {
+            const TT* M6 = qpM + (size_t)q * 6 * 36 + r_ * 6;
+            TT ra[3], rp[3], ro[3], rc[3];
 #pragma unroll
             for (int u = 0; u < 3; ++u) {
-                v0 += (pj[b] * ra[u] + dj[b] * rp[u]) * tc[b][u];
-                v1 += (pj[b] * ro[u] + dj[b] * rc[u]) * tc[b][u];
+                ra[u] = M6[col0 + u] + M6[36 + col0 + u];
+                rp[u] = M6[108 + col0 + u];
+                ro[u] = M6[144 + col0 + u];
+                rc[u] = M6[180 + col0 + u];
             }
-            R0[b] = v0;
-            R1[b] = v1;
-        }
-        // rows in groups of three behind a scalar branch on N (rows past N are skipped)
+            const TT rm = M6[72 + s_];
+            const TT* ph = phiT + (size_t)q * kPhiStride;
+            TT pj[3], dj[3];
 #pragma unroll
-        for (int i0 = 0; i0 < kMaxElemNodes; i0 += 3) {
-            if (i0 < Nu) {
+            for (int b = 0; b < 3; ++b) {
+                pj[b] = ph[jc[b] * 2];
+                dj[b] = ph[jc[b] * 2 + 1];
+            }
+            // every row's pair for this QP in registers (whole vectors, one wait), then the rows
+            constexpr int VW = 16 / (int)sizeof(TT);
+            typedef TT vec_t __attribute__((ext_vector_type(16 / sizeof(TT))));
+            TT phv[kPhiStride];
 #pragma unroll
-                for (int i = i0; i < i0 + 3 && i < kMaxElemNodes; ++i) {
-                    const TT pi = phv[i * 2], di = phv[i * 2 + 1];
+            for (int c = 0; c < kPhiStride / VW; ++c) {
+                const vec_t v = *reinterpret_cast<const vec_t*>(ph + c * VW);
 #pragma unroll
-                    for (int b = 0; b < 3; ++b) acc[i][b] += pi * R0[b] + di * R1[b];
+                for (int e = 0; e < VW; ++e) phv[c * VW + e] = v[e];
+            }
+            TT R0[3], R1[3];
+#pragma unroll
+            for (int b = 0; b < 3; ++b) {
+                TT v0 = pj[b] * rm, v1 = TT(0);
+#pragma unroll
+                for (int u = 0; u < 3; ++u) {
+                    v0 += (pj[b] * ra[u] + dj[b] * rp[u]) * tc[b][u];
+                    v1 += (pj[b] * ro[u] + dj[b] * rc[u]) * tc[b][u];
                 }
+                R0[b] = v0;
+                R1[b] = v1;
             }
-        }
-    }
-    __syncthreads();
-
-    KYN_PHASE_MARK(4);
-    // ---- write out
-    if (kFused) {
-        // the overlay is free after the barrier above: zero the matrix view, then write
-        TT* A = reinterpret_cast<TT*>(sm + L.A);
-        {   // 16-byte stores; the matrix view is 16-byte aligned and reserved to a 16-byte multiple
-            uint4* A4 = reinterpret_cast<uint4*>(sm + L.A);
-            const int nv = (n * L.lda * (int)sizeof(TT) + 15) / 16;
-            for (int k = tid; k < nv; k += blockDim.x) A4[k] = make_uint4(0u, 0u, 0u, 0u);
-        }
-        __syncthreads();
-        const TT cond = (TT)S.p.conditioner;
-        if (active) {
+            // rows in groups of three behind a scalar branch on N (rows past N are skipped)
 #pragma unroll
-            for (int i = 0; i < kMaxElemNodes; ++i) {
-                if (i < N) {
-                    const int row = loc[i] + r_;
+            for (int i0 = 0; i0 < kMaxElemNodes; i0 += 3) {
+                if (i0 < Nu) {
 #pragma unroll
-                    for (int b = 0; b < 3; ++b) {
-                        const int j = jg + 7 * b;
-                        if (j < N) A[row * L.lda + loc[j] + s_] = acc[i][b] * cond;
+                    for (int i = i0; i < i0 + 3 && i < kMaxElemNodes; ++i) {
+                        const TT pi = phv[i * 2], di = phv[i * 2 + 1];
+#pragma unroll
+                        for (int b = 0; b < 3; ++b) acc[i][b] += pi * R0[b] + di * R1[b];
                     }
                 }
             }
         }
         __syncthreads();
-    } else {
-        if (active) {
-            double* out = S.elem_mat + S.elem_mat_off[e];
+
+        KYN_PHASE_MARK(4);
+        // ---- write out
+        if (kFused) {
+            // the overlay is free after the barrier above: zero the matrix view, then write
+            TT* A = reinterpret_cast<TT*>(sm + L.A);
+            {   // 16-byte stores; the matrix view is 16-byte aligned and reserved to a 16-byte multiple
+                uint4* A4 = reinterpret_cast<uint4*>(sm + L.A);
+                const int nv = (n * L.lda * (int)sizeof(TT) + 15) / 16;
+                for (int k = tid; k < nv; k += blockDim.x) A4[k] = make_uint4(0u, 0u, 0u, 0u);
+            }
+            __syncthreads();
+            const TT cond = (TT)S.p.conditioner;
+            if (active) {
 #pragma unroll
-            for (int i = 0; i < kMaxElemNodes; ++i) {
-                if (i < N) {
+                for (int i = 0; i < kMaxElemNodes; ++i) {
+                    if (i < N) {
+                        const int row = loc[i] + r_;
 #pragma unroll
-                    for (int b = 0; b < 3; ++b) {
-                        const int j = jg + 7 * b;
-                        if (j < N) out[(size_t)(i * N + j) * 36 + cc] = (double)acc[i][b];
+                        for (int b = 0; b < 3; ++b) {
+                            const int j = jg + 7 * b;
+                            if (j < N) A[row * L.lda + loc[j] + s_] = acc[i][b] * cond;
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+        } else {
+            if (active) {
+                double* out = mat_out;
+#pragma unroll
+                for (int i = 0; i < kMaxElemNodes; ++i) {
+                    if (i < N) {
+#pragma unroll
+                        for (int b = 0; b < 3; ++b) {
+                            const int j = jg + 7 * b;
+                            if (j < N) out[(size_t)(i * N + j) * 36 + cc] = (double)acc[i][b];
+                        }
                     }
                 }
             }
@@ -413,6 +605,8 @@ struct LuScratch {
                    // buffer while this panel is still being applied); after the LU, buffer 0
     void* lcol;    // two 16 x 16 column-major L11 tiles (panel p in tile p & 1), storage type
     int* sync;     // look-ahead counter: waves 1.. add one each when the next panel's columns are done
+    int wait_limit;  // polls before wave 0 gives up waiting on the counter (< 0: at once)
+    int* error;      // device error flags (bit kDevErrLuWait set on giving up)
 };
 __device__ __forceinline__ int* phys_buf(const LuScratch& W, int c0) {
     return W.phys_of + ((c0 / kPanel) & 1) * kLuTile;
@@ -869,8 +1063,17 @@ __device__ __forceinline__ void lu_factor_augmented(TT* A, int n, int lda, int* 
             if (wn > 0) {
                 lu_apply<TT>(A, n, lda, c0, w, j0, j0 + wn, 0, nw, W);  // wave 0's share of the look-ahead
                 done += nw - 1;
-                while (__hip_atomic_load(W.sync, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) < done)
+                // bounded wait (every other wave adds one per panel, so it ends; the bound keeps a
+                // broken count from hanging the kernel: the wave then goes on and flags the error)
+                for (int polls = 0;
+                     __hip_atomic_load(W.sync, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) < done; ++polls) {
+                    if (W.wait_limit < 0 || polls >= W.wait_limit) {
+                        if ((tid & 63) == 0)
+                            __hip_atomic_fetch_or(W.error, kDevErrLuWait, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        break;
+                    }
                     __builtin_amdgcn_s_sleep(1);
+                }
                 KYN_SPAN_END(0, t_sp, 0);
                 lu_panel<TT>(A, n, lda, j0, wn, piv, W);
             }
@@ -890,6 +1093,143 @@ __device__ __forceinline__ void lu_factor_augmented(TT* A, int n, int lda, int* 
     if (((n - 1) / kPanel) & 1) {  // final mapping into buffer 0
         for (int i = tid; i < n; i += blockDim.x) W.phys_of[i] = W.phys_of[kLuTile + i];
         __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Unpivoted blocked LU of the FP32 tangent (blocks flagged block_nopiv at setup: dynamic
+// solve, every node of the block on a beam element, no translations-only nodes).  There the
+// conditioned iteration matrix beta h^2 (beta' M + gamma' G + K T) of the node rows is a small
+// perturbation of an SPD matrix (mass plus stiffness), which factors stably without row
+// interchanges, and the constraint rows come last: their pivots are those of the Schur
+// complement -B (beta h^2 K)^-1 B^T, after which nothing else is eliminated.  (The FP64 forms
+// keep the pivoted LU above, whose factors are the oracle's bit for bit.)
+//
+// Panels of 16 columns, three phases each, one barrier after each:
+//  P1  every wave factors the 16 x 16 diagonal tile in lanes 0..15 (row per lane, each pivot
+//      row to the wave by readlane: no cross-wave traffic) and applies it, as it goes, to up to
+//      48 rows below in lanes 16..63 (L21 = A21 U11^-1, rows split over the waves); wave 0
+//      writes the tile back and keeps L11 as a compact column-major tile for P2;
+//  P2  U12 = L11^-1 A12, one column per thread (right-hand side column n included);
+//  P3  A22 -= L21 U12 on the matrix cores: 16 x 16 output tiles, each the sum of four
+//      v_mfma_f32_16x16x4f32 products over the panel's 16 columns, tiles round-robin over the
+//      waves, rows past n - 1 and columns past n masked.
+// FP32 arithmetic with explicit FMAs (no reliance on contraction).  Afterwards column n holds
+// y = L^-1 b and the row map is the identity.
+// ---------------------------------------------------------------------------------------
+template <int J>
+__device__ __forceinline__ void np_tile_steps(float (&a)[kPanel], int w, int lane) {
+    if constexpr (J < kPanel) {
+        if (J < w) {  // uniform
+            float pr[kPanel];
+#pragma unroll
+            for (int c = J; c < kPanel; ++c) pr[c] = rdlane(a[c], J);  // pivot row J (lane J)
+            const float inv = 1.f / pr[J];
+            if (lane > J) {  // tile rows below the pivot and every L21 row
+                const float l = a[J] * inv;
+                a[J] = l;
+#pragma unroll
+                for (int c = J + 1; c < kPanel; ++c) a[c] = __builtin_fmaf(-l, pr[c], a[c]);
+            }
+        }
+        np_tile_steps<J + 1>(a, w, lane);
+    }
+}
+
+__device__ __forceinline__ void lu_factor_nopiv_f32(float* A, int n, int lda, const LuScratch& W) {
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int nw = blockDim.x >> 6;
+    for (int r = tid; r < kLuTile; r += blockDim.x) W.phys_of[r] = r;
+    float* Lc = reinterpret_cast<float*>(W.lcol);
+    for (int k0 = 0; k0 < n; k0 += kPanel) {
+        const int w = (n - k0) < kPanel ? (n - k0) : kPanel;
+        const int r0 = k0 + kPanel;  // first row / column after the panel
+        // ---- P1: diagonal tile + this wave's rows below.  Every wave reads the tile, so it is
+        // written back only after the barrier (by wave 0, during P2, which does not touch it);
+        // the L21 rows belong to one wave each and go back at once.
+        float a[kPanel];
+        {
+            const int row = lane < kPanel ? k0 + lane : r0 + wv * (64 - kPanel) + (lane - kPanel);
+            const bool ok = lane < kPanel ? lane < w : row < n;
+            const int rs = ok ? row : k0;  // clamped: loads unconditional, values selected after
+#pragma unroll
+            for (int c = 0; c < kPanel; ++c) {
+                a[c] = A[rs * lda + k0 + (c < w ? c : 0)];
+                pin(a[c]);
+                a[c] = (ok && c < w) ? a[c] : 0.f;
+            }
+            np_tile_steps<0>(a, w, lane);
+            if (ok && lane >= kPanel) {
+#pragma unroll
+                for (int c = 0; c < kPanel; ++c) A[row * lda + k0 + c] = a[c];
+            }
+            if (wv == 0 && lane < kPanel) {  // compact unit-lower L11, column-major, zeros elsewhere
+#pragma unroll
+                for (int k = 0; k < kPanel; ++k) Lc[k * kPanel + lane] = (k < lane && lane < w) ? a[k] : 0.f;
+            }
+        }
+        __syncthreads();
+        if (wv == 0 && lane < w) {  // the factored tile: U11 and the unit L11 below its diagonal
+#pragma unroll
+            for (int c = 0; c < kPanel; ++c)
+                if (c < w) A[(k0 + lane) * lda + k0 + c] = a[c];
+        }
+        // ---- P2: U12 rows (columns r0 .. n, the rhs included); a partial last panel has only
+        // the rhs column after it
+        for (int j = k0 + w + tid; j <= n; j += blockDim.x) {
+            float u[kPanel];
+#pragma unroll
+            for (int i = 0; i < kPanel; ++i) {
+                u[i] = A[(k0 + (i < w ? i : 0)) * lda + j];
+                pin(u[i]);
+                u[i] = i < w ? u[i] : 0.f;
+            }
+            u_rows_group<float, 0>(Lc, u);
+            u_rows_group<float, 4>(Lc, u);
+            u_rows_group<float, 8>(Lc, u);
+            u_rows_group<float, 12>(Lc, u);
+#pragma unroll
+            for (int i = 1; i < kPanel; ++i)
+                if (i < w) A[(k0 + i) * lda + j] = u[i];
+        }
+        __syncthreads();
+        // ---- P3: trailing update on MFMA (full panels with rows after them only)
+        const int m = n - r0;       // trailing rows
+        const int mc = n + 1 - r0;  // trailing columns, rhs included
+        if (w == kPanel && m > 0) {
+            const int TR = (m + 15) >> 4, TC = (mc + 15) >> 4;
+            const int li = lane & 15, lk = lane >> 4;  // operand row/column and k within a step
+            for (int t = wv; t < TR * TC; t += nw) {
+                const int ti = t / TC, tj = t - TC * (t / TC);
+                const int ia = r0 + 16 * ti + li;  // A-operand row (L21)
+                const int jb = r0 + 16 * tj + li;  // B-operand / output column
+                const bool oka = ia < n, okb = jb <= n;
+                float ao[4], bo[4];
+#pragma unroll
+                for (int ks = 0; ks < 4; ++ks) {
+                    ao[ks] = A[(oka ? ia : r0) * lda + k0 + 4 * ks + lk];
+                    bo[ks] = A[(k0 + 4 * ks + lk) * lda + (okb ? jb : r0)];
+                }
+                f32x4 acc;
+                const int ic = r0 + 16 * ti + 4 * lk;  // first output row of this lane
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const bool okc = okb && ic + r < n;
+                    float v = A[(okc ? ic + r : r0) * lda + (okb ? jb : r0)];
+                    pin(v);
+                    acc[r] = okc ? v : 0.f;
+                }
+#pragma unroll
+                for (int ks = 0; ks < 4; ++ks) {
+                    const float a_ = oka ? -ao[ks] : 0.f, b_ = okb ? bo[ks] : 0.f;
+                    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a_, b_, acc, 0, 0, 0);
+                }
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    if (okb && ic + r < n) A[(ic + r) * lda + jb] = acc[r];
+            }
+            __syncthreads();
+        }
     }
 }
 
@@ -931,33 +1271,6 @@ __device__ __forceinline__ void back_sweep(const TT* A, int n, int lda, const in
                     const int i = lane + 64 * a;
                     const double upd = fma(-(double)U[a][c], xk, r[a]);
                     r[a] = i < k ? upd : (i == k ? xk : r[a]);
-                }
-            }
-        }
-    }
-}
-
-// Forward substitution L y = r (unit L) by wave 0, the oracle's order: r_i -= l_ik y_k for
-// i > k, k ascending, in blocks of 8 columns.
-template <typename TT>
-__device__ __forceinline__ void forward_sweep(const TT* A, int n, int lda, const int* pr, double* r) {
-    const int lane = threadIdx.x & 63;
-    for (int kb = 0; kb < n; kb += kSweepBlock) {
-        TT Lc[3][kSweepBlock];
-#pragma unroll
-        for (int a = 0; a < 3; ++a)
-#pragma unroll
-            for (int c = 0; c < kSweepBlock; ++c) Lc[a][c] = A[pr[a] * lda + (kb + c < n ? kb + c : n - 1)];
-#pragma unroll
-        for (int c = 0; c < kSweepBlock; ++c) {
-            const int k = kb + c;
-            if (k < n) {
-                const double yk = rdlane(sel3(r, k >> 6), k & 63);
-#pragma unroll
-                for (int a = 0; a < 3; ++a) {
-                    const int i = lane + 64 * a;
-                    const double upd = fma(-(double)Lc[a][c], yk, r[a]);
-                    r[a] = ((i > k) & (i < n)) ? upd : r[a];
                 }
             }
         }
@@ -1072,33 +1385,82 @@ __device__ __forceinline__ void back_substitute_cols(const TT* A, int n, int lda
     }
 }
 
-// Triangular solves with the LDS factors (logical row i at physical row phys_of[i]), FP64
-// accumulation: forward substitution by wave 0, then the blocked back substitution of the
-// first solve (all waves; the same values as a column-oriented sweep).  b: LDS, original row
-// order, read through phys_of (not modified); x -> xs.
+// Forward substitution L y = r (unit L; logical row i at physical row phys_of[i]) on xs (FP64,
+// logical order, in place), in blocks of 16 rows from the top, mirroring back_substitute: once a
+// block's y is known, wave 0 updates the rows of the next block with it and solves that block
+// (lane <-> row, y_k to the block's lanes by readlane), while the other waves update all rows
+// further down (thread <-> row); one barrier per block.  Each row receives r_i -= l_ik y_k for
+// k ascending, as in the column-oriented loop.
 template <typename TT>
-__device__ __forceinline__ void lu_solve(const TT* A, int n, int lda, const int* phys_of, const double* b, double* xs) {
-    const int tid = threadIdx.x;
-    KYN_SPAN_BEGIN(t_fw);
-    if (tid < 64) {
-        const int lane = tid;
-        double r[3];
-        int pr[3];
-        // P b: the factorization's row interchanges (k <-> piv[k], k ascending) composed on the
-        // identity are phys_of, so entry i of the interchanged b is b[phys_of[i]]: one gather
-        // instead of n sequential swaps
+__device__ __forceinline__ double fwd_block_solve(const TT* A, int lda, const int* phys_of, int kb, int nb, double r,
+                                                  int lane) {
+    const bool mine = lane < nb;
+    const int pr = phys_of[kb + (mine ? lane : 0)];
+    TT l[kBackBlock];
 #pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            const int i = lane + 64 * a;
-            pr[a] = i < n ? phys_of[i] : 0;
-            r[a] = i < n ? b[pr[a]] : 0.;
+    for (int c = 0; c < kBackBlock; ++c) l[c] = A[pr * lda + kb + (c < nb ? c : 0)];
+#pragma unroll
+    for (int c = 0; c < kBackBlock; ++c) {
+        if (c < nb) {  // uniform
+            const double yk = rdlane(r, c);
+            const double upd = fma(-(double)l[c], yk, r);
+            r = lane > c ? upd : r;
         }
-        forward_sweep<TT>(A, n, lda, pr, r);
+    }
+    return r;
+}
+// row i -= its entries in block columns kb .. kb + 15 times y, columns ascending
+template <typename TT>
+__device__ __forceinline__ double fwd_row_update(const TT* A, int lda, const int* phys_of, int i, int kb,
+                                                 const double* xs, double r) {
+    const TT* Ar = A + phys_of[i] * lda + kb;
+    TT l[kBackBlock];
+    double yk[kBackBlock];
 #pragma unroll
-        for (int a = 0; a < 3; ++a)
-            if (lane + 64 * a < n) xs[lane + 64 * a] = r[a];
+    for (int c = 0; c < kBackBlock; ++c) {  // loads first, one wait
+        l[c] = Ar[c];
+        yk[c] = xs[kb + c];
+    }
+#pragma unroll
+    for (int c = 0; c < kBackBlock; ++c) r = fma(-(double)l[c], yk[c], r);
+    return r;
+}
+template <typename TT>
+__device__ __forceinline__ void forward_substitute(const TT* A, int n, int lda, const int* phys_of, double* xs) {
+    const int tid = threadIdx.x, lane = tid & 63;
+    int nb = n < kBackBlock ? n : kBackBlock;
+    if (tid < 64) {
+        const double r = fwd_block_solve<TT>(A, lda, phys_of, 0, nb, lane < nb ? xs[lane] : 0., lane);
+        if (lane < nb) xs[lane] = r;
     }
     __syncthreads();
+    for (int kb = 0; kb + kBackBlock < n; kb += kBackBlock) {  // block kb is solved: apply it below
+        const int kn = kb + kBackBlock;
+        const int nbn = (n - kn) < kBackBlock ? (n - kn) : kBackBlock;
+        if (tid < 64) {
+            double r = 0.;
+            if (lane < nbn) r = fwd_row_update<TT>(A, lda, phys_of, kn + lane, kb, xs, xs[kn + lane]);
+            r = fwd_block_solve<TT>(A, lda, phys_of, kn, nbn, r, lane);
+            if (lane < nbn) xs[kn + lane] = r;
+        } else {
+            for (int i = kn + kBackBlock + tid - 64; i < n; i += blockDim.x - 64)
+                xs[i] = fwd_row_update<TT>(A, lda, phys_of, i, kb, xs, xs[i]);
+        }
+        __syncthreads();
+    }
+}
+
+// Triangular solves with the LDS factors (logical row i at physical row phys_of[i]), FP64
+// accumulation, both blocked over all waves.  b: LDS, original row order, read through phys_of
+// (not modified); x -> xs.
+template <typename TT>
+__device__ __forceinline__ void lu_solve(const TT* A, int n, int lda, const int* phys_of, const double* b, double* xs) {
+    KYN_SPAN_BEGIN(t_fw);
+    // P b: the factorization's row interchanges (k <-> piv[k], k ascending) composed on the
+    // identity are phys_of, so entry i of the interchanged b is b[phys_of[i]]: one gather
+    for (int i = threadIdx.x; i < n; i += blockDim.x) xs[i] = b[phys_of[i]];
+    __syncthreads();
+    forward_substitute<TT>(A, n, lda, phys_of, xs);
     KYN_SPAN_END(6, t_fw, 0);
     back_substitute<TT>(A, n, lda, phys_of, xs, false);
 }
@@ -1439,9 +1801,16 @@ __device__ __forceinline__ void factor_solve_uncondition(const DevSys& S, int b,
     for (int i = t; i < n; i += blockDim.x) A[i * lda + n] = (TT)rhs[i];
     __syncthreads();
     const LuScratch W{reinterpret_cast<int*>(sm + L.phys_of), reinterpret_cast<void*>(sm + L.lcol),
-                      reinterpret_cast<int*>(sm + L.lu_sync)};
+                      reinterpret_cast<int*>(sm + L.lu_sync), S.lu_wait_limit, S.dev_error};
     KYN_PHASE_MARK(7);
-    lu_factor_augmented<TT>(A, n, lda, piv, W);
+    if constexpr (sizeof(TT) == 4) {
+        if (S.block_nopiv[b])
+            lu_factor_nopiv_f32(A, n, lda, W);
+        else
+            lu_factor_augmented<TT>(A, n, lda, piv, W);
+    } else {
+        lu_factor_augmented<TT>(A, n, lda, piv, W);
+    }
     KYN_PHASE_MARK(8);
     back_substitute<TT>(A, n, lda, W.phys_of, xs);
     KYN_PHASE_MARK(9);
@@ -1511,7 +1880,12 @@ __device__ __forceinline__ void fused_step_body(const DevSys& S, int Nm, int Qm,
     const double ndofs = (double)S.model_ndofs[m];
     int converged = 0;
     int iter = 0;
+    double err_prev = __builtin_inf();  // convergence error of the previous Newton iteration
     for (; iter < p.max_iter; ++iter) {
+        // FP64 refinement of the FP32 solve where it decides the final state: the Newton
+        // increments contract by ~20x per iteration, so a solve error matters only in the last
+        // iterations, whose predecessors' errors are small (an inexact-Newton forcing rule)
+        const bool refine_now = S.refine_sweeps > 0 && (S.refine_err_bound <= 0. || err_prev < S.refine_err_bound);
         double err_sum = 0.;
         for (int b = S.model_block_begin[m]; b < S.model_block_end[m]; ++b) {
             const int n = S.block_n[b];
@@ -1528,21 +1902,19 @@ __device__ __forceinline__ void fused_step_body(const DevSys& S, int Nm, int Qm,
             condition_rhs(S, b, rhs);
             KYN_PHASE_MARK(6);
             // unfactored copy for refinement: row-major, row stride (n + 3) & ~3, at the block's own offset
-            TT* Ac = (S.Acopy != nullptr && S.refine_sweeps > 0) ? reinterpret_cast<TT*>(S.Acopy) + S.block_C_off[b]
-                                                                 : nullptr;
+            TT* Ac = (S.Acopy != nullptr && refine_now) ? reinterpret_cast<TT*>(S.Acopy) + S.block_C_off[b] : nullptr;
             factor_solve_uncondition<TT>(S, b, A, Ac, L.lda, piv, rhs, xs, xs2, sm, L);
             KYN_PHASE_MARK(10);
             err_sum += block_error_and_update(S, b, xs, red);
             KYN_PHASE_MARK(11);
         }
         const double err = sqrt(err_sum / ndofs);
+        err_prev = err;
         if (t == 0) S.err_hist[(size_t)m * p.max_iter + iter] = err;
-        if (err <= 1.) {
+        // the reference's loop test `err > 1.` (step.hpp:54): a NaN error also ends the loop,
+        // and the step then counts as converged
+        if (!(err > 1.)) {
             converged = 1;
-            ++iter;
-            break;
-        }
-        if (isnan(err)) {  // the reference's `err > 1` loop test would end here as converged
             ++iter;
             break;
         }
@@ -1622,7 +1994,8 @@ __global__ void __launch_bounds__(kThreads) k_plain_element(DevSys S, int Nm, in
     const int e = blockIdx.x;
     const int m = S.block_model[S.node_block[S.elem_nodes[S.elem_node_off[e]]]];
     if (!S.model_active[m]) return;
-    element_phase<double, false>(S, e, sm, L, 0);
+    element_phase<double, false>(S, e, sm, L, 0, S.elem_res + (size_t)S.elem_node_off[e] * 6,
+                                 S.elem_mat + S.elem_mat_off[e]);
 }
 
 // UpdateSystemVariablesMasses / UpdateSystemVariablesSprings
@@ -2002,7 +2375,7 @@ __device__ __forceinline__ void cond_factor_body(DevSys& S, const double* rin) {
         M[i * lda + j] = (TT)v;
     }
     __syncthreads();
-    const LuScratch W{ph, lc, sync};
+    const LuScratch W{ph, lc, sync, S.lu_wait_limit, S.dev_error};
     lu_factor_augmented<TT>(M, ne, lda, piv, W, nrhs);
     double* Y = S.Ybuf + S.cond_Y_off[ce];
     back_substitute_cols<TT>(M, ne, lda, ph, nrhs, Y, ne);
@@ -2124,10 +2497,10 @@ __global__ void __launch_bounds__(kThreads) k_plain_update(DevSys S, int iter) {
     if (t == 0) {
         S.err_hist[(size_t)m * S.p.max_iter + iter] = err;
         S.model_iters[m] = iter + 1;
-        if (err <= 1.) {
+        if (!(err > 1.)) {  // step.hpp:54: a NaN error ends the loop as converged
             S.model_status[m] = 0;
             S.model_active[m] = 0;
-        } else if (iter + 1 >= S.p.max_iter || isnan(err)) {
+        } else if (iter + 1 >= S.p.max_iter) {
             S.model_status[m] = 1;
             S.model_active[m] = 0;
         } else {

@@ -51,6 +51,8 @@ struct DevSys {
     int num_models, num_blocks, num_elems, num_nodes, num_cons;
     int precision;      // 0 FP64 tangent, 1 FP32 tangent
     int refine_sweeps;  // FP64 iterative refinement sweeps (FP32 tangent)
+    double refine_err_bound;  // refine only in Newton iterations whose predecessor's convergence
+                              // error is below this (<= 0: in every iteration)
     int max_block_cons; // most constraints in one block (fused LDS layout)
 
     // ---- state (src/state/state.hpp), [n][7] / [n][6]
@@ -88,6 +90,9 @@ struct DevSys {
     // its rotational rows in the block are held at x = 0 by a unit diagonal and a zero rhs
     const int* node_pin;
     const int* block_npin;        // pinned nodes in the block (0: nothing to do)
+    // 1: the FP32-tangent factorization of the block takes no row interchanges (dynamic solve,
+    // every node on a beam element, no translations-only nodes, no masses or springs)
+    const int* block_nopiv;
 
     // ---- elements (src/elements/beams/beams.hpp)
     const int *elem_N, *elem_Q;
@@ -179,6 +184,15 @@ struct DevSys {
     int sched_num_loaded;
     int sched_steps;
     int* sched_slot;             // device counter: schedule row of the next step
+
+    // ---- bounded waits: a wave that waits on another (the blocked LU's look-ahead counter)
+    //      gives up after lu_wait_limit polls (< 0: at its first wait), sets bit 0 of
+    //      dev_error and goes on, so the kernel still ends; the host reports the flag
+    int lu_wait_limit;
+    int* dev_error;
 };
+
+constexpr int kLuWaitLimitDefault = 1 << 20;  // polls of ~64 cycles: ~28 ms at 2.4 GHz
+constexpr int kDevErrLuWait = 1;
 
 }  // namespace kyn

@@ -28,13 +28,16 @@ def main():
     ap.add_argument("--rotors", type=int, default=512)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--refine", type=int, default=0)
+    ap.add_argument("--refine-bound", type=float, default=0.0,
+                    help="refine only Newton iterations whose previous convergence error is below this (0: all)")
     args = ap.parse_args()
     lib = KynLib(LIB)
     lib.L.kyn_phase_read.argtypes = [C.POINTER(C.c_ulonglong)]
     specs = W.ensemble_config_specs(num_rotors=args.rotors, num_steps=args.steps + 1)
     cd, tips, loads = W.ensemble_schedule(specs, args.steps + 1)
     prec = TANGENT_FP32 if args.precision == "fp32" else TANGENT_FP64
-    s = KynSystem(lib, specs, exec_mode=EXEC_FUSED, precision=prec, refine_sweeps=args.refine)
+    s = KynSystem(lib, specs, exec_mode=EXEC_FUSED, precision=prec, refine_sweeps=args.refine,
+                  refine_err_bound=args.refine_bound)
     s.set_schedule(args.steps + 1, cd, tips, loads)
     s.step_many(1, use_graph=False)
     it0, _ = s.totals()
@@ -47,7 +50,7 @@ def main():
     blocks_per_model = np.array([len(sp.beams) for sp in specs])
     count = int(((it1 - it0) * blocks_per_model).sum())
     total = 0.0
-    print(f"{args.precision} refine {args.refine}, {args.rotors} rotors x {args.steps} steps, "
+    print(f"{args.precision} refine {args.refine} (bound {args.refine_bound:g}), {args.rotors} rotors x {args.steps} steps, "
           f"{count} block-iterations, step kernel {s.last_kernel_ms():.3f} ms")
     for i, name in enumerate(PHASES, start=1):
         c = ((a[i] - a[i - 1]) % (1 << 64)) / count

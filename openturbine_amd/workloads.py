@@ -96,6 +96,17 @@ def quat_from_rotvec(rv) -> np.ndarray:
     return np.array([math.cos(a / 2.0), rv[0] * s, rv[1] * s, rv[2] * s])
 
 
+def rotvec_from_quat(q) -> np.ndarray:
+    """Rotation vector (axis times angle in [0, pi]) of a unit quaternion {w, x, y, z}."""
+    q = np.asarray(q, dtype=np.float64)
+    if q[0] < 0.0:
+        q = -q
+    s = float(np.sqrt(q[1:] @ q[1:]))
+    if s < 1e-300:
+        return 2.0 * q[1:]
+    return (2.0 * math.atan2(s, q[0]) / s) * q[1:]
+
+
 def quat_compose(a, b) -> np.ndarray:
     return np.array([
         a[0] * b[0] - a[1] * b[1] - a[2] * b[2] - a[3] * b[3],
@@ -589,7 +600,7 @@ def rotor_config_spec(num_nodes=11, num_qps=20, length=60.0, omega=1.0, hub_radi
 
 
 def _rotor_spec(name, x0, u, v, vd, s, beams, cons, roots, gravity, h, num_steps, rho_inf,
-                max_iter, atol, rtol, tip_load_sigma, seed, rotor_of_root=None):
+                max_iter, atol, rtol, tip_load_sigma, seed, rotor_of_root=None, tip_load_tau=0.0):
     root_pos = {cid: x0[node, :3].copy() for cid, node, _, _ in roots}
     n = x0.shape[0]
     tips = np.array([tip for _, _, tip, _ in roots])
@@ -604,13 +615,34 @@ def _rotor_spec(name, x0, u, v, vd, s, beams, cons, roots, gravity, h, num_steps
             out[cid] = np.concatenate([quat_rotate(q, p) - p, q])
         return out
 
+    def white(step):
+        nb = len(tips)
+        keys = (np.uint64(step) * np.uint64(nb) + np.arange(nb, dtype=np.uint64)) * np.uint64(3)
+        return tip_load_sigma * normal(seed + 7, keys[:, None] + np.arange(3, dtype=np.uint64)[None, :])
+
+    # tip_load_tau > 0: f_k = rho f_(k-1) + sqrt(1 - rho^2) w_k with rho = exp(-h / tau) and w_k
+    # white normal(0, sigma): every step's load is still normal(0, sigma), successive steps are
+    # correlated over tau seconds.  Values are generated once, in step order, and cached.
+    ar = {"f": np.zeros((0, len(tips), 3))}
+    rho = math.exp(-h / tip_load_tau) if tip_load_tau > 0.0 else 0.0
+
+    def tip_values(step):
+        if tip_load_tau <= 0.0:
+            return white(step)
+        f = ar["f"]
+        if step >= f.shape[0]:
+            k0 = f.shape[0]
+            ext = np.empty((max(step + 1, 2 * k0), len(tips), 3))
+            ext[:k0] = f
+            for k in range(k0, ext.shape[0]):
+                ext[k] = white(k) if k == 0 else rho * ext[k - 1] + math.sqrt(1.0 - rho * rho) * white(k)
+            ar["f"] = f = ext
+        return f[step]
+
     def tip_loads(step):
         f = np.zeros((n, 6))
         if tip_load_sigma > 0.0:
-            nb = len(tips)
-            keys = (np.uint64(step) * np.uint64(nb) + np.arange(nb, dtype=np.uint64)) * np.uint64(3)
-            vals = tip_load_sigma * normal(seed + 7, keys[:, None] + np.arange(3, dtype=np.uint64)[None, :])
-            f[tips, :3] = vals
+            f[tips, :3] = tip_values(step)
         return f
 
     def step_inputs(step, _state=None):
@@ -629,14 +661,18 @@ def _rotor_spec(name, x0, u, v, vd, s, beams, cons, roots, gravity, h, num_steps
 
 def ensemble_config_specs(num_rotors=4096, h=0.01, num_steps=100, rho_inf=0.9, max_iter=50,
                           atol=1e-10, rtol=1e-6, seed=SEED, tip_load_sigma=1e4, hub_radius=2.0,
-                          gravity=(0.0, 0.0, -9.81), first_rotor=0):
+                          gravity=(0.0, 0.0, -9.81), first_rotor=0, tip_load_tau=0.1):
     """The batched ensemble: independent 3-blade rotors, per-blade N uniform in
     {6, 11, 16, 21}, Q = 2N-2, L uniform in [30, 120] m, omega uniform in [0.5, 1.5] rad/s,
     per-step tip loads normal(0, tip_load_sigma) N.  Sections are the config-1 base section
     scaled geometrically to the blade (s = L / 10 m: mass, EA, GA x s^2; rotary inertia, GJ,
     EI x s^4): with unscaled base sections a 120 m blade sags ~127 m under gravity and the
-    reference's Newton iteration diverges, and the random tip loads drive the light tips
-    unstable.  Returns one ModelSpec per rotor."""
+    reference's Newton iteration diverges.  The tip loads are correlated from step to step over
+    tip_load_tau seconds (an AR(1) sequence whose every value is normal(0, tip_load_sigma)):
+    loads drawn afresh each step (tip_load_tau = 0) keep pumping energy into the high modes of
+    the light GLL tip nodes until the reference's Newton iteration diverges (the oracle: 7 of
+    the first 64 rotors within 150 steps, the first at step 75; none with tau = 0.1 s).
+    Returns one ModelSpec per rotor."""
     specs = []
     for r in range(first_rotor, first_rotor + num_rotors):
         keys = np.uint64(r) * np.uint64(8) + np.arange(8, dtype=np.uint64)
@@ -652,7 +688,7 @@ def ensemble_config_specs(num_rotors=4096, h=0.01, num_steps=100, rho_inf=0.9, m
         x0, uu, v, vd, s = nt.arrays()
         spec = _rotor_spec(f"ensemble_rotor_{r}", x0, uu, v, vd, s, beams, cons, roots,
                            np.array(gravity), h, num_steps, rho_inf, max_iter, atol, rtol,
-                           tip_load_sigma, seed=seed + 1000003 * (r + 1))
+                           tip_load_sigma, seed=seed + 1000003 * (r + 1), tip_load_tau=tip_load_tau)
         spec.meta["rotor"] = r
         spec.meta["blade_nodes"] = Ns
         specs.append(spec)

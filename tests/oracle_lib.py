@@ -12,6 +12,8 @@ import numpy as np
 
 ROOT = Path(__file__).resolve().parents[1]
 LIB_PATH = ROOT / "oracle" / "liboracle_kyn.so"
+# the same sources built with -ffp-contract=off: differs from LIB_PATH only in rounding
+NC_LIB_PATH = ROOT / "oracle" / "liboracle_kyn_nocontract.so"
 
 _dp = C.POINTER(C.c_double)
 _sp = C.POINTER(C.c_size_t)
