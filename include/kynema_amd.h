@@ -210,6 +210,14 @@ int kyn_debug_assemble(kyn_system s);
 int kyn_debug_set_wait_limit(kyn_system s, int polls);
 int kyn_get_element_terms(kyn_system s, size_t element, double* residual /* [N][6] */,
                           double* system_matrix /* [N][N][36] */);
+/* No counterpart in the reference: for callers that cannot allocate device memory (HIP graph
+ * capture, serving loops).  One beam element's FP64 residual [N][6] and iteration matrix
+ * [N][N][36] at the system's current state -- the values kyn_debug_assemble followed by
+ * kyn_get_element_terms gives at that state -- formed in a file-scope device array of the
+ * library (one per GPU, 128 016 bytes: N <= 21, larger elements return KYN_ERR_INVALID) and
+ * copied to the caller's buffers (host or device memory, NULL to skip).  Allocates nothing.
+ * The system's device must be current; calls that overlap on one GPU race for the array. */
+int kyn_element_terms_noalloc(kyn_system s, size_t element, double* residual, double* system_matrix);
 
 /* ---- model inspection (host only) ---- */
 size_t kyn_model_num_nodes(kyn_model m);

@@ -173,6 +173,7 @@ class KynLib:
         L.kyn_debug_assemble.argtypes = [C.c_void_p]
         L.kyn_debug_set_wait_limit.argtypes = [C.c_void_p, C.c_int]
         L.kyn_get_element_terms.argtypes = [C.c_void_p, C.c_size_t, _dp, _dp]
+        L.kyn_element_terms_noalloc.argtypes = [C.c_void_p, C.c_size_t, _dp, _dp]
         L.kyn_get_motion.argtypes = [C.c_void_p, _dp]
         L.kyn_model_num_nodes.argtypes = [C.c_void_p]
         L.kyn_model_num_nodes.restype = C.c_size_t

@@ -45,6 +45,8 @@ void launch_f64_to_f32(const double* x, float* y, size_t count, hipStream_t st);
 void launch_pack_motion(const DevSys& S, double* out, hipStream_t st);
 void launch_add_f32_to_f64(const float* d, double* x, int count, hipStream_t st);
 void launch_plain_epilogue(const DevSys& S, hipStream_t st);
+hipError_t element_terms_scratch(int device, double** ptr);
+void launch_element_terms(const DevSys& S, int Nm, int Qm, int e, double* res, double* mat, hipStream_t st);
 }  // namespace kyn
 
 namespace {
@@ -221,6 +223,7 @@ struct kyn_system_s {
     int exec_mode{KYN_EXEC_FUSED};
     int64_t mat_total{0}, A_total{0};
     int Nm{1}, Qm{1}, nm{1};
+    int device{0};  // the HIP device current at creation
     float last_step_ms{0.f}, last_kernel_ms{0.f};
     // host copies of the layout
     size_t num_models{}, num_nodes{}, num_cons{}, num_elems{}, num_blocks{};
@@ -1006,6 +1009,7 @@ kyn_system kyn_system_create(const kyn_model* models, size_t num_models, const k
         sys->nm = nm;
         int dev = 0;
         hip_check(hipGetDevice(&dev), "hipGetDevice");
+        sys->device = dev;
         int max_lds = 0;
         hip_check(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev), "attr");
         size_t need = 0;
@@ -1833,6 +1837,34 @@ int kyn_get_element_terms(kyn_system s, size_t element, double* residual, double
             hip_check(hipMemcpy(system_matrix, s->S.elem_mat + moff, N * N * 36 * sizeof(double), hipMemcpyDeviceToHost),
                       "D2H");
         }
+        return KYN_OK;
+    });
+}
+
+int kyn_element_terms_noalloc(kyn_system s, size_t element, double* residual, double* system_matrix) {
+    return guarded([&] {
+        if (!s || element >= s->num_elems) throw KynError(KYN_ERR_INVALID, "bad element");
+        const int N = s->elem_N[element];
+        if (N > kyn::kMaxElemNodes)
+            throw KynError(KYN_ERR_INVALID, "element has " + std::to_string(N) + " nodes; the no-allocation element "
+                                            "terms hold at most " + std::to_string(kyn::kMaxElemNodes));
+        int cur = -1;
+        hip_check(hipGetDevice(&cur), "hipGetDevice");
+        if (cur != s->device)
+            throw KynError(KYN_ERR_INVALID, "make the system's device (" + std::to_string(s->device) + ") current first");
+        double* scr = nullptr;
+        hip_check(kyn::element_terms_scratch(s->device, &scr), "hipGetSymbolAddress(scratch)");
+        double* res_d = scr;
+        double* mat_d = scr + kyn::kMaxElemNodes * 6;
+        kyn::launch_element_terms(s->S, s->Nm, s->Qm, static_cast<int>(element), res_d, mat_d, s->stream);
+        hip_check(hipGetLastError(), "k_element_terms launch");
+        const size_t n = static_cast<size_t>(N);
+        if (residual)
+            hip_check(hipMemcpyAsync(residual, res_d, n * 6 * sizeof(double), hipMemcpyDefault, s->stream), "copy");
+        if (system_matrix)
+            hip_check(hipMemcpyAsync(system_matrix, mat_d, n * n * 36 * sizeof(double), hipMemcpyDefault, s->stream),
+                      "copy");
+        hip_check(hipStreamSynchronize(s->stream), "sync");
         return KYN_OK;
     });
 }

@@ -20,6 +20,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <mutex>
 
 #include "kyn_device.hpp"
 #include "kyn_system.hpp"
@@ -1998,6 +1999,14 @@ __global__ void __launch_bounds__(kThreads) k_plain_element(DevSys S, int Nm, in
                                  S.elem_mat + S.elem_mat_off[e]);
 }
 
+// One element's residual and iteration matrix at the current state, with its own tangent
+// blocks (the no-allocation element-terms entry point): res [N][6], mat [N][N][36].
+__global__ void __launch_bounds__(kThreads) k_element_terms(DevSys S, int Nm, int Qm, int e, double* res, double* mat) {
+    extern __shared__ __align__(16) char sm[];
+    const Lay L = make_layout(Nm, Qm, 1, 0, false, 8, false);
+    element_phase<double, false, true>(S, e, sm, L, 0, res, mat);
+}
+
 // UpdateSystemVariablesMasses / UpdateSystemVariablesSprings
 // (src/step/update_system_variables_masses.hpp, update_system_variables_springs.hpp):
 // one thread per point element, masses first
@@ -2538,6 +2547,8 @@ hipError_t set_lds_limits(size_t bytes) {
     if (e != hipSuccess) return e;
     e = hipFuncSetAttribute((const void*)k_plain_element, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
     if (e != hipSuccess) return e;
+    e = hipFuncSetAttribute((const void*)k_element_terms, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (e != hipSuccess) return e;
     e = hipFuncSetAttribute((const void*)k_cond_factor_f64, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
     if (e != hipSuccess) return e;
     e = hipFuncSetAttribute((const void*)k_cond_factor_f32, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
@@ -2552,6 +2563,40 @@ void launch_fused_step(const DevSys& S, int Nm, int Qm, int nm, hipStream_t st) 
     else
         hipLaunchKernelGGL(k_fused_step_f64, dim3(S.num_models), dim3(kThreads), lds, st, S, Nm, Qm, nm);
 }
+// ---------------------------------------------------------------------------------------
+// Element-terms scratch of the no-allocation entry point: one element's residual [N][6] and
+// iteration matrix [N][N][36] for N <= kMaxElemNodes (128 016 bytes), a file-scope device array,
+// one per GPU, held until the process ends.  Host code never passes the array's name to a
+// kernel or a copy: it uses the device address looked up once per device below.
+// ---------------------------------------------------------------------------------------
+static __device__ __align__(256) double scratch[kElemTermsScratch];
+
+namespace {
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxScratchDevices] = {};
+}  // namespace
+
+hipError_t element_terms_scratch(int device, double** ptr) {
+    if (device < 0 || device >= kMaxScratchDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        int cur = -1;
+        hipError_t e = hipGetDevice(&cur);
+        if (e != hipSuccess) return e;
+        if (cur != device) return hipErrorInvalidDevice;  // the caller makes `device` current
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[device] = static_cast<double*>(p);
+    }
+    *ptr = g_scratch_ptr[device];
+    return hipSuccess;
+}
+
+void launch_element_terms(const DevSys& S, int Nm, int Qm, int e, double* res, double* mat, hipStream_t st) {
+    hipLaunchKernelGGL(k_element_terms, dim3(1), dim3(kThreads), element_lds_bytes(Nm, Qm), st, S, Nm, Qm, e, res, mat);
+}
+
 void launch_advance_slot(const DevSys& S, hipStream_t st) {
     hipLaunchKernelGGL(k_advance_slot, dim3(1), dim3(64), 0, st, S);
 }

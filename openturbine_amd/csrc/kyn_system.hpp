@@ -19,6 +19,9 @@ constexpr int kMaxElemQps = 64;
 constexpr int kThreads = 256;      // workgroup size of every per-element / per-block kernel
 constexpr int kMaxBlockDofs = 143; // register LU tile holds n + 1 <= 144 columns ([A | b])
 constexpr int kCondMaxLam = 16;    // condensed solve: constraint rows touching one element
+// scratch of the no-allocation element-terms entry point: residual [N][6] + matrix [N][N][36]
+constexpr int kElemTermsScratch = kMaxElemNodes * 6 + kMaxElemNodes * kMaxElemNodes * 36;
+constexpr int kMaxScratchDevices = 64;
 
 // Constraint types (src/constraints/constraint_type.hpp:9-21)
 constexpr int kConFixedBC = 1, kConPrescribedBC = 2, kConRigidJoint = 3, kConRevoluteJoint = 4,
