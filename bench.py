@@ -318,8 +318,10 @@ def main():
     ap.add_argument("--skip-plain", action="store_true")
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--refine", type=int, default=1, help="FP64 refinement sweeps of the FP32-tangent solve")
-    ap.add_argument("--refine-bound", type=float, default=1e3,
-                    help="refine only Newton iterations whose previous convergence error is below this (0: all)")
+    ap.add_argument("--refine-bound", type=float, default=0.0,
+                    help="refine only Newton iterations whose previous convergence error is below this (0: all; "
+                         "the default: with a bound of 1e3 the worst of 512 rotors ends 3.5e-9 from the oracle "
+                         "after 100 steps, with every iteration refined 2.8e-10)")
     ap.add_argument("--skip-unrefined", action="store_true", help="skip the FP32 leg without refinement")
     ap.add_argument("--structure-steps", type=int, default=10, help="steps of the large-structure legs (0: skip)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",

@@ -174,6 +174,16 @@ __device__ __forceinline__ float r_entry_rot(const float* M6, bool t, int sp, fl
     return __builtin_fmaf(pj, za, dj * zb);
 }
 
+// Column ecol of one half of the R operand -> node j, row r and column s' of the 6 x 6 block;
+// padded columns (ecol >= 18 N) map to the last column (their products are never written out).
+__device__ __forceinline__ void mfma_column(int ecol, int N, int& j, int& r, int& sp) {
+    const int ec = ecol < 18 * N ? ecol : 18 * N - 1;
+    j = ec / 18;
+    const int rem = ec - 18 * j;
+    r = rem / 3;
+    sp = rem - 3 * r;
+}
+
 template <bool kFused>
 __device__ __forceinline__ void contract_mfma_f32(const DevSys& S, char* sm, const Lay& L, int n, int N, int Q,
                                                   const int* loc, double* mat_out) {
@@ -186,7 +196,6 @@ __device__ __forceinline__ void contract_mfma_f32(const DevSys& S, char* sm, con
     const float* phiT = reinterpret_cast<const float*>(sm + L.phiT);
     const double* tr = reinterpret_cast<const double*>(sm + L.tr);
     const int TH = (18 * Nu + 15) >> 4;
-    const int ntiles = 2 * TH;
     const bool two_rows = Nu > 16;
     const int npairs = Qu >> 1;  // QP pairs with both QPs present; an odd Q adds one with QP Q - 1 only
     // A operand: row li (and 16 + li) of L at column k: phi (t = 0) or phi' (t = 1) of the lane's QP
@@ -205,53 +214,94 @@ __device__ __forceinline__ void contract_mfma_f32(const DevSys& S, char* sm, con
 #pragma unroll
         for (int b = 0; b < 2; ++b) acc[s][b] = f32x4{0.f, 0.f, 0.f, 0.f};
 
+    // Tiles in pairs within one half: tile pair (s', s' + 1) of half h runs one QP loop for
+    // both tiles -- two independent MFMA chains per QP pair, the loads of one tile overlapping
+    // the other's arithmetic.  acc[h * kTilesPerHalf + s'] holds tile wv + 4 s' of half h.
+    constexpr int kTilesPerHalf = kMfmaTilesPerWave / 2;
 #pragma unroll
-    for (int s = 0; s < kMfmaTilesPerWave; ++s) {
-        const int T = wv + 4 * s;
-        if (T < ntiles) {
-            const int h = T >= TH ? 1 : 0;
-            const int ecol = 16 * (T - h * TH) + li;
-            // padded columns (ecol >= 18 N) form the entry of the last column; discarded at write-out
-            const int ec = ecol < 18 * Nu ? ecol : 18 * Nu - 1;
-            const int j = ec / 18, rem = ec - 18 * j, r_ = rem / 3, sp = rem - 3 * r_;
-            f32x4 c0 = acc[s][0], c1 = acc[s][1];
-            if (h == 0) {
-                for (int ks = 0; ks < npairs; ++ks) {
-                    const float* ph = phq + ks * (2 * kPhiStride);
-                    const float R = r_entry_trans(M6q + ks * 432 + r_ * 6, t, sp, ph[2 * j], ph[2 * j + 1]);
-                    const float a1 = ph[ib];
-                    c0 = __builtin_amdgcn_mfma_f32_16x16x4f32(ph[ia], R, c0, 0, 0, 0);
-                    if (two_rows) c1 = __builtin_amdgcn_mfma_f32_16x16x4f32(row2 ? a1 : 0.f, R, c1, 0, 0, 0);
+    for (int h = 0; h < 2; ++h) {
+#pragma unroll
+        for (int s2 = 0; s2 < kTilesPerHalf; s2 += 2) {
+            const int TA = wv + 4 * s2;  // the pair's second tile, TA + 4, may lie past the half:
+            if (TA < TH) {               // it then forms clamped columns that are never written out
+                const int sA = h * kTilesPerHalf + s2;
+                int jA, rA, pA, jB, rB, pB;
+                mfma_column(16 * TA + li, Nu, jA, rA, pA);
+                mfma_column(16 * (TA + 4) + li, Nu, jB, rB, pB);
+                f32x4 a0 = acc[sA][0], a1 = acc[sA][1], b0 = acc[sA + 1][0], b1 = acc[sA + 1][1];
+                if (h == 0) {
+                    for (int ks = 0; ks < npairs; ++ks) {
+                        const float* ph = phq + ks * (2 * kPhiStride);
+                        const float* M6 = M6q + ks * 432;
+                        const float RA = r_entry_trans(M6 + rA * 6, t, pA, ph[2 * jA], ph[2 * jA + 1]);
+                        const float RB = r_entry_trans(M6 + rB * 6, t, pB, ph[2 * jB], ph[2 * jB + 1]);
+                        const float x0 = ph[ia], y1 = ph[ib];
+                        const float x1 = row2 ? y1 : 0.f;
+                        a0 = __builtin_amdgcn_mfma_f32_16x16x4f32(x0, RA, a0, 0, 0, 0);
+                        b0 = __builtin_amdgcn_mfma_f32_16x16x4f32(x0, RB, b0, 0, 0, 0);
+                        if (two_rows) {
+                            a1 = __builtin_amdgcn_mfma_f32_16x16x4f32(x1, RA, a1, 0, 0, 0);
+                            b1 = __builtin_amdgcn_mfma_f32_16x16x4f32(x1, RB, b1, 0, 0, 0);
+                        }
+                    }
+                    if (tail) {
+                        const float* ph = phiT + qt * kPhiStride;
+                        const float* M6 = qpM + qt * 216;
+                        const float RA = r_entry_trans(M6 + rA * 6, t, pA, ph[2 * jA], ph[2 * jA + 1]);
+                        const float RB = r_entry_trans(M6 + rB * 6, t, pB, ph[2 * jB], ph[2 * jB + 1]);
+                        const float y0 = ph[ia], y1 = ph[ib];
+                        const float x0 = tail_valid ? y0 : 0.f, x1 = tail_valid && row2 ? y1 : 0.f;
+                        a0 = __builtin_amdgcn_mfma_f32_16x16x4f32(x0, RA, a0, 0, 0, 0);
+                        b0 = __builtin_amdgcn_mfma_f32_16x16x4f32(x0, RB, b0, 0, 0, 0);
+                        if (two_rows) {
+                            a1 = __builtin_amdgcn_mfma_f32_16x16x4f32(x1, RA, a1, 0, 0, 0);
+                            b1 = __builtin_amdgcn_mfma_f32_16x16x4f32(x1, RB, b1, 0, 0, 0);
+                        }
+                    }
+                } else {
+                    // column s' of node j's rotational tangent block, per tile
+                    const double* TjA = tr + jA * 9 + pA;
+                    const double* TjB = tr + jB * 9 + pB;
+                    const float tA0 = (float)TjA[0], tA1 = (float)TjA[3], tA2 = (float)TjA[6];
+                    const float tB0 = (float)TjB[0], tB1 = (float)TjB[3], tB2 = (float)TjB[6];
+                    for (int ks = 0; ks < npairs; ++ks) {
+                        const float* ph = phq + ks * (2 * kPhiStride);
+                        const float* M6 = M6q + ks * 432;
+                        const float RA =
+                            r_entry_rot(M6 + rA * 6, t, pA, ph[2 * jA], ph[2 * jA + 1], tA0, tA1, tA2);
+                        const float RB =
+                            r_entry_rot(M6 + rB * 6, t, pB, ph[2 * jB], ph[2 * jB + 1], tB0, tB1, tB2);
+                        const float x0 = ph[ia], y1 = ph[ib];
+                        const float x1 = row2 ? y1 : 0.f;
+                        a0 = __builtin_amdgcn_mfma_f32_16x16x4f32(x0, RA, a0, 0, 0, 0);
+                        b0 = __builtin_amdgcn_mfma_f32_16x16x4f32(x0, RB, b0, 0, 0, 0);
+                        if (two_rows) {
+                            a1 = __builtin_amdgcn_mfma_f32_16x16x4f32(x1, RA, a1, 0, 0, 0);
+                            b1 = __builtin_amdgcn_mfma_f32_16x16x4f32(x1, RB, b1, 0, 0, 0);
+                        }
+                    }
+                    if (tail) {
+                        const float* ph = phiT + qt * kPhiStride;
+                        const float* M6 = qpM + qt * 216;
+                        const float RA =
+                            r_entry_rot(M6 + rA * 6, t, pA, ph[2 * jA], ph[2 * jA + 1], tA0, tA1, tA2);
+                        const float RB =
+                            r_entry_rot(M6 + rB * 6, t, pB, ph[2 * jB], ph[2 * jB + 1], tB0, tB1, tB2);
+                        const float y0 = ph[ia], y1 = ph[ib];
+                        const float x0 = tail_valid ? y0 : 0.f, x1 = tail_valid && row2 ? y1 : 0.f;
+                        a0 = __builtin_amdgcn_mfma_f32_16x16x4f32(x0, RA, a0, 0, 0, 0);
+                        b0 = __builtin_amdgcn_mfma_f32_16x16x4f32(x0, RB, b0, 0, 0, 0);
+                        if (two_rows) {
+                            a1 = __builtin_amdgcn_mfma_f32_16x16x4f32(x1, RA, a1, 0, 0, 0);
+                            b1 = __builtin_amdgcn_mfma_f32_16x16x4f32(x1, RB, b1, 0, 0, 0);
+                        }
+                    }
                 }
-                if (tail) {
-                    const float* ph = phiT + qt * kPhiStride;
-                    const float R = r_entry_trans(qpM + qt * 216 + r_ * 6, t, sp, ph[2 * j], ph[2 * j + 1]);
-                    const float a0 = ph[ia], a1 = ph[ib];
-                    c0 = __builtin_amdgcn_mfma_f32_16x16x4f32(tail_valid ? a0 : 0.f, R, c0, 0, 0, 0);
-                    if (two_rows)
-                        c1 = __builtin_amdgcn_mfma_f32_16x16x4f32(tail_valid && row2 ? a1 : 0.f, R, c1, 0, 0, 0);
-                }
-            } else {
-                const double* Tj = tr + j * 9 + sp;  // column s' of node j's rotational tangent block
-                const float t0 = (float)Tj[0], t1 = (float)Tj[3], t2 = (float)Tj[6];
-                for (int ks = 0; ks < npairs; ++ks) {
-                    const float* ph = phq + ks * (2 * kPhiStride);
-                    const float R = r_entry_rot(M6q + ks * 432 + r_ * 6, t, sp, ph[2 * j], ph[2 * j + 1], t0, t1, t2);
-                    const float a1 = ph[ib];
-                    c0 = __builtin_amdgcn_mfma_f32_16x16x4f32(ph[ia], R, c0, 0, 0, 0);
-                    if (two_rows) c1 = __builtin_amdgcn_mfma_f32_16x16x4f32(row2 ? a1 : 0.f, R, c1, 0, 0, 0);
-                }
-                if (tail) {
-                    const float* ph = phiT + qt * kPhiStride;
-                    const float R = r_entry_rot(qpM + qt * 216 + r_ * 6, t, sp, ph[2 * j], ph[2 * j + 1], t0, t1, t2);
-                    const float a0 = ph[ia], a1 = ph[ib];
-                    c0 = __builtin_amdgcn_mfma_f32_16x16x4f32(tail_valid ? a0 : 0.f, R, c0, 0, 0, 0);
-                    if (two_rows)
-                        c1 = __builtin_amdgcn_mfma_f32_16x16x4f32(tail_valid && row2 ? a1 : 0.f, R, c1, 0, 0, 0);
-                }
+                acc[sA][0] = a0;
+                acc[sA][1] = a1;
+                acc[sA + 1][0] = b0;
+                acc[sA + 1][1] = b1;
             }
-            acc[s][0] = c0;
-            acc[s][1] = c1;
         }
     }
     __syncthreads();  // every wave done with the element view
@@ -275,10 +325,10 @@ __device__ __forceinline__ void contract_mfma_f32(const DevSys& S, char* sm, con
     }
 #pragma unroll
     for (int s = 0; s < kMfmaTilesPerWave; ++s) {
-        const int T = wv + 4 * s;
-        if (T < ntiles) {
-            const int h = T >= TH ? 1 : 0;
-            const int ecol = 16 * (T - h * TH) + li;
+        const int h = s / kTilesPerHalf;
+        const int T = wv + 4 * (s % kTilesPerHalf);  // tile within half h
+        if (T < TH) {
+            const int ecol = 16 * T + li;
             if (ecol < 18 * Nu) {
                 const int j = ecol / 18, rem = ecol - 18 * j, r_ = rem / 3, col = 3 * h + rem - 3 * r_;
                 if (kFused) {

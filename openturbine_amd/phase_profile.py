@@ -25,13 +25,14 @@ LIB = Path(__file__).resolve().parent / "libkynema_amd_phases.so"
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--precision", choices=["fp32", "fp64"], default="fp32")
+    ap.add_argument("--lib", default=str(LIB), help="profiling library to load (default: the in-tree build)")
     ap.add_argument("--rotors", type=int, default=512)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--refine", type=int, default=0)
     ap.add_argument("--refine-bound", type=float, default=0.0,
                     help="refine only Newton iterations whose previous convergence error is below this (0: all)")
     args = ap.parse_args()
-    lib = KynLib(LIB)
+    lib = KynLib(Path(args.lib))
     lib.L.kyn_phase_read.argtypes = [C.POINTER(C.c_ulonglong)]
     specs = W.ensemble_config_specs(num_rotors=args.rotors, num_steps=args.steps + 1)
     cd, tips, loads = W.ensemble_schedule(specs, args.steps + 1)
