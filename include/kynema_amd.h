@@ -52,7 +52,10 @@ extern "C" {
 
 /* ---- tangent precision ---- */
 #define KYN_TANGENT_FP64 0
-#define KYN_TANGENT_FP32 1 /* FP32 tangent + FP32 LU, FP64 residual/update, FP64 refinement */
+/* FP32 tangent (QP matrices in FP32, node-pair contraction on the matrix cores), FP64 residual,
+ * rotations and update.  Fused form: FP32 LU of each block with FP64 refinement.  Plain form:
+ * small blocks factor in FP64; large blocks factor in FP32 with FP64 refinement. */
+#define KYN_TANGENT_FP32 1
 
 /* ---- state fields for kyn_get_state / kyn_set_state (src/state/state.hpp:29-38) ---- */
 #define KYN_X0 0      /* [n][7] */
@@ -83,9 +86,7 @@ typedef struct {
     int precision;     /* KYN_TANGENT_FP64 or KYN_TANGENT_FP32 */
     int refine_sweeps; /* FP64 iterative-refinement sweeps of the FP32 solve (0..8) */
     double refine_err_bound; /* fused form: refine only in Newton iterations whose previous
-                                convergence error is below this bound (<= 0: every iteration);
-                                the increments contract ~20x per iteration, so the last two
-                                iterations, which set the final state, stay refined */
+                                convergence error is below this bound (<= 0: every iteration) */
 } kyn_exec_options;
 
 /* Last error message of the calling thread (empty if none). */
@@ -210,6 +211,12 @@ int kyn_debug_assemble(kyn_system s);
 int kyn_debug_set_wait_limit(kyn_system s, int polls);
 int kyn_get_element_terms(kyn_system s, size_t element, double* residual /* [N][6] */,
                           double* system_matrix /* [N][N][36] */);
+/* The CRS matrix of a model whose dofs form one large block (plain form), in the reference's
+ * dof order: the graph of Solver's full matrix (src/solver/create_full_matrix.hpp:19-74) and
+ * the values of the last assembly (Solver::A after AssembleSystemMatrix, conditioned).  Returns
+ * the number of nonzeros (0 on error); writes row_ptrs [num_dofs + 1], col_inds and values
+ * [nnz] when cap >= nnz and all three are non-NULL. */
+size_t kyn_debug_get_crs(kyn_system s, size_t model, size_t* row_ptrs, size_t* col_inds, double* values, size_t cap);
 /* No counterpart in the reference: for callers that cannot allocate device memory (HIP graph
  * capture, serving loops).  One beam element's FP64 residual [N][6] and iteration matrix
  * [N][N][36] at the system's current state -- the values kyn_debug_assemble followed by

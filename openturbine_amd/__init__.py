@@ -174,6 +174,8 @@ class KynLib:
         L.kyn_debug_set_wait_limit.argtypes = [C.c_void_p, C.c_int]
         L.kyn_get_element_terms.argtypes = [C.c_void_p, C.c_size_t, _dp, _dp]
         L.kyn_element_terms_noalloc.argtypes = [C.c_void_p, C.c_size_t, _dp, _dp]
+        L.kyn_debug_get_crs.argtypes = [C.c_void_p, C.c_size_t, _sp, _sp, _dp, C.c_size_t]
+        L.kyn_debug_get_crs.restype = C.c_size_t
         L.kyn_get_motion.argtypes = [C.c_void_p, _dp]
         L.kyn_model_num_nodes.argtypes = [C.c_void_p]
         L.kyn_model_num_nodes.restype = C.c_size_t
@@ -510,6 +512,22 @@ class KynSystem:
         if inputs:
             self.apply_inputs(inputs)
         self._check(self.lib.L.kyn_debug_assemble(self.h))
+
+    def crs_matrix(self, model=0):
+        """(row_ptrs, col_inds, values) of a model that forms one large block (plain form), in the
+        reference's dof order: the graph and the values of the last assembly."""
+        L = self.lib.L
+        nnz = L.kyn_debug_get_crs(self.h, model, None, None, None, 0)
+        if nnz == 0:
+            raise RuntimeError(self.lib.error())
+        ndofs = int(L.kyn_system_num_dofs(self.h, model))
+        rp = np.zeros(ndofs + 1, dtype=np.uint64)
+        ci = np.zeros(nnz, dtype=np.uint64)
+        v = np.zeros(nnz)
+        got = L.kyn_debug_get_crs(self.h, model, rp.ctypes.data_as(_sp), ci.ctypes.data_as(_sp), v.ctypes.data_as(_dp),
+                                  nnz)
+        assert got == nnz
+        return rp.astype(np.int64), ci.astype(np.int64), v
 
     def element_terms(self, elem, N):
         r = np.zeros((N, 6))

@@ -35,6 +35,8 @@ void launch_plain_iteration(const DevSys& S, int Nm, int Qm, int nm, int iter, b
                             hipStream_t st, hipEvent_t e0, hipEvent_t e1, bool defer_update);
 void launch_plain_update(const DevSys& S, int iter, hipStream_t st);
 void launch_large_uncondition(const DevSys& S, int b, int n, hipStream_t st);
+void launch_crs_residual(const DevSys& S, int b, int n, const double* x, double* r, hipStream_t st);
+void launch_crs_to_dense(const DevSys& S, int b, int n, hipStream_t st);
 size_t cond_lds(int ne, int me);
 void launch_cond_factor(const DevSys& S, size_t lds, bool fp32, const double* rin, hipStream_t st);
 void launch_cond_interface(const DevSys& S, int b, const double* rin, hipStream_t st);
@@ -241,6 +243,7 @@ struct kyn_system_s {
     std::vector<int> large_blocks, large_n, large_nsys;
     std::vector<int64_t> large_off;
     std::vector<int> large_cond, large_nz;  // condensed solve (else dense), interface size
+    std::vector<int64_t> crs_rp_off, crs_nz_off;  // per block (large blocks only)
     std::vector<int64_t> large_Z_off;
     size_t cond_lds_bytes{0};
     rocblas_handle blas{};
@@ -298,6 +301,54 @@ struct Builder {
     int vec_total = 0;
     int64_t mat_total = 0;
 };
+
+// CRS graph of one large block's KKT matrix in block-local dofs (the graph of
+// create_full_matrix.hpp:19-74: compute_row_ptrs, compute_col_inds, then sort_crs_graph).  A
+// node's rows hold its own six dofs, the dofs of every node it shares a beam element or a spring
+// with, and the rows of the constraints on it; a constraint's rows hold its target's and base's
+// dofs.  Columns ascending and distinct in every row.
+void block_crs_graph(const Builder& B, int b, std::vector<int>& rp, std::vector<int>& cols) {
+    const int n = B.block_n[b];
+    std::vector<std::vector<int>> row(static_cast<size_t>(n));
+    auto couple = [&](int u, int v) {  // the 6 x 6 node block: rows of u, columns of v
+        const int lu = B.node_local[u], lv = B.node_local[v];
+        for (int r = 0; r < 6; ++r)
+            for (int c = 0; c < 6; ++c) row[lu + r].push_back(lv + c);
+    };
+    for (int k = B.block_node_begin[b]; k < B.block_node_end[b]; ++k) couple(B.block_nodes[k], B.block_nodes[k]);
+    for (int k = B.block_elem_begin[b]; k < B.block_elem_end[b]; ++k) {
+        const int e = B.block_elems[k], N = B.elem_N[e], off = B.elem_node_off[e];
+        for (int i = 0; i < N; ++i)
+            for (int j = 0; j < N; ++j) couple(B.elem_nodes[off + i], B.elem_nodes[off + j]);
+    }
+    for (int k = B.block_spring_begin[b]; k < B.block_spring_end[b]; ++k) {
+        const int sp = B.block_springs[k];
+        couple(B.spring_n1[sp], B.spring_n2[sp]);
+        couple(B.spring_n2[sp], B.spring_n1[sp]);
+    }
+    for (int k = B.block_con_begin[b]; k < B.block_con_end[b]; ++k) {
+        const int c = B.block_cons[k], lr = B.con_local[c];
+        const int nodes[2] = {B.con_target[c], con_two_node(B.con_type[c]) ? B.con_base[c] : -1};
+        for (int node : nodes) {
+            if (node < 0) continue;
+            const int l = B.node_local[node];
+            for (int r = 0; r < B.con_rows[c]; ++r)
+                for (int d = 0; d < 6; ++d) {
+                    row[lr + r].push_back(l + d);
+                    row[l + d].push_back(lr + r);
+                }
+        }
+    }
+    rp.assign(static_cast<size_t>(n) + 1, 0);
+    cols.clear();
+    for (int i = 0; i < n; ++i) {
+        auto& v = row[i];
+        std::sort(v.begin(), v.end());
+        v.erase(std::unique(v.begin(), v.end()), v.end());
+        cols.insert(cols.end(), v.begin(), v.end());
+        rp[i + 1] = static_cast<int>(cols.size());
+    }
+}
 
 void add_model(Builder& B, const kyn_model_s& mo, bool dynamic) {
     const int node0 = static_cast<int>(B.node_block.size());
@@ -664,12 +715,15 @@ void blas_check(rocblas_status st, const char* what) {
         throw KynError(KYN_ERR_HIP, std::string(what) + " failed: " + rocblas_status_to_string(st));
 }
 
-// SolveSystem for the large blocks (src/step/solve_system.hpp:14-39 with a dense library LU in
-// place of KLU): LU with partial pivoting of the column-major KKT matrix (rocSOLVER getrf:
-// largest |a_ik|, first index on ties, as the small-block LU) and the triangular solves on the
-// conditioned rhs; FP32 option: the factorization in FP32 and FP64 iterative refinement against
-// the FP64 matrix, r = b - A x, `refine_sweeps` correction solves.  Then UnconditionSolution.
-// Blocks of models that are no longer iterating are skipped.
+// SolveSystem for the large blocks (src/step/solve_system.hpp:14-39), from the CRS matrix the
+// assembly wrote.  Condensed blocks: element eliminations read their rows of the CRS matrix,
+// the interface system is solved densely; FP32 option: element factors in FP32 and FP64
+// refinement on r = b - A x computed on the CRS matrix.  Other blocks: the CRS matrix expanded
+// to a dense column-major one, LU with partial pivoting (rocSOLVER getrf: largest |a_ik|, first
+// index on ties, as the small-block LU) and the triangular solves on the conditioned rhs; FP32
+// option: the factorization in FP32 and FP64 iterative refinement against the FP64 matrix,
+// `refine_sweeps` correction solves.  Then UnconditionSolution.  Blocks of models that are no
+// longer iterating are skipped.
 void solve_large_blocks(kyn_system_s* s) {
     auto& S = s->S;
     const hipStream_t st = s->stream;
@@ -706,12 +760,8 @@ void solve_large_blocks(kyn_system_s* s) {
                 for (size_t k = 0; k < s->large_blocks.size(); ++k) {
                     const int b = s->large_blocks[k], n = s->large_n[k];
                     if (!s->large_cond[k] || !s->model_active_host[s->block_model[b]]) continue;
-                    // r = b - A x
-                    kyn::launch_block_axpy_copy(S, b, n, nullptr, nullptr, s->rfull, st);  // r = b
-                    const double alpha = -1., beta = 1.;
-                    blas_check(rocblas_dgemv(s->blas, rocblas_operation_none, n, n, &alpha, S.Abig + s->large_off[k], n,
-                                             S.xsol + s->block_vec_off[b], 1, &beta, s->rfull + s->block_vec_off[b], 1),
-                               "rocblas_dgemv");
+                    // r = b - A x on the CRS matrix (FP64)
+                    kyn::launch_crs_residual(S, b, n, S.xsol, s->rfull, st);
                 }
                 cond_solve(s->rfull, s->dfull);
                 for (size_t k = 0; k < s->large_blocks.size(); ++k) {
@@ -733,6 +783,8 @@ void solve_large_blocks(kyn_system_s* s) {
         double* A = S.Abig + s->large_off[k];
         double* x = S.xsol + s->block_vec_off[b];
         const double* rhs = S.rhs + s->block_vec_off[b];
+        hip_check(hipMemsetAsync(A, 0, static_cast<size_t>(n) * n * sizeof(double), st), "memset");
+        kyn::launch_crs_to_dense(S, b, n, st);
         if (S.precision == KYN_TANGENT_FP64) {
             hip_check(hipMemcpyAsync(x, rhs, n * sizeof(double), hipMemcpyDeviceToDevice, st), "D2D");
             blas_check(rocsolver_dgetrf(s->blas, n, n, A, n, s->ipiv, s->info), "rocsolver_dgetrf");
@@ -975,7 +1027,7 @@ kyn_system kyn_system_create(const kyn_model* models, size_t num_models, const k
         // blocks above one workgroup's LU take the large path (plain form): column-major HBM
         // matrix, library LU; the others keep the per-block LDS factorization
         std::vector<int> block_large(B.block_n.size(), 0);
-        int64_t A_small = 0, A_big = 0;
+        int64_t A_small = 0;
         B.block_C_off.assign(B.block_n.size(), 0);
         B.C_total = 0;
         for (size_t b = 0; b < B.block_n.size(); ++b) {
@@ -989,9 +1041,8 @@ kyn_system kyn_system_create(const kyn_model* models, size_t num_models, const k
                 sys->large_blocks.push_back(static_cast<int>(b));
                 sys->large_n.push_back(n);
                 sys->large_nsys.push_back(B.block_nsys[b]);
-                sys->large_off.push_back(A_big);
+                sys->large_off.push_back(0);  // dense offset: set once the condensed blocks are known
                 B.block_A_off[b] = 0;
-                A_big += static_cast<int64_t>(n) * n;
             } else {
                 nm = std::max(nm, n);
                 B.block_A_off[b] = A_small;
@@ -1001,9 +1052,6 @@ kyn_system kyn_system_create(const kyn_model* models, size_t num_models, const k
             }
         }
         B.A_total = A_small;
-        if (sys->exec_mode == KYN_EXEC_PLAIN && S.precision != KYN_TANGENT_FP64 && sys->large_blocks.empty())
-            throw KynError(KYN_ERR_UNSUPPORTED, "the plain form takes the FP32 option only for large blocks "
-                                                "(mixed-precision library LU); small blocks factor in FP64");
         sys->Nm = Nm;
         sys->Qm = Qm;
         sys->nm = nm;
@@ -1085,11 +1133,26 @@ kyn_system kyn_system_create(const kyn_model* models, size_t num_models, const k
         S.block_C_off = Ar.up(B.block_C_off);
         S.block_large = Ar.up(block_large);
         {
-            std::vector<int64_t> big_off(B.block_n.size(), 0);
-            for (size_t k = 0; k < sys->large_blocks.size(); ++k) big_off[sys->large_blocks[k]] = sys->large_off[k];
-            S.block_big_off = Ar.up(big_off);
-            S.Abig = Ar.zeros<double>(static_cast<size_t>(A_big));
-            S.Abig32 = (S.precision != KYN_TANGENT_FP64 && A_big > 0) ? Ar.zeros<float>(static_cast<size_t>(A_big)) : nullptr;
+            // CRS graphs of the large blocks; the dense matrices of blocks that do not condense are
+            // allocated below, once the condensed blocks are known
+            sys->crs_rp_off.assign(B.block_n.size(), 0);
+            sys->crs_nz_off.assign(B.block_n.size(), 0);
+            {
+                std::vector<int> rp_all, col_all, rp, cols;
+                for (int b : sys->large_blocks) {
+                    block_crs_graph(B, b, rp, cols);
+                    sys->crs_rp_off[b] = static_cast<int64_t>(rp_all.size());
+                    sys->crs_nz_off[b] = static_cast<int64_t>(col_all.size());
+                    rp_all.insert(rp_all.end(), rp.begin(), rp.end());
+                    col_all.insert(col_all.end(), cols.begin(), cols.end());
+                }
+                S.crs_rp_off = Ar.up(sys->crs_rp_off);
+                S.crs_nz_off = Ar.up(sys->crs_nz_off);
+                S.crs_rp = Ar.up(rp_all);
+                S.crs_col = Ar.up(col_all);
+                S.crs_nnz = static_cast<int64_t>(col_all.size());
+                S.crs_val = Ar.zeros<double>(std::max<size_t>(col_all.size(), 1));
+            }
             // work items (nodes, then constraints, of each large block) and node incidence lists
             std::vector<int> items;
             for (int b : sys->large_blocks) {
@@ -1143,6 +1206,8 @@ kyn_system kyn_system_create(const kyn_model* models, size_t num_models, const k
                     zl, ce_begin(B.block_n.size() + 1, 0);
                 std::vector<int64_t> Y_off, S_off, Z_off(B.block_n.size(), 0);
                 int64_t Y_tot = 0, S_tot = 0, Z_tot = 0;
+                std::vector<int> dof_ce(static_cast<size_t>(B.vec_total), -1), dof_cl(static_cast<size_t>(B.vec_total), -1),
+                    dof_z(static_cast<size_t>(B.vec_total), -1);
                 std::vector<int> nz_of(B.block_n.size(), 0), cond_of(B.block_n.size(), 0), nce_of(B.block_n.size(), 0);
                 for (int b : sys->large_blocks) {
                     bool ok = true;
@@ -1195,6 +1260,7 @@ kyn_system kyn_system_create(const kyn_model* models, size_t num_models, const k
                             zl.push_back(i);
                         }
                     nz_of[b] = nz;
+                    for (int i = 0; i < B.block_n[b]; ++i) dof_z[B.block_vec_off[b] + i] = zidx[i];
                     Z_off[b] = Z_tot;
                     Z_tot += static_cast<int64_t>(nz) * nz;
                     int idx = 0;
@@ -1203,7 +1269,12 @@ kyn_system kyn_system_create(const kyn_model* models, size_t num_models, const k
                         ce_elem.push_back(e);
                         ce_block.push_back(b);
                         for (int i = 0; i < B.elem_N[e]; ++i)
-                            for (int c = 0; c < 6; ++c) rows.push_back(B.node_local[B.elem_nodes[B.elem_node_off[e] + i]] + c);
+                            for (int c = 0; c < 6; ++c) {
+                                const int r = B.node_local[B.elem_nodes[B.elem_node_off[e] + i]] + c;
+                                dof_ce[B.block_vec_off[b] + r] = static_cast<int>(ce_elem.size()) - 1;
+                                dof_cl[B.block_vec_off[b] + r] = 6 * i + c;
+                                rows.push_back(r);
+                            }
                         rows_off.push_back(static_cast<int>(rows.size()));
                         for (int r : elam[idx]) {
                             lam.push_back(r);
@@ -1240,6 +1311,9 @@ kyn_system kyn_system_create(const kyn_model* models, size_t num_models, const k
                 S.block_ce_begin = Ar.up(ce_begin);
                 S.block_Z_off = Ar.up(Z_off);
                 S.Zbuf = Ar.zeros<double>(static_cast<size_t>(Z_tot));
+                S.dof_cond_elem = Ar.up(dof_ce);
+                S.dof_cond_local = Ar.up(dof_cl);
+                S.dof_zidx = Ar.up(dof_z);
                 S.zrhs = Ar.zeros<double>(static_cast<size_t>(z_off.back()));
                 for (size_t k = 0; k < sys->large_blocks.size(); ++k) {
                     const int b = sys->large_blocks[k];
@@ -1247,6 +1321,22 @@ kyn_system kyn_system_create(const kyn_model* models, size_t num_models, const k
                     sys->large_nz.push_back(nz_of[b]);
                     sys->large_Z_off.push_back(Z_off[b]);
                 }
+            }
+            {
+                // dense column-major matrices of the large blocks that do not condense (library LU)
+                std::vector<int64_t> big_off(B.block_n.size(), 0);
+                int64_t dense = 0;
+                for (size_t k = 0; k < sys->large_blocks.size(); ++k) {
+                    sys->large_off[k] = dense;
+                    if (sys->large_cond.empty() || !sys->large_cond[k]) {
+                        big_off[sys->large_blocks[k]] = dense;
+                        dense += static_cast<int64_t>(sys->large_n[k]) * sys->large_n[k];
+                    }
+                }
+                S.block_big_off = Ar.up(big_off);
+                S.Abig = dense > 0 ? Ar.zeros<double>(static_cast<size_t>(dense)) : nullptr;
+                S.Abig32 = (S.precision != KYN_TANGENT_FP64 && dense > 0) ? Ar.zeros<float>(static_cast<size_t>(dense))
+                                                                          : nullptr;
             }
             if (!sys->large_blocks.empty()) {
                 const int nbig = *std::max_element(sys->large_n.begin(), sys->large_n.end());
@@ -1823,6 +1913,59 @@ int kyn_debug_assemble(kyn_system s) {
         return run_step(s, false);
     });
 }
+size_t kyn_debug_get_crs(kyn_system s, size_t model, size_t* row_ptrs, size_t* col_inds, double* values, size_t cap) {
+    size_t nnz = 0;
+    const int rc = guarded([&] {
+        if (!s || model >= s->num_models) throw KynError(KYN_ERR_INVALID, "bad model");
+        const auto& g = s->model_gdof_to_block_vec[model];
+        int b = -1;
+        for (int v : g) {
+            if (v < 0) continue;
+            const int blk = static_cast<int>(std::upper_bound(s->block_vec_off.begin(), s->block_vec_off.end(), v) -
+                                             s->block_vec_off.begin()) - 1;
+            if (b >= 0 && blk != b) throw KynError(KYN_ERR_INVALID, "the model's dofs span several blocks");
+            b = blk;
+        }
+        if (b < 0 || std::find(s->large_blocks.begin(), s->large_blocks.end(), b) == s->large_blocks.end())
+            throw KynError(KYN_ERR_INVALID, "the model's dofs are not one large block (no CRS matrix)");
+        const int n = s->block_n[b], vb = s->block_vec_off[b];
+        std::vector<int> rp(static_cast<size_t>(n) + 1);
+        hip_check(hipMemcpy(rp.data(), s->S.crs_rp + s->crs_rp_off[b], rp.size() * sizeof(int), hipMemcpyDeviceToHost),
+                  "D2H");
+        std::vector<int> ci(static_cast<size_t>(rp[n]));
+        std::vector<double> val(ci.size());
+        hip_check(hipMemcpy(ci.data(), s->S.crs_col + s->crs_nz_off[b], ci.size() * sizeof(int), hipMemcpyDeviceToHost),
+                  "D2H");
+        hip_check(hipMemcpy(val.data(), s->S.crs_val + s->crs_nz_off[b], val.size() * sizeof(double),
+                            hipMemcpyDeviceToHost),
+                  "D2H");
+        std::vector<int> gdof_of(static_cast<size_t>(n), -1);  // block-local dof -> reference dof
+        for (size_t i = 0; i < g.size(); ++i)
+            if (g[i] >= 0) gdof_of[g[i] - vb] = static_cast<int>(i);
+        std::vector<std::vector<std::pair<size_t, double>>> rows(g.size());
+        for (size_t i = 0; i < g.size(); ++i) {
+            if (g[i] < 0) continue;
+            const int lr = g[i] - vb;
+            for (int p = rp[lr]; p < rp[lr + 1]; ++p)
+                if (gdof_of[ci[p]] >= 0) rows[i].emplace_back(static_cast<size_t>(gdof_of[ci[p]]), val[p]);
+            std::sort(rows[i].begin(), rows[i].end());
+            nnz += rows[i].size();
+        }
+        if (cap < nnz || !row_ptrs || !col_inds || !values) return KYN_OK;
+        size_t k = 0;
+        row_ptrs[0] = 0;
+        for (size_t i = 0; i < g.size(); ++i) {
+            for (const auto& e : rows[i]) {
+                col_inds[k] = e.first;
+                values[k++] = e.second;
+            }
+            row_ptrs[i + 1] = k;
+        }
+        return KYN_OK;
+    });
+    return rc == KYN_OK ? nnz : 0;
+}
+
 int kyn_get_element_terms(kyn_system s, size_t element, double* residual, double* system_matrix) {
     return guarded([&] {
         if (!s || element >= s->num_elems) throw KynError(KYN_ERR_INVALID, "bad element");

@@ -2049,6 +2049,18 @@ __global__ void __launch_bounds__(kThreads) k_plain_element(DevSys S, int Nm, in
                                  S.elem_mat + S.elem_mat_off[e]);
 }
 
+// FP32-tangent option of the plain form: QP matrices in FP32 and the node-pair contraction on
+// the matrix cores (as in the fused FP32 form), element terms stored in FP64.
+__global__ void __launch_bounds__(kThreads) k_plain_element_f32(DevSys S, int Nm, int Qm) {
+    extern __shared__ __align__(16) char sm[];
+    const Lay L = make_layout(Nm, Qm, 1, 0, false, 4, false);
+    const int e = blockIdx.x;
+    const int m = S.block_model[S.node_block[S.elem_nodes[S.elem_node_off[e]]]];
+    if (!S.model_active[m]) return;
+    element_phase<float, false>(S, e, sm, L, 0, S.elem_res + (size_t)S.elem_node_off[e] * 6,
+                                S.elem_mat + S.elem_mat_off[e]);
+}
+
 // One element's residual and iteration matrix at the current state, with its own tangent
 // blocks (the no-allocation element-terms entry point): res [N][6], mat [N][N][36].
 __global__ void __launch_bounds__(kThreads) k_element_terms(DevSys S, int Nm, int Qm, int e, double* res, double* mat) {
@@ -2275,10 +2287,31 @@ __global__ void __launch_bounds__(kThreads) k_plain_solve(DevSys S, int nm) {
     factor_solve_uncondition<double>(S, b, A, nullptr, L.lda, piv, rhs, xs, xs2, sm, L);
 }
 
-// Large blocks: assembly of the column-major KKT matrix, one workgroup per item.  A node item
-// owns its 6 columns and its 6 rhs entries, a constraint item its rows' columns and rhs
-// entries; each entry receives its contributions in the order of k_plain_assemble (elements in
-// order, masses, springs, then constraints), so the matrix equals the small-block one.
+// Large blocks: position of entry (row, col) among the block's CRS nonzeros (columns ascending
+// in every row: a binary search), -1 if the graph does not hold it.
+__device__ __forceinline__ int64_t crs_find(const DevSys& S, int b, int row, int col) {
+    const int* rp = S.crs_rp + S.crs_rp_off[b];
+    const int* ci = S.crs_col + S.crs_nz_off[b];
+    int lo = rp[row], hi = rp[row + 1];
+    const int end = hi;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (ci[mid] < col) lo = mid + 1;
+        else hi = mid;
+    }
+    return (lo < end && ci[lo] == col) ? S.crs_nz_off[b] + lo : -1;
+}
+
+// Every nonzero of the large blocks to zero (before each assembly)
+__global__ void k_crs_zero(DevSys S) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < S.crs_nnz; i += (int64_t)gridDim.x * blockDim.x)
+        S.crs_val[i] = 0.;
+}
+
+// Large blocks: assembly into the CRS values, one workgroup per item.  A node item owns its 6
+// columns and its 6 rhs entries, a constraint item its rows' columns and rhs entries; each
+// entry receives its contributions in the order of k_plain_assemble (elements in order, masses,
+// springs, then constraints), so the matrix equals the small-block one.
 __global__ void __launch_bounds__(kThreads) k_large_assemble(DevSys S) {
     const int item = S.big_item[blockIdx.x], t = threadIdx.x;
     const bool is_node = item >= 0;
@@ -2286,21 +2319,25 @@ __global__ void __launch_bounds__(kThreads) k_large_assemble(DevSys S) {
     const int b = is_node ? S.node_block[node] : S.node_block[S.con_target[con]];
     const int m = S.block_model[b];
     if (!S.model_active[m]) return;
-    const int n = S.block_n[b], nsys = S.block_nsys[b], off = S.block_vec_off[b];
-    double* A = S.Abig + S.block_big_off[b];
+    const int nsys = S.block_nsys[b], off = S.block_vec_off[b];
+    double* V = S.crs_val;
     const double cond = S.p.conditioner;
+    // entry (row, col) += v; every (row, col) an item adds to lies in the graph
+    auto add = [&](int row, int col, double v) {
+        const int64_t p = crs_find(S, b, row, col);
+        if (p >= 0) V[p] += v;
+    };
     if (!is_node) {
         const int rows = S.con_rows[con], lr = S.con_local[con];
         const int type = S.con_type[con];
         const bool two = con_two_node(type);
         const int lt = S.node_local[S.con_target[con]];
         const int lb = two ? S.node_local[S.con_base[con]] : 0;
-        for (int k = t; k < rows * n; k += blockDim.x) A[(size_t)(lr + k / n) * n + k % n] = 0.;
-        __syncthreads();
+        // B^T: the constraint's columns, rows of its target (and base) dofs
         for (int q = t; q < rows * 6; q += blockDim.x) {
             const int i = q / 6, j = q % 6;
-            A[(size_t)(lr + i) * n + lt + j] += S.con_tgradT[con * 36 + j * 6 + i];
-            if (two) A[(size_t)(lr + i) * n + lb + j] += S.con_bgradT[con * 36 + j * 6 + i];
+            add(lt + j, lr + i, S.con_tgradT[con * 36 + j * 6 + i]);
+            if (two) add(lb + j, lr + i, S.con_bgradT[con * 36 + j * 6 + i]);
         }
         if (t < rows) {
             const double raw = 0. + S.con_res[con * 6 + t];
@@ -2310,8 +2347,6 @@ __global__ void __launch_bounds__(kThreads) k_large_assemble(DevSys S) {
         return;
     }
     const int l = S.node_local[node];
-    for (int k = t; k < 6 * n; k += blockDim.x) A[(size_t)(l + k / n) * n + k % n] = 0.;
-    __syncthreads();
     // beams (ContributeBeamsToSparseMatrix), elements in order
     for (int q = S.node_elem_begin[node]; q < S.node_elem_begin[node + 1]; ++q) {
         const int e = S.node_elem[2 * q], jl = S.node_elem[2 * q + 1];
@@ -2320,14 +2355,14 @@ __global__ void __launch_bounds__(kThreads) k_large_assemble(DevSys S) {
         for (int k = t; k < N * 36; k += blockDim.x) {
             const int i = k / 36, rc = k % 36;
             const int row = S.node_local[S.elem_nodes[noff + i]] + rc / 6;
-            A[(size_t)(l + rc % 6) * n + row] += Me[(size_t)(i * N + jl) * 36 + rc] * cond;
+            add(row, l + rc % 6, Me[(size_t)(i * N + jl) * 36 + rc] * cond);
         }
         __syncthreads();
     }
     // masses (diagonal node block)
     for (int q = S.node_mass_begin[node]; q < S.node_mass_begin[node + 1]; ++q) {
         const int e = S.node_mass[q];
-        if (t < 36) A[(size_t)(l + t % 6) * n + l + t / 6] += S.mass_mat[e * 36 + t] * cond;
+        if (t < 36) add(l + t / 6, l + t % 6, S.mass_mat[e * 36 + t] * cond);
         __syncthreads();
     }
     // springs: this node's column block of [a, -a; -a, a]
@@ -2339,7 +2374,7 @@ __global__ void __launch_bounds__(kThreads) k_large_assemble(DevSys S) {
             const int half = t / 9, rc = t % 9;  // row node (n1, n2) of the column node
             const int lr = half ? l2 : l1;
             const bool diag = (half == 0) == (node == n1);
-            A[(size_t)(l + rc % 3) * n + lr + rc / 3] += (diag ? 1. : -1.) * S.spring_mat[e * 9 + rc] * cond;
+            add(lr + rc / 3, l + rc % 3, (diag ? 1. : -1.) * S.spring_mat[e * 9 + rc] * cond);
         }
         __syncthreads();
     }
@@ -2351,12 +2386,15 @@ __global__ void __launch_bounds__(kThreads) k_large_assemble(DevSys S) {
         const double* G = tgt ? S.con_tgrad + c * 36 : S.con_bgrad + c * 36;
         for (int k = t; k < rows * 6; k += blockDim.x) {
             const int i = k / 6, j = k % 6;
-            A[(size_t)(l + j) * n + lr + i] += G[i * 6 + j];
+            add(lr + i, l + j, G[i * 6 + j]);
         }
         __syncthreads();
     }
     const bool pin = S.node_pin[node] != 0;  // translations only: rotational rows held at zero
-    if (pin && t >= 3 && t < 6) A[(size_t)(l + t) * n + l + t] = 1.;
+    if (pin && t >= 3 && t < 6) {
+        const int64_t p = crs_find(S, b, l + t, l + t);
+        if (p >= 0) V[p] = 1.;
+    }
     // residual entries of this node
     if (t < 6) {
         double r = 0.;
@@ -2407,8 +2445,10 @@ __device__ __forceinline__ void cond_factor_body(DevSys& S, const double* rin) {
     const int ce = blockIdx.x, t = threadIdx.x;
     const int b = S.cond_elem_block[ce];
     if (!S.model_active[S.block_model[b]]) return;
-    const int n = S.block_n[b];
-    const double* A = S.Abig + S.block_big_off[b];
+    const int vb = S.block_vec_off[b];
+    const int* rp = S.crs_rp + S.crs_rp_off[b];
+    const int* ci = S.crs_col + S.crs_nz_off[b];
+    const double* V = S.crs_val + S.crs_nz_off[b];
     const int r0 = S.cond_rows_off[ce], ne = S.cond_rows_off[ce + 1] - r0;
     const int l0 = S.cond_lam_off[ce], me = S.cond_lam_off[ce + 1] - l0;
     const int* rows = S.cond_rows + r0;
@@ -2424,14 +2464,23 @@ __device__ __forceinline__ void cond_factor_body(DevSys& S, const double* rin) {
     int* sync = reinterpret_cast<int*>(sm + off);
     off += 16;
     int* piv = reinterpret_cast<int*>(sm + off);
-    const double* rhs = rin + S.block_vec_off[b];
-    for (int k = t; k < ne * (ne + nrhs); k += blockDim.x) {  // column by column: coalesced reads
-        const int i = k % ne, j = k / ne;
-        double v;
-        if (j < ne) v = A[(size_t)rows[j] * n + rows[i]];
-        else if (j < ne + me) v = A[(size_t)lam[j - ne] * n + rows[i]];
-        else v = rhs[rows[i]];
-        M[i * lda + j] = (TT)v;
+    const double* rhs = rin + vb;
+    for (int k = t; k < ne * lda; k += blockDim.x) M[k] = TT(0);
+    __syncthreads();
+    // thread per element row: its CRS row's entries go to the element's own columns (K_e) or to
+    // the constraint-row columns of the element (B_e^T); the rhs entry last
+    for (int i = t; i < ne; i += blockDim.x) {
+        const int row = rows[i];
+        for (int p = rp[row]; p < rp[row + 1]; ++p) {
+            const int c = ci[p];
+            if (S.dof_cond_elem[vb + c] == ce) {
+                M[i * lda + S.dof_cond_local[vb + c]] = (TT)V[p];
+            } else {
+                for (int k = 0; k < me; ++k)
+                    if (lam[k] == c) M[i * lda + ne + k] = (TT)V[p];
+            }
+        }
+        M[i * lda + ne + me] = (TT)rhs[row];
     }
     __syncthreads();
     const LuScratch W{ph, lc, sync, S.lu_wait_limit, S.dev_error};
@@ -2440,10 +2489,13 @@ __device__ __forceinline__ void cond_factor_body(DevSys& S, const double* rin) {
     back_substitute_cols<TT>(M, ne, lda, ph, nrhs, Y, ne);
     __syncthreads();
     double* Sm = S.Sbuf + S.cond_S_off[ce];
-    for (int q = t; q < me * nrhs; q += blockDim.x) {
+    for (int q = t; q < me * nrhs; q += blockDim.x) {  // S_e = B_e Y: constraint row lam[k]'s entries in this element
         const int k = q % me, c = q / me;
         double acc = 0.;
-        for (int i = 0; i < ne; ++i) acc += A[(size_t)rows[i] * n + lam[k]] * Y[(size_t)c * ne + i];
+        for (int p = rp[lam[k]]; p < rp[lam[k] + 1]; ++p) {
+            const int col = ci[p];
+            if (S.dof_cond_elem[vb + col] == ce) acc += V[p] * Y[(size_t)c * ne + S.dof_cond_local[vb + col]];
+        }
         Sm[(size_t)c * me + k] = acc;
     }
 }
@@ -2458,13 +2510,23 @@ __global__ void __launch_bounds__(kThreads) k_cond_factor_f32(DevSys S, const do
 __global__ void __launch_bounds__(kThreads) k_cond_interface(DevSys S, int b, const double* rin) {
     const int t = threadIdx.x;
     if (!S.model_active[S.block_model[b]]) return;
-    const int n = S.block_n[b], z0 = S.block_z_off[b], nz = S.block_z_off[b + 1] - z0;
+    const int z0 = S.block_z_off[b], nz = S.block_z_off[b + 1] - z0, vb = S.block_vec_off[b];
     const int* zr = S.cond_z + z0;
-    const double* A = S.Abig + S.block_big_off[b];
+    const int* rp = S.crs_rp + S.crs_rp_off[b];
+    const int* ci = S.crs_col + S.crs_nz_off[b];
+    const double* V = S.crs_val + S.crs_nz_off[b];
     double* Z = S.Zbuf + S.block_Z_off[b];
     double* rz = S.zrhs + z0;
-    for (int k = t; k < nz * nz; k += blockDim.x) Z[k] = A[(size_t)zr[k / nz] * n + zr[k % nz]];
-    for (int i = t; i < nz; i += blockDim.x) rz[i] = rin[S.block_vec_off[b] + zr[i]];
+    for (int k = t; k < nz * nz; k += blockDim.x) Z[k] = 0.;
+    __syncthreads();
+    for (int a = t; a < nz; a += blockDim.x) {  // interface row a: its CRS row's interface columns
+        const int row = zr[a];
+        for (int p = rp[row]; p < rp[row + 1]; ++p) {
+            const int zc = S.dof_zidx[vb + ci[p]];
+            if (zc >= 0) Z[(size_t)zc * nz + a] = V[p];
+        }
+        rz[a] = rin[vb + row];
+    }
     __syncthreads();
     for (int ce = S.block_ce_begin[b]; ce < S.block_ce_begin[b + 1]; ++ce) {
         const int l0 = S.cond_lam_off[ce], me = S.cond_lam_off[ce + 1] - l0;
@@ -2474,6 +2536,32 @@ __global__ void __launch_bounds__(kThreads) k_cond_interface(DevSys S, int b, co
         if (t < me) rz[lz[t]] -= Sm[(size_t)me * me + t];
         __syncthreads();
     }
+}
+
+// r = b - A x over one large block's rows (CRS, thread per row, columns ascending)
+__global__ void k_crs_residual(DevSys S, int b, const double* x, double* r) {
+    const int row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= S.block_n[b] || !S.model_active[S.block_model[b]]) return;
+    const int vb = S.block_vec_off[b];
+    const int* rp = S.crs_rp + S.crs_rp_off[b];
+    const int* ci = S.crs_col + S.crs_nz_off[b];
+    const double* V = S.crs_val + S.crs_nz_off[b];
+    double acc = 0.;
+    for (int p = rp[row]; p < rp[row + 1]; ++p) acc += V[p] * x[vb + ci[p]];
+    r[vb + row] = S.rhs[vb + row] - acc;
+}
+
+// A block's CRS values into its dense column-major matrix (zeroed beforehand): the library-LU
+// path of blocks that do not condense
+__global__ void k_crs_to_dense(DevSys S, int b) {
+    const int row = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = S.block_n[b];
+    if (row >= n || !S.model_active[S.block_model[b]]) return;
+    const int* rp = S.crs_rp + S.crs_rp_off[b];
+    const int* ci = S.crs_col + S.crs_nz_off[b];
+    const double* V = S.crs_val + S.crs_nz_off[b];
+    double* A = S.Abig + S.block_big_off[b];
+    for (int p = rp[row]; p < rp[row + 1]; ++p) A[(size_t)ci[p] * n + row] = V[p];
 }
 
 // Element dofs from the interface solution: x_e = Y_b - Y_B lambda_e; interface rows copied
@@ -2599,6 +2687,8 @@ hipError_t set_lds_limits(size_t bytes) {
     if (e != hipSuccess) return e;
     e = hipFuncSetAttribute((const void*)k_element_terms, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
     if (e != hipSuccess) return e;
+    e = hipFuncSetAttribute((const void*)k_plain_element_f32, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (e != hipSuccess) return e;
     e = hipFuncSetAttribute((const void*)k_cond_factor_f64, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
     if (e != hipSuccess) return e;
     e = hipFuncSetAttribute((const void*)k_cond_factor_f32, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
@@ -2659,16 +2749,24 @@ void launch_plain_iteration(const DevSys& S, int Nm, int Qm, int nm, int iter, b
     hipLaunchKernelGGL(k_plain_clear_flag, dim3(1), dim3(64), 0, st, S);
     hipLaunchKernelGGL(k_plain_tangent, dim3((S.num_nodes + 255) / 256), dim3(256), 0, st, S);
     if (e0) (void)hipEventRecord(e0, st);
-    if (S.num_elems > 0)
-        hipLaunchKernelGGL(k_plain_element, dim3(S.num_elems), dim3(kThreads), element_lds_bytes(Nm, Qm), st, S, Nm, Qm);
+    if (S.num_elems > 0) {
+        if (S.precision)
+            hipLaunchKernelGGL(k_plain_element_f32, dim3(S.num_elems), dim3(kThreads),
+                               (size_t)make_layout(Nm, Qm, 1, 0, false, 4, false).total, st, S, Nm, Qm);
+        else
+            hipLaunchKernelGGL(k_plain_element, dim3(S.num_elems), dim3(kThreads), element_lds_bytes(Nm, Qm), st, S, Nm,
+                               Qm);
+    }
     if (e1) (void)hipEventRecord(e1, st);
     if (S.num_masses + S.num_springs > 0)
         hipLaunchKernelGGL(k_plain_points, dim3((S.num_masses + S.num_springs + 63) / 64), dim3(64), 0, st, S);
     if (S.num_cons > 0)
         hipLaunchKernelGGL(k_plain_constraints, dim3((S.num_cons + 63) / 64), dim3(64), 0, st, S);
     hipLaunchKernelGGL(k_plain_assemble, dim3(S.num_blocks), dim3(kThreads), 0, st, S);
-    if (S.num_big_items > 0)
+    if (S.num_big_items > 0) {
+        hipLaunchKernelGGL(k_crs_zero, dim3(512), dim3(256), 0, st, S);
         hipLaunchKernelGGL(k_large_assemble, dim3(S.num_big_items), dim3(kThreads), 0, st, S);
+    }
     if (!solve) return;
     hipLaunchKernelGGL(k_plain_solve, dim3(S.num_blocks), dim3(kThreads), solve_lds_bytes(nm), st, S, nm);
     if (!defer_update) launch_plain_update(S, iter, st);
@@ -2694,6 +2792,12 @@ void launch_cond_scatter_z(const DevSys& S, int b, int nz, double* xout, hipStre
 }
 void launch_block_axpy_copy(const DevSys& S, int b, int n, const double* d, double* x, double* r, hipStream_t st) {
     hipLaunchKernelGGL(k_block_axpy_copy, dim3((n + 255) / 256), dim3(256), 0, st, S, b, d, x, r);
+}
+void launch_crs_residual(const DevSys& S, int b, int n, const double* x, double* r, hipStream_t st) {
+    hipLaunchKernelGGL(k_crs_residual, dim3((n + 255) / 256), dim3(256), 0, st, S, b, x, r);
+}
+void launch_crs_to_dense(const DevSys& S, int b, int n, hipStream_t st) {
+    hipLaunchKernelGGL(k_crs_to_dense, dim3((n + 255) / 256), dim3(256), 0, st, S, b);
 }
 void launch_large_uncondition(const DevSys& S, int b, int n, hipStream_t st) {
     hipLaunchKernelGGL(k_large_uncondition, dim3((n + 255) / 256), dim3(256), 0, st, S, b);

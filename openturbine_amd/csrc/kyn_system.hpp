@@ -131,10 +131,20 @@ struct DevSys {
     double *con_tlres, *con_blres;                            // [c][6]
 
     // ---- large blocks (more dofs than one workgroup's LU holds; plain form only): the KKT
-    //      matrix is assembled column-major in HBM by one workgroup per node / constraint
-    //      (its columns, contributions in the oracle's order) and factored by a library LU
+    //      matrix is assembled into a CRS graph (the reference's create_full_matrix layout:
+    //      per row the node's own dofs, the dofs of every element sharing the node and the
+    //      constraint rows on it; constraint rows hold their target and base dofs; columns
+    //      ascending) by one workgroup per node / constraint (its columns, contributions in the
+    //      oracle's order).  Blocks that do not condense are expanded to a dense column-major
+    //      matrix for a library LU.
     const int* block_large;           // 1 if the block takes the large path
-    const int64_t* block_big_off;     // into Abig / Abig32 (n x n, column-major)
+    const int64_t* crs_rp_off;        // per block: into crs_rp (n + 1 block-local row pointers)
+    const int64_t* crs_nz_off;        // per block: into crs_col / crs_val
+    const int* crs_rp;
+    const int* crs_col;               // block-local column of each nonzero
+    double* crs_val;
+    int64_t crs_nnz;                  // nonzeros of all large blocks
+    const int64_t* block_big_off;     // into Abig / Abig32 (blocks that do not condense)
     double* Abig;
     float* Abig32;                    // FP32 copy for the mixed-precision factorization
     int num_big_items;
@@ -164,6 +174,12 @@ struct DevSys {
     const int64_t* block_Z_off;       // into Zbuf (nz x nz column-major)
     double* Zbuf;
     double* zrhs;                     // [sum nz]
+    // per block-vector entry of a condensed block: the condensed element owning the dof (-1:
+    // interface), the dof's index among that element's rows, and its interface index (-1:
+    // eliminated)
+    const int* dof_cond_elem;
+    const int* dof_cond_local;
+    const int* dof_zidx;
 
     // ---- solver
     void* A;              // blocks' dense KKT matrices (double, or float for FP32 tangent)

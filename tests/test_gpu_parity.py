@@ -265,6 +265,57 @@ def test_small_ensemble_fused_fp64_and_fp32(kyn, orc, refine):
         assert e32 < 1e-9
 
 
+def mixed_rotor():
+    """The first ensemble rotor with a 21-node blade and a smaller one."""
+    for sp in W.ensemble_config_specs(num_rotors=32, num_steps=3):
+        if 21 in sp.meta["blade_nodes"] and min(sp.meta["blade_nodes"]) < 21:
+            return sp
+    raise AssertionError("no mixed rotor")
+
+
+@pytest.mark.parametrize("case", ["rotor", "cantilever"])
+def test_fp32_element_terms_on_the_matrix_cores(kyn, orc, case):
+    """The FP32 tangent's node-pair contraction (v_mfma_f32_16x16x4f32 tiles; the plain form's
+    FP32 option writes its result out) against the oracle's FP64 element terms: every element of
+    a rotor with N = 21 (two 16-row tiles) and smaller blades (Q = 2N - 2), and the config-1
+    cantilever (N = 8, Q = 11: the odd-Q tail).  Bar: 2e-6 of the element matrix's largest
+    entry (FP32 QP matrices and products over 2Q terms).  The residual stays FP64: 1e-11 (the
+    strain x0' + u' - R x0' cancels most of its terms, so the FMA contractions the compiler picks
+    in this instantiation move it by a few 1e-12 of the largest entry)."""
+    spec = mixed_rotor() if case == "rotor" else W.cantilever_config_spec(num_steps=2)
+    g = ota.KynSystem(kyn, spec, exec_mode=ota.EXEC_PLAIN, precision=ota.TANGENT_FP32)
+    o = OracleSystem(orc, spec)
+    inputs = spec.step_inputs(0, o.get) if spec.step_inputs else None
+    g.debug_assemble(inputs)
+    o.debug_assemble(inputs)
+    Ns = [len(b.nodes) for b in spec.beams]
+    for e, N in enumerate(Ns):
+        r_g, m_g = g.element_terms(e, N)
+        r_o, m_o = o.element_terms(e, N, update_tangent=True)
+        dm, dr = rel(m_g, m_o), rel(r_g, r_o)
+        print(f"{case} element {e} (N = {N}, Q = {len(spec.beams[e].quad)}): FP32 matrix rel {dm:.1e}, "
+              f"residual rel {dr:.1e}", flush=True)
+        assert dm < 2e-6 and dr < 1e-11, (e, dm, dr)
+
+
+def test_plain_form_fp32_tangent(kyn, orc):
+    """The plain form's FP32-tangent option (QP matrices in FP32, contraction on the matrix
+    cores, FP64 block LU: an inexact-Jacobian Newton method) on the config-2 rotor for 100 steps
+    against the FP64 oracle: final state within 1e-9, Newton counts equal in all but a few steps."""
+    spec = W.rotor_config_spec(num_steps=100)
+    g = ota.KynSystem(kyn, spec, exec_mode=ota.EXEC_PLAIN, precision=ota.TANGENT_FP32)
+    o = OracleSystem(orc, spec)
+    differ = 0
+    for k in range(100):
+        inputs = spec.step_inputs(k, o.get)
+        assert g.step(inputs) == 0 and o.step(inputs) == 0
+        differ += int(len(g.convergence()) != len(o.convergence()))
+    eq = rel(g.get("q"), o.get("q"))
+    print(f"plain FP32 tangent, config-2 rotor, 100 steps: final q rel {eq:.2e}, {differ} steps with other "
+          f"Newton counts", flush=True)
+    assert eq < 1e-9 and differ <= 5
+
+
 @pytest.mark.parametrize("use_graph", [False, True])
 def test_schedule_and_graph_replay_match_host_driven_steps(kyn, use_graph):
     specs = W.ensemble_config_specs(num_rotors=3, num_steps=4)
