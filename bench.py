@@ -105,11 +105,13 @@ def lu_update_flops(specs):
     return np.array(out)
 
 
-def measured_traffic(refine):
+def measured_traffic(refine, refine_bound):
     """HBM bytes per launch of the FP32 step kernel from the committed counter summary for this
-    refinement setting (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs, copy-calibrated), if present."""
-    name = f"k_fused_step_f32_refine{refine}_ensemble4096_traffic.txt" if refine else \
-        "k_fused_step_f32_ensemble4096_traffic.txt"
+    refinement setting (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs, copy-calibrated), if present.
+    The summaries are taken with every Newton iteration refined (bound 0)."""
+    if refine_bound > 0:
+        return None
+    name = f"k_fused_step_f32_refine{refine}_ensemble4096_traffic.txt"
     path = ROOT / "profiles" / name
     if not path.exists():
         return None
@@ -406,7 +408,7 @@ def main():
     lu_mfma_flops_launch = float(np.sum(iters_per_step * lu_update_flops(specs)))
     kern_s = (r32["kernel_ms"] if r32["kernel_ms"] > 0 else ms_per_step) / 1000.0
     achieved = bytes_launch / kern_s / 1e9
-    traffic = measured_traffic(args.refine)
+    traffic = measured_traffic(args.refine, args.refine_bound)
     plain_rate = args.plain_steps / rp["wall_s"] if rp else None
     result = {
         "metric": METRIC,
