@@ -209,6 +209,11 @@ int kyn_debug_assemble(kyn_system s);
  * step returns KYN_ERR_HIP; polls < 0 gives up at the first wait (tests of that error path).
  * No counterpart in the reference (its LU runs on the host). */
 int kyn_debug_set_wait_limit(kyn_system s, int polls);
+/* Engine of the FP32 tangent's node-pair contraction per element node count N: bit N of `mask`
+ * set selects the matrix cores, clear the VALU register tile.  The default is the compiled-in
+ * tuning table (openturbine_amd/autotune.py); this call overrides it (the autotuner, tests).
+ * No counterpart in the reference. */
+int kyn_debug_set_contraction_mask(kyn_system s, unsigned mask);
 int kyn_get_element_terms(kyn_system s, size_t element, double* residual /* [N][6] */,
                           double* system_matrix /* [N][N][36] */);
 /* The CRS matrix of a model whose dofs form one large block (plain form), in the reference's

@@ -172,6 +172,7 @@ class KynLib:
         L.kyn_read_restart.argtypes = [C.c_void_p, C.c_size_t, C.POINTER(C.c_ubyte), C.c_size_t]
         L.kyn_debug_assemble.argtypes = [C.c_void_p]
         L.kyn_debug_set_wait_limit.argtypes = [C.c_void_p, C.c_int]
+        L.kyn_debug_set_contraction_mask.argtypes = [C.c_void_p, C.c_uint]
         L.kyn_get_element_terms.argtypes = [C.c_void_p, C.c_size_t, _dp, _dp]
         L.kyn_element_terms_noalloc.argtypes = [C.c_void_p, C.c_size_t, _dp, _dp]
         L.kyn_debug_get_crs.argtypes = [C.c_void_p, C.c_size_t, _sp, _sp, _dp, C.c_size_t]

@@ -22,6 +22,7 @@
 
 #include "../../include/kynema_amd.h"
 #include "kyn_system.hpp"
+#include "kyn_tuning.inc"
 
 namespace kyn {
 size_t fused_lds_bytes(int Nm, int Qm, int nm, int cm, int refine, int precision);
@@ -1058,6 +1059,7 @@ kyn_system kyn_system_create(const kyn_model* models, size_t num_models, const k
         int dev = 0;
         hip_check(hipGetDevice(&dev), "hipGetDevice");
         sys->device = dev;
+        S.contract_mfma_mask = kyn::kContractMfmaMask;
         int max_lds = 0;
         hip_check(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev), "attr");
         size_t need = 0;
@@ -1892,6 +1894,15 @@ int kyn_read_restart(kyn_system s, size_t model, const unsigned char* in, size_t
             hip_check(hipMemcpy(p + n0 * w, in + off, n * w * sizeof(double), hipMemcpyHostToDevice), "H2D");
             off += n * w * sizeof(double);
         }
+        return KYN_OK;
+    });
+}
+
+int kyn_debug_set_contraction_mask(kyn_system s, unsigned mask) {
+    return guarded([&] {
+        if (!s) throw KynError(KYN_ERR_INVALID, "null system");
+        s->S.contract_mfma_mask = mask;
+        s->graph_ready = false;  // the captured step holds the kernel arguments
         return KYN_OK;
     });
 }

@@ -353,6 +353,143 @@ __device__ __forceinline__ void contract_mfma_f32(const DevSys& S, char* sm, con
 }
 
 // ---------------------------------------------------------------------------------------
+// Node-pair contraction on the VALU (the FP64 tangent's, and the FP32 tangent's where the tuning
+// table picks it for the element size).
+// ---------------------------------------------------------------------------------------
+template <typename TT, bool kFused>
+__device__ __forceinline__ void contract_valu(const DevSys& S, char* sm, const Lay& L, int n, int N, int Q,
+                                              const int* loc, double* mat_out) {
+    const int tid = threadIdx.x;
+    const TT* qpM = reinterpret_cast<const TT*>(sm + L.qpM);
+    const TT* phiT = reinterpret_cast<const TT*>(sm + L.phiT);
+    const double* tr = reinterpret_cast<const double*>(sm + L.tr);
+    // ---- contraction, one pass over the QPs, no barrier
+    // Thread t < 252 owns component cc = t % 36 = (r, s) and nodes j = t / 36 + 7 b (b < 3);
+    // per QP it forms its own R0_q[j][cc], R1_q[j][cc] from row r of the QP matrices and
+    // column s of T_j, then accumulates S_ij[cc] += phi_i R0 + phi'_i R1 for every row i
+    // (the QP's phi pairs: broadcast vector loads, all rows at once).  Each R entry is formed by
+    // exactly one thread.
+    const int cc = tid % 36, jg = tid / 36;
+    const bool active = tid < kContractThreads;
+    const int r_ = cc / 6, s_ = cc - 6 * (cc / 6);
+    const bool rot = s_ >= 3;
+    const int col0 = rot ? 3 : 0;  // columns of A/P/O/C combined with T_j's column (s - 3)
+    TT tc[3][3];
+    int jc[3];
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+        const int j = jg + 7 * b;
+        const bool ok = active && j < N;
+        jc[b] = ok ? j : 0;
+#pragma unroll
+        for (int u = 0; u < 3; ++u)
+            tc[b][u] = !ok ? TT(0) : (rot ? (TT)tr[j * 9 + u * 3 + (s_ - 3)] : (u == s_ ? TT(1) : TT(0)));
+    }
+    TT acc[kMaxElemNodes][3];
+#pragma unroll
+    for (int i = 0; i < kMaxElemNodes; ++i)
+#pragma unroll
+        for (int b = 0; b < 3; ++b) acc[i][b] = TT(0);
+    const int Nu = __builtin_amdgcn_readfirstlane(N);  // uniform: a scalar branch per row group
+    for (int q = 0; q < Q; ++q) {
+        const TT* M6 = qpM + (size_t)q * 6 * 36 + r_ * 6;
+        TT ra[3], rp[3], ro[3], rc[3];
+#pragma unroll
+        for (int u = 0; u < 3; ++u) {
+            ra[u] = M6[col0 + u] + M6[36 + col0 + u];
+            rp[u] = M6[108 + col0 + u];
+            ro[u] = M6[144 + col0 + u];
+            rc[u] = M6[180 + col0 + u];
+        }
+        const TT rm = M6[72 + s_];
+        const TT* ph = phiT + (size_t)q * kPhiStride;
+        TT pj[3], dj[3];
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+            pj[b] = ph[jc[b] * 2];
+            dj[b] = ph[jc[b] * 2 + 1];
+        }
+        // every row's pair for this QP in registers (whole vectors, one wait), then the rows
+        constexpr int VW = 16 / (int)sizeof(TT);
+        typedef TT vec_t __attribute__((ext_vector_type(16 / sizeof(TT))));
+        TT phv[kPhiStride];
+#pragma unroll
+        for (int c = 0; c < kPhiStride / VW; ++c) {
+            const vec_t v = *reinterpret_cast<const vec_t*>(ph + c * VW);
+#pragma unroll
+            for (int e = 0; e < VW; ++e) phv[c * VW + e] = v[e];
+        }
+        TT R0[3], R1[3];
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+            TT v0 = pj[b] * rm, v1 = TT(0);
+#pragma unroll
+            for (int u = 0; u < 3; ++u) {
+                v0 += (pj[b] * ra[u] + dj[b] * rp[u]) * tc[b][u];
+                v1 += (pj[b] * ro[u] + dj[b] * rc[u]) * tc[b][u];
+            }
+            R0[b] = v0;
+            R1[b] = v1;
+        }
+        // rows in groups of three behind a scalar branch on N (rows past N are skipped)
+#pragma unroll
+        for (int i0 = 0; i0 < kMaxElemNodes; i0 += 3) {
+            if (i0 < Nu) {
+#pragma unroll
+                for (int i = i0; i < i0 + 3 && i < kMaxElemNodes; ++i) {
+                    const TT pi = phv[i * 2], di = phv[i * 2 + 1];
+#pragma unroll
+                    for (int b = 0; b < 3; ++b) acc[i][b] += pi * R0[b] + di * R1[b];
+                }
+            }
+        }
+    }
+    __syncthreads();
+
+    KYN_PHASE_MARK(4);
+    // ---- write out
+    if (kFused) {
+        // the overlay is free after the barrier above: zero the matrix view, then write
+        TT* A = reinterpret_cast<TT*>(sm + L.A);
+        {   // 16-byte stores; the matrix view is 16-byte aligned and reserved to a 16-byte multiple
+            uint4* A4 = reinterpret_cast<uint4*>(sm + L.A);
+            const int nv = (n * L.lda * (int)sizeof(TT) + 15) / 16;
+            for (int k = tid; k < nv; k += blockDim.x) A4[k] = make_uint4(0u, 0u, 0u, 0u);
+        }
+        __syncthreads();
+        const TT cond = (TT)S.p.conditioner;
+        if (active) {
+#pragma unroll
+            for (int i = 0; i < kMaxElemNodes; ++i) {
+                if (i < N) {
+                    const int row = loc[i] + r_;
+#pragma unroll
+                    for (int b = 0; b < 3; ++b) {
+                        const int j = jg + 7 * b;
+                        if (j < N) A[row * L.lda + loc[j] + s_] = acc[i][b] * cond;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    } else {
+        if (active) {
+            double* out = mat_out;
+#pragma unroll
+            for (int i = 0; i < kMaxElemNodes; ++i) {
+                if (i < N) {
+#pragma unroll
+                    for (int b = 0; b < 3; ++b) {
+                        const int j = jg + 7 * b;
+                        if (j < N) out[(size_t)(i * N + j) * 36 + cc] = (double)acc[i][b];
+                    }
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------
 // Element phase: gather, QP values, residual, contraction.
 //   kFused: residual goes into the block rhs (-f + R) and the tangent into LDS matrix A
 //           (times the conditioner, at block-local dofs); otherwise into res_out ([N][6]) and
@@ -498,133 +635,14 @@ __device__ __forceinline__ void element_phase(const DevSys& S, int e, char* sm, 
 
     KYN_PHASE_MARK(3);
     if constexpr (sizeof(TT) == 4) {
-        // FP32 tangent: the contraction on the matrix cores
-        contract_mfma_f32<kFused>(S, sm, L, n, N, Q, loc, mat_out);
+        // FP32 tangent: the matrix cores or the VALU register tile, per element size (the
+        // tuning table: bit N of contract_mfma_mask)
+        if ((S.contract_mfma_mask >> N) & 1u)
+            contract_mfma_f32<kFused>(S, sm, L, n, N, Q, loc, mat_out);
+        else
+            contract_valu<TT, kFused>(S, sm, L, n, N, Q, loc, mat_out);
     } else {
-        // ---- contraction, one pass over the QPs, no barrier
-        // Thread t < 252 owns component cc = t % 36 = (r, s) and nodes j = t / 36 + 7 b (b < 3);
-        // per QP it forms its own R0_q[j][cc], R1_q[j][cc] from row r of the QP matrices and
-        // column s of T_j, then accumulates S_ij[cc] += phi_i R0 + phi'_i R1 for every row i
-        // (the QP's phi pairs: broadcast vector loads, all rows at once).  Each R entry is formed by
-        // exactly one thread.
-        const int cc = tid % 36, jg = tid / 36;
-        const bool active = tid < kContractThreads;
-        const int r_ = cc / 6, s_ = cc - 6 * (cc / 6);
-        const bool rot = s_ >= 3;
-        const int col0 = rot ? 3 : 0;  // columns of A/P/O/C combined with T_j's column (s - 3)
-        TT tc[3][3];
-        int jc[3];
-#pragma unroll
-        for (int b = 0; b < 3; ++b) {
-            const int j = jg + 7 * b;
-            const bool ok = active && j < N;
-            jc[b] = ok ? j : 0;
-#pragma unroll
-            for (int u = 0; u < 3; ++u)
-                tc[b][u] = !ok ? TT(0) : (rot ? (TT)tr[j * 9 + u * 3 + (s_ - 3)] : (u == s_ ? TT(1) : TT(0)));
-        }
-        TT acc[kMaxElemNodes][3];
-#pragma unroll
-        for (int i = 0; i < kMaxElemNodes; ++i)
-#pragma unroll
-            for (int b = 0; b < 3; ++b) acc[i][b] = TT(0);
-        const int Nu = __builtin_amdgcn_readfirstlane(N);  // uniform: a scalar branch per row group
-        for (int q = 0; q < Q; ++q) {
-            const TT* M6 = qpM + (size_t)q * 6 * 36 + r_ * 6;
-            TT ra[3], rp[3], ro[3], rc[3];
-#pragma unroll
-            for (int u = 0; u < 3; ++u) {
-                ra[u] = M6[col0 + u] + M6[36 + col0 + u];
-                rp[u] = M6[108 + col0 + u];
-                ro[u] = M6[144 + col0 + u];
-                rc[u] = M6[180 + col0 + u];
-            }
-            const TT rm = M6[72 + s_];
-            const TT* ph = phiT + (size_t)q * kPhiStride;
-            TT pj[3], dj[3];
-#pragma unroll
-            for (int b = 0; b < 3; ++b) {
-                pj[b] = ph[jc[b] * 2];
-                dj[b] = ph[jc[b] * 2 + 1];
-            }
-            // every row's pair for this QP in registers (whole vectors, one wait), then the rows
-            constexpr int VW = 16 / (int)sizeof(TT);
-            typedef TT vec_t __attribute__((ext_vector_type(16 / sizeof(TT))));
-            TT phv[kPhiStride];
-#pragma unroll
-            for (int c = 0; c < kPhiStride / VW; ++c) {
-                const vec_t v = *reinterpret_cast<const vec_t*>(ph + c * VW);
-#pragma unroll
-                for (int e = 0; e < VW; ++e) phv[c * VW + e] = v[e];
-            }
-            TT R0[3], R1[3];
-#pragma unroll
-            for (int b = 0; b < 3; ++b) {
-                TT v0 = pj[b] * rm, v1 = TT(0);
-#pragma unroll
-                for (int u = 0; u < 3; ++u) {
-                    v0 += (pj[b] * ra[u] + dj[b] * rp[u]) * tc[b][u];
-                    v1 += (pj[b] * ro[u] + dj[b] * rc[u]) * tc[b][u];
-                }
-                R0[b] = v0;
-                R1[b] = v1;
-            }
-            // rows in groups of three behind a scalar branch on N (rows past N are skipped)
-#pragma unroll
-            for (int i0 = 0; i0 < kMaxElemNodes; i0 += 3) {
-                if (i0 < Nu) {
-#pragma unroll
-                    for (int i = i0; i < i0 + 3 && i < kMaxElemNodes; ++i) {
-                        const TT pi = phv[i * 2], di = phv[i * 2 + 1];
-#pragma unroll
-                        for (int b = 0; b < 3; ++b) acc[i][b] += pi * R0[b] + di * R1[b];
-                    }
-                }
-            }
-        }
-        __syncthreads();
-
-        KYN_PHASE_MARK(4);
-        // ---- write out
-        if (kFused) {
-            // the overlay is free after the barrier above: zero the matrix view, then write
-            TT* A = reinterpret_cast<TT*>(sm + L.A);
-            {   // 16-byte stores; the matrix view is 16-byte aligned and reserved to a 16-byte multiple
-                uint4* A4 = reinterpret_cast<uint4*>(sm + L.A);
-                const int nv = (n * L.lda * (int)sizeof(TT) + 15) / 16;
-                for (int k = tid; k < nv; k += blockDim.x) A4[k] = make_uint4(0u, 0u, 0u, 0u);
-            }
-            __syncthreads();
-            const TT cond = (TT)S.p.conditioner;
-            if (active) {
-#pragma unroll
-                for (int i = 0; i < kMaxElemNodes; ++i) {
-                    if (i < N) {
-                        const int row = loc[i] + r_;
-#pragma unroll
-                        for (int b = 0; b < 3; ++b) {
-                            const int j = jg + 7 * b;
-                            if (j < N) A[row * L.lda + loc[j] + s_] = acc[i][b] * cond;
-                        }
-                    }
-                }
-            }
-            __syncthreads();
-        } else {
-            if (active) {
-                double* out = mat_out;
-#pragma unroll
-                for (int i = 0; i < kMaxElemNodes; ++i) {
-                    if (i < N) {
-#pragma unroll
-                        for (int b = 0; b < 3; ++b) {
-                            const int j = jg + 7 * b;
-                            if (j < N) out[(size_t)(i * N + j) * 36 + cc] = (double)acc[i][b];
-                        }
-                    }
-                }
-            }
-        }
+        contract_valu<TT, kFused>(S, sm, L, n, N, Q, loc, mat_out);
     }
 }
 

@@ -137,6 +137,8 @@ struct DevSys {
     //      ascending) by one workgroup per node / constraint (its columns, contributions in the
     //      oracle's order).  Blocks that do not condense are expanded to a dense column-major
     //      matrix for a library LU.
+    unsigned contract_mfma_mask;      // FP32 tangent: bit N set -> element size N contracts on the
+                                      // matrix cores, else on the VALU register tile
     const int* block_large;           // 1 if the block takes the large path
     const int64_t* crs_rp_off;        // per block: into crs_rp (n + 1 block-local row pointers)
     const int64_t* crs_nz_off;        // per block: into crs_col / crs_val

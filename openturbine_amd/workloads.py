@@ -661,7 +661,7 @@ def _rotor_spec(name, x0, u, v, vd, s, beams, cons, roots, gravity, h, num_steps
 
 def ensemble_config_specs(num_rotors=4096, h=0.01, num_steps=100, rho_inf=0.9, max_iter=50,
                           atol=1e-10, rtol=1e-6, seed=SEED, tip_load_sigma=1e4, hub_radius=2.0,
-                          gravity=(0.0, 0.0, -9.81), first_rotor=0, tip_load_tau=0.1):
+                          gravity=(0.0, 0.0, -9.81), first_rotor=0, tip_load_tau=0.1, fixed_nodes=None):
     """The batched ensemble: independent 3-blade rotors, per-blade N uniform in
     {6, 11, 16, 21}, Q = 2N-2, L uniform in [30, 120] m, omega uniform in [0.5, 1.5] rad/s,
     per-step tip loads normal(0, tip_load_sigma) N.  Sections are the config-1 base section
@@ -672,12 +672,13 @@ def ensemble_config_specs(num_rotors=4096, h=0.01, num_steps=100, rho_inf=0.9, m
     loads drawn afresh each step (tip_load_tau = 0) keep pumping energy into the high modes of
     the light GLL tip nodes until the reference's Newton iteration diverges (the oracle: 7 of
     the first 64 rotors within 150 steps, the first at step 75; none with tau = 0.1 s).
+    fixed_nodes: every blade with that many nodes (single-size ensembles for tuning).
     Returns one ModelSpec per rotor."""
     specs = []
     for r in range(first_rotor, first_rotor + num_rotors):
         keys = np.uint64(r) * np.uint64(8) + np.arange(8, dtype=np.uint64)
         u = uniform01(seed + 1, keys)
-        Ns = [6 + 5 * min(int(u[b] * 4.0), 3) for b in range(3)]
+        Ns = [6 + 5 * min(int(u[b] * 4.0), 3) for b in range(3)] if fixed_nodes is None else [int(fixed_nodes)] * 3
         Ls = [30.0 + 90.0 * u[3 + b] for b in range(3)]
         om = 0.5 + float(u[6])
         nt = _NodeTable()

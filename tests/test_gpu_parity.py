@@ -273,17 +273,21 @@ def mixed_rotor():
     raise AssertionError("no mixed rotor")
 
 
+@pytest.mark.parametrize("engine", ["mfma", "valu"])
 @pytest.mark.parametrize("case", ["rotor", "cantilever"])
-def test_fp32_element_terms_on_the_matrix_cores(kyn, orc, case):
+def test_fp32_element_terms_on_the_matrix_cores(kyn, orc, case, engine):
     """The FP32 tangent's node-pair contraction (v_mfma_f32_16x16x4f32 tiles; the plain form's
     FP32 option writes its result out) against the oracle's FP64 element terms: every element of
     a rotor with N = 21 (two 16-row tiles) and smaller blades (Q = 2N - 2), and the config-1
-    cantilever (N = 8, Q = 11: the odd-Q tail).  Bar: 2e-6 of the element matrix's largest
+    cantilever (N = 8, Q = 11: the odd-Q tail).  Both engines the tuning table chooses between
+    (`engine`: matrix cores, or the VALU register tile in FP32) meet the same bars.  Bar: 2e-6 of
+    the element matrix's largest
     entry (FP32 QP matrices and products over 2Q terms).  The residual stays FP64: 1e-11 (the
     strain x0' + u' - R x0' cancels most of its terms, so the FMA contractions the compiler picks
     in this instantiation move it by a few 1e-12 of the largest entry)."""
     spec = mixed_rotor() if case == "rotor" else W.cantilever_config_spec(num_steps=2)
     g = ota.KynSystem(kyn, spec, exec_mode=ota.EXEC_PLAIN, precision=ota.TANGENT_FP32)
+    assert kyn.L.kyn_debug_set_contraction_mask(g.h, 0x3FFFFF if engine == "mfma" else 0) == ota.KYN_OK
     o = OracleSystem(orc, spec)
     inputs = spec.step_inputs(0, o.get) if spec.step_inputs else None
     g.debug_assemble(inputs)
@@ -293,7 +297,7 @@ def test_fp32_element_terms_on_the_matrix_cores(kyn, orc, case):
         r_g, m_g = g.element_terms(e, N)
         r_o, m_o = o.element_terms(e, N, update_tangent=True)
         dm, dr = rel(m_g, m_o), rel(r_g, r_o)
-        print(f"{case} element {e} (N = {N}, Q = {len(spec.beams[e].quad)}): FP32 matrix rel {dm:.1e}, "
+        print(f"{case} element {e} (N = {N}, Q = {len(spec.beams[e].quad)}, {engine}): FP32 matrix rel {dm:.1e}, "
               f"residual rel {dr:.1e}", flush=True)
         assert dm < 2e-6 and dr < 1e-11, (e, dm, dr)
 
