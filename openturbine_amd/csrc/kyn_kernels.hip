@@ -1174,12 +1174,14 @@ __device__ __forceinline__ void lu_factor_augmented(TT* A, int n, int lda, int* 
 // complement -B (beta h^2 K)^-1 B^T, after which nothing else is eliminated.  (The FP64 forms
 // keep the pivoted LU above, whose factors are the oracle's bit for bit.)
 //
-// Panels of 16 columns, three phases each, one barrier after each:
+// Panels of 16 columns, two barriers each:
 //  P1  every wave factors the 16 x 16 diagonal tile in lanes 0..15 (row per lane, each pivot
-//      row to the wave by readlane: no cross-wave traffic) and applies it, as it goes, to up to
-//      48 rows below in lanes 16..63 (L21 = A21 U11^-1, rows split over the waves); wave 0
-//      writes the tile back and keeps L11 as a compact column-major tile for P2;
-//  P2  U12 = L11^-1 A12, one column per thread (right-hand side column n included);
+//      row to the wave by readlane, the pivot's hardware reciprocal: no cross-wave traffic)
+//      and applies it, as it goes, to up to 48 rows below in lanes 16..63 (L21 = A21 U11^-1,
+//      rows split over the waves);
+//  P2  in the same wave, with no barrier: U12 = L11^-1 A12 for 48 columns per wave in lanes
+//      16..63 (right-hand side column n included), L11 read by readlane from the wave's own
+//      copy of the factored tile; then the barrier, after which wave 0 writes the tile back;
 //  P3  A22 -= L21 U12 on the matrix cores: 16 x 16 output tiles, each the sum of four
 //      v_mfma_f32_16x16x4f32 products over the panel's 16 columns, tiles round-robin over the
 //      waves, rows past n - 1 and columns past n masked.
@@ -1193,7 +1195,7 @@ __device__ __forceinline__ void np_tile_steps(float (&a)[kPanel], int w, int lan
             float pr[kPanel];
 #pragma unroll
             for (int c = J; c < kPanel; ++c) pr[c] = rdlane(a[c], J);  // pivot row J (lane J)
-            const float inv = 1.f / pr[J];
+            const float inv = __builtin_amdgcn_rcpf(pr[J]);  // hardware reciprocal (1 ulp): FP64 refinement follows
             if (lane > J) {  // tile rows below the pivot and every L21 row
                 const float l = a[J] * inv;
                 a[J] = l;
@@ -1209,12 +1211,12 @@ __device__ __forceinline__ void lu_factor_nopiv_f32(float* A, int n, int lda, co
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int nw = blockDim.x >> 6;
     for (int r = tid; r < kLuTile; r += blockDim.x) W.phys_of[r] = r;
-    float* Lc = reinterpret_cast<float*>(W.lcol);
     for (int k0 = 0; k0 < n; k0 += kPanel) {
+        KYN_SPAN_BEGIN(t_p1);
         const int w = (n - k0) < kPanel ? (n - k0) : kPanel;
         const int r0 = k0 + kPanel;  // first row / column after the panel
         // ---- P1: diagonal tile + this wave's rows below.  Every wave reads the tile, so it is
-        // written back only after the barrier (by wave 0, during P2, which does not touch it);
+        // written back only after the barrier (by wave 0, during P3, which does not touch it);
         // the L21 rows belong to one wave each and go back at once.
         float a[kPanel];
         {
@@ -1232,36 +1234,43 @@ __device__ __forceinline__ void lu_factor_nopiv_f32(float* A, int n, int lda, co
 #pragma unroll
                 for (int c = 0; c < kPanel; ++c) A[row * lda + k0 + c] = a[c];
             }
-            if (wv == 0 && lane < kPanel) {  // compact unit-lower L11, column-major, zeros elsewhere
+        }
+        KYN_SPAN_END(7, t_p1, 0);
+        KYN_SPAN_BEGIN(t_p2);
+        // ---- P2, before the barrier: U12 = L11^-1 A12 for the columns after the panel (the rhs
+        // column n included; a partial last panel has only the rhs after it), 48 columns per wave
+        // in lanes 16..63.  L11 is this wave's own copy of the factored tile (lane i holds row i:
+        // L[i][k] = a[k] for k < i), read by readlane.  These columns are disjoint from every
+        // entry P1 reads or writes in any wave.
+        for (int jb = k0 + w + wv * (64 - kPanel); jb <= n; jb += nw * (64 - kPanel)) {
+            const int j = jb + lane - kPanel;
+            const bool okj = lane >= kPanel && j <= n;
+            const int js = okj ? j : k0;  // clamped: loads unconditional
+            float u[kPanel];
 #pragma unroll
-                for (int k = 0; k < kPanel; ++k) Lc[k * kPanel + lane] = (k < lane && lane < w) ? a[k] : 0.f;
+            for (int i = 0; i < kPanel; ++i) {
+                u[i] = A[(k0 + (i < w ? i : 0)) * lda + js];
+                pin(u[i]);
+                u[i] = (okj && i < w) ? u[i] : 0.f;
+            }
+#pragma unroll
+            for (int i = 1; i < kPanel; ++i)
+#pragma unroll
+                for (int k = 0; k < i; ++k) u[i] = __builtin_fmaf(-rdlane(a[k], i), u[k], u[i]);
+            if (okj) {
+#pragma unroll
+                for (int i = 1; i < kPanel; ++i)
+                    if (i < w) A[(k0 + i) * lda + j] = u[i];
             }
         }
         __syncthreads();
+        KYN_SPAN_END(8, t_p2, 0);
+        KYN_SPAN_BEGIN(t_p3);
         if (wv == 0 && lane < w) {  // the factored tile: U11 and the unit L11 below its diagonal
 #pragma unroll
             for (int c = 0; c < kPanel; ++c)
                 if (c < w) A[(k0 + lane) * lda + k0 + c] = a[c];
         }
-        // ---- P2: U12 rows (columns r0 .. n, the rhs included); a partial last panel has only
-        // the rhs column after it
-        for (int j = k0 + w + tid; j <= n; j += blockDim.x) {
-            float u[kPanel];
-#pragma unroll
-            for (int i = 0; i < kPanel; ++i) {
-                u[i] = A[(k0 + (i < w ? i : 0)) * lda + j];
-                pin(u[i]);
-                u[i] = i < w ? u[i] : 0.f;
-            }
-            u_rows_group<float, 0>(Lc, u);
-            u_rows_group<float, 4>(Lc, u);
-            u_rows_group<float, 8>(Lc, u);
-            u_rows_group<float, 12>(Lc, u);
-#pragma unroll
-            for (int i = 1; i < kPanel; ++i)
-                if (i < w) A[(k0 + i) * lda + j] = u[i];
-        }
-        __syncthreads();
         // ---- P3: trailing update on MFMA (full panels with rows after them only)
         const int m = n - r0;       // trailing rows
         const int mc = n + 1 - r0;  // trailing columns, rhs included
@@ -1299,6 +1308,7 @@ __device__ __forceinline__ void lu_factor_nopiv_f32(float* A, int n, int lda, co
             }
             __syncthreads();
         }
+        KYN_SPAN_END(9, t_p3, 0);
     }
 }
 

@@ -11,7 +11,7 @@ __device__ unsigned long long kyn_phase_acc[16];
         if (threadIdx.x == 0) atomicAdd(&kyn_phase_acc[i], (unsigned long long)clock64()); \
     } while (0)
 
-__device__ unsigned long long kyn_span_acc[8];
+__device__ unsigned long long kyn_span_acc[12];
 #define KYN_SPAN_BEGIN(v) const unsigned long long v = clock64()
 #define KYN_SPAN_END(slot, v, who)                                                              \
     do {                                                                                        \
@@ -22,10 +22,10 @@ __device__ unsigned long long kyn_span_acc[8];
 
 extern "C" int kyn_phase_read(unsigned long long* out) {
     if (hipMemcpyFromSymbol(out, HIP_SYMBOL(kyn_phase_acc), sizeof(unsigned long long) * 16) != hipSuccess) return -1;
-    return hipMemcpyFromSymbol(out + 16, HIP_SYMBOL(kyn_span_acc), sizeof(unsigned long long) * 8) == hipSuccess ? 0 : -1;
+    return hipMemcpyFromSymbol(out + 16, HIP_SYMBOL(kyn_span_acc), sizeof(unsigned long long) * 12) == hipSuccess ? 0 : -1;
 }
 extern "C" int kyn_phase_reset() {
     unsigned long long z[16] = {0};
     if (hipMemcpyToSymbol(HIP_SYMBOL(kyn_phase_acc), z, sizeof(z)) != hipSuccess) return -1;
-    return hipMemcpyToSymbol(HIP_SYMBOL(kyn_span_acc), z, sizeof(unsigned long long) * 8) == hipSuccess ? 0 : -1;
+    return hipMemcpyToSymbol(HIP_SYMBOL(kyn_span_acc), z, sizeof(unsigned long long) * 12) == hipSuccess ? 0 : -1;
 }

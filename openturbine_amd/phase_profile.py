@@ -45,7 +45,7 @@ def main():
     lib.L.kyn_phase_reset()
     s.step_many(args.steps, use_graph=False)
     it1, _ = s.totals()
-    acc = (C.c_ulonglong * 24)()
+    acc = (C.c_ulonglong * 28)()
     lib.L.kyn_phase_read(acc)
     a = [int(v) for v in acc]
     blocks_per_model = np.array([len(sp.beams) for sp in specs])
@@ -60,7 +60,8 @@ def main():
     print(f"  {'total per block-iteration':32s} {total:10.0f} cycles")
     spans = ["LU wait for look-ahead (wave 0)", "LU wait + panel (wave 0)", "LU look-ahead + trailing (wave 1)",
              "LU panel steps incl. barrier", "refinement residual (HBM copy)", "refinement solve",
-             "refinement forward sweep (wave 0)"]
+             "refinement forward sweep (wave 0)", "unpivoted LU: diagonal tile + L21", "unpivoted LU: U12 rows",
+             "unpivoted LU: MFMA trailing update"]
     for i, name in enumerate(spans):
         if i < 4 or a[16 + i]:
             print(f"    {name:34s} {a[16 + i] / count:10.0f} cycles")
