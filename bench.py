@@ -2,15 +2,18 @@
 
 Workload (one "step" = one generalized-alpha time step of every model): 4096 independent
 3-blade rotors, one beam element per blade with N uniform in {6, 11, 16, 21} GLL nodes and
-Q = 2N - 2 Gauss points, L in [30, 120] m, hub speed in [0.5, 1.5] rad/s, gravity, normal
-tip loads (sigma 1e4 N) every step, h = 0.01 s, rho_inf = 0.9, Newton with the reference's
-increment criterion (atol 1e-10, rtol 1e-6).  Inputs are synthetic (SplitMix64 seed
+Q = 2N - 2 Gauss points, L in [30, 120] m, hub speed in [0.5, 1.5] rad/s, gravity, tip loads
+normal(0, 1e4 N) in every step (correlated from step to step over 0.1 s: with loads drawn
+afresh each step the reference's Newton iteration diverges on some rotors), h = 0.01 s,
+rho_inf = 0.9, Newton with the reference's increment criterion (atol 1e-10, rtol 1e-6).  Inputs are synthetic (SplitMix64 seed
 20261202) and resident in HBM (device input schedule) before timing starts.
 
 Timed legs (all on the one GPU; each the median of --repeats runs of --steps steps after the
 warm-up, every run timed with a synchronize on both sides of its step loop):
   value            fused FP32-tangent form with --refine FP64 iterative-refinement sweeps per
-                   solve, one hipGraph-replayed kernel per step
+                   solve (in every Newton iteration unless --refine-bound), one
+                   hipGraph-replayed kernel per step; null (with `error` set) if any model-step
+                   of this leg failed to converge
   fp32_without_refinement  the same without refinement
   fused_fp64       fused form, FP64 tangent
   plain_fp64       the reference's launch structure (one kernel per stage, host reads the
@@ -23,7 +26,10 @@ roofline it is measured against), so the matrix-core rate of its MFMA work (cont
 accuracy (extra field, untimed): the FP32-tangent and FP64 fused forms against the CPU FP64
 oracle on the first --acc-rotors rotors over --acc-steps steps (BASELINE: 100 steps): final q
 per model relative to that model's max |q|, worst and median, models above 1e-9, and Newton
-iteration totals that differ from the oracle's.
+iteration totals that differ from the oracle's.  The FP32 form is an inexact-tangent Newton
+method: where its path takes one iteration more or less than the oracle's in some step, the
+final state differs by up to a few 1e-9 of the model's max |q| (the figure is reported as
+measured, per model, not held to a bar here; tests/test_gpu_parity.py holds smaller cases).
 config3 (extra field, untimed): the FP32-tangent form of the config-1 cantilever over 200 steps
 against the oracle: max-norm tip displacement and rotation-vector history error, Newton counts,
 refinement sweeps per solve.
@@ -31,8 +37,9 @@ cpu_baseline: the CPU oracle (same algorithm, dense pivoted LU instead of KLU), 
 -march=native on the host it runs on, on a bounded sample of the same ensemble, OpenMP over
 rotors on all host cores; a single-thread run of a smaller sample beside it.
 structure (extra field): the large single structure (tower 8 + shaft 8 + 3 x 16 blade elements
-of 21 nodes / 40 points, joints, 8459 dofs) in the plain form, FP64 and FP32 element factors
-with two FP64 refinement sweeps; steps/s and the element kernel's algorithmic HBM rate.
+of 21 nodes / 40 points, joints, 8459 dofs) in the plain form (CRS assembly, condensed solve),
+FP64 and the FP32-tangent option (FP32 element factors, two FP64 refinement sweeps); steps/s
+and the element kernel's algorithmic HBM rate.
 """
 
 import argparse
