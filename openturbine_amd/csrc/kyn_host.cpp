@@ -13,9 +13,11 @@
 #include <array>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <memory>
 #include <numeric>
+#include <set>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -38,6 +40,7 @@ void launch_plain_update(const DevSys& S, int iter, hipStream_t st);
 void launch_large_uncondition(const DevSys& S, int b, int n, hipStream_t st);
 void launch_crs_residual(const DevSys& S, int b, int n, const double* x, double* r, hipStream_t st);
 void launch_crs_to_dense(const DevSys& S, int b, int n, hipStream_t st);
+void launch_tree_interface(const DevSys& S, int b, hipStream_t st);
 size_t cond_lds(int ne, int me);
 void launch_cond_factor(const DevSys& S, size_t lds, bool fp32, const double* rin, hipStream_t st);
 void launch_cond_interface(const DevSys& S, int b, const double* rin, hipStream_t st);
@@ -245,6 +248,7 @@ struct kyn_system_s {
     std::vector<int64_t> large_off;
     std::vector<int> large_cond, large_nz;  // condensed solve (else dense), interface size
     std::vector<int64_t> crs_rp_off, crs_nz_off;  // per block (large blocks only)
+    std::vector<int> tree_nvert;                  // per block: vertices of its interface tree (0: dense solve)
     std::vector<int64_t> large_Z_off;
     size_t cond_lds_bytes{0};
     rocblas_handle blas{};
@@ -302,6 +306,136 @@ struct Builder {
     int vec_total = 0;
     int64_t mat_total = 0;
 };
+
+// Interface system of a condensed block as a tree of small blocks ("vertices": the rows of one
+// constraint, or the six dofs of one free node).  Two vertices are coupled when their rows can
+// share an entry of Z = A[z, z] - sum_e B_e K_e^-1 B_e^T: constraints touching one element
+// (through S_e), a constraint and a free node it acts on.  When that graph is a forest, block
+// elimination from the leaves has no fill: per vertex v, S_v = Z_vv - sum_children Z_vc X_c,
+// X_v = S_v^-1 Z_v,parent, zhat_v = S_v^-1 (r_v - sum_children Z_vc zhat_c), then from the roots
+// z_v = zhat_v - X_v z_parent.  Vertices of one height (elimination) or depth (substitution)
+// are independent.  Returns false (dense solve) when the graph has a cycle, a vertex has more
+// than kTreeMaxBlock rows, there are more than kTreeMaxVerts vertices, or a free node would be a
+// leaf (a massless one would leave its block singular).
+struct TreeSchedule {
+    std::vector<int> size, row_off, rows, parent, child_off, children;  // per vertex (block-local ids)
+    std::vector<int> elim, elim_lvl, back, back_lvl;                    // vertex ids by level; level offsets
+};
+
+bool build_interface_tree(const Builder& B, int b, const std::map<int, int>& owner,
+                          const std::vector<std::vector<int>>& elam, const std::vector<int>& zidx, TreeSchedule& T) {
+    std::vector<std::vector<int>> vrows;
+    std::vector<int> is_free;
+    std::map<int, int> vert_of_row;  // block-local constraint row -> vertex
+    std::map<int, int> vert_of_free;  // free node -> vertex
+    for (int q = B.block_con_begin[b]; q < B.block_con_end[b]; ++q) {
+        const int c = B.block_cons[q];
+        std::vector<int> r;
+        for (int k = 0; k < B.con_rows[c]; ++k) {
+            r.push_back(zidx[B.con_local[c] + k]);
+            vert_of_row[B.con_local[c] + k] = static_cast<int>(vrows.size());
+        }
+        vrows.push_back(r);
+        is_free.push_back(0);
+    }
+    for (int q = B.block_node_begin[b]; q < B.block_node_end[b]; ++q) {
+        const int node = B.block_nodes[q];
+        if (owner.count(node)) continue;
+        std::vector<int> r;
+        for (int k = 0; k < 6; ++k) r.push_back(zidx[B.node_local[node] + k]);
+        vert_of_free[node] = static_cast<int>(vrows.size());
+        vrows.push_back(r);
+        is_free.push_back(1);
+    }
+    const int V = static_cast<int>(vrows.size());
+    if (V == 0 || V > kyn::kTreeMaxVerts) return false;
+    for (const auto& r : vrows)
+        if (static_cast<int>(r.size()) > kyn::kTreeMaxBlock) return false;
+    std::set<std::pair<int, int>> edges;
+    auto edge = [&](int u, int v) {
+        if (u != v) edges.insert({std::min(u, v), std::max(u, v)});
+    };
+    for (const auto& l : elam) {
+        std::set<int> vs;
+        for (int row : l) vs.insert(vert_of_row.at(row));
+        for (int u : vs)
+            for (int v : vs) edge(u, v);
+    }
+    for (int q = B.block_con_begin[b]; q < B.block_con_end[b]; ++q) {
+        const int c = B.block_cons[q];
+        const int vc = vert_of_row.at(B.con_local[c]);
+        const int nodes[2] = {B.con_target[c], con_two_node(B.con_type[c]) ? B.con_base[c] : -1};
+        for (int node : nodes)
+            if (node >= 0 && vert_of_free.count(node)) edge(vc, vert_of_free[node]);
+    }
+    // forest check (union-find), adjacency
+    std::vector<int> uf(V);
+    std::iota(uf.begin(), uf.end(), 0);
+    std::function<int(int)> find = [&](int x) { return uf[x] == x ? x : uf[x] = find(uf[x]); };
+    std::vector<std::vector<int>> adj(V);
+    for (const auto& e : edges) {
+        const int a = find(e.first), c = find(e.second);
+        if (a == c) return false;  // a cycle: fill-in, not this solver
+        uf[a] = c;
+        adj[e.first].push_back(e.second);
+        adj[e.second].push_back(e.first);
+    }
+    // roots: per component the free node of highest degree, else the vertex of highest degree
+    std::vector<int> parent(V, -2), depth(V, 0), order;
+    for (int s = 0; s < V; ++s) {
+        if (parent[s] != -2) continue;
+        std::vector<int> comp{s};
+        for (size_t k = 0; k < comp.size(); ++k)
+            for (int u : adj[comp[k]])
+                if (std::find(comp.begin(), comp.end(), u) == comp.end()) comp.push_back(u);
+        int root = comp[0];
+        for (int u : comp) {
+            const auto key = [&](int x) { return std::make_pair(is_free[x], static_cast<int>(adj[x].size())); };
+            if (key(u) > key(root)) root = u;
+        }
+        parent[root] = -1;
+        std::vector<int> bfs{root};
+        for (size_t k = 0; k < bfs.size(); ++k)
+            for (int u : adj[bfs[k]])
+                if (parent[u] == -2) {
+                    parent[u] = bfs[k];
+                    depth[u] = depth[bfs[k]] + 1;
+                    bfs.push_back(u);
+                }
+        order.insert(order.end(), bfs.begin(), bfs.end());
+    }
+    std::vector<std::vector<int>> ch(V);
+    for (int v = 0; v < V; ++v)
+        if (parent[v] >= 0) ch[parent[v]].push_back(v);
+    for (int v = 0; v < V; ++v)
+        if (is_free[v] && ch[v].empty() && parent[v] >= 0) return false;  // free-node leaf
+    std::vector<int> height(V, 0);
+    for (auto it = order.rbegin(); it != order.rend(); ++it)
+        for (int c : ch[*it]) height[*it] = std::max(height[*it], height[c] + 1);
+    T = TreeSchedule{};
+    for (int v = 0; v < V; ++v) {
+        T.size.push_back(static_cast<int>(vrows[v].size()));
+        T.row_off.push_back(static_cast<int>(T.rows.size()));
+        T.rows.insert(T.rows.end(), vrows[v].begin(), vrows[v].end());
+        T.parent.push_back(parent[v]);
+        T.child_off.push_back(static_cast<int>(T.children.size()));
+        T.children.insert(T.children.end(), ch[v].begin(), ch[v].end());
+    }
+    T.row_off.push_back(static_cast<int>(T.rows.size()));
+    T.child_off.push_back(static_cast<int>(T.children.size()));
+    auto levels = [&](const std::vector<int>& key, std::vector<int>& ids, std::vector<int>& off) {
+        const int L = *std::max_element(key.begin(), key.end()) + 1;
+        off.assign(1, 0);
+        for (int l = 0; l < L; ++l) {
+            for (int v = 0; v < V; ++v)
+                if (key[v] == l) ids.push_back(v);
+            off.push_back(static_cast<int>(ids.size()));
+        }
+    };
+    levels(height, T.elim, T.elim_lvl);
+    levels(depth, T.back, T.back_lvl);
+    return true;
+}
 
 // CRS graph of one large block's KKT matrix in block-local dofs (the graph of
 // create_full_matrix.hpp:19-74: compute_row_ptrs, compute_col_inds, then sort_crs_graph).  A
@@ -740,6 +874,11 @@ void solve_large_blocks(kyn_system_s* s) {
                 const int b = s->large_blocks[k], nz = s->large_nz[k];
                 if (!s->large_cond[k] || !s->model_active_host[s->block_model[b]]) continue;
                 kyn::launch_cond_interface(S, b, rin, st);
+                if (s->tree_nvert[b] > 0) {  // tree-structured interface: block elimination in one launch
+                    kyn::launch_tree_interface(S, b, st);
+                    hip_check(hipGetLastError(), "k_tree_interface launch");
+                    continue;
+                }
                 double* Z = S.Zbuf + s->large_Z_off[k];
                 double* z = S.zrhs + zoff_host(s, b);
                 blas_check(rocsolver_dgetrf(s->blas, nz, nz, Z, nz, s->ipiv, s->info), "rocsolver_dgetrf");
@@ -1210,6 +1349,9 @@ kyn_system kyn_system_create(const kyn_model* models, size_t num_models, const k
                 int64_t Y_tot = 0, S_tot = 0, Z_tot = 0;
                 std::vector<int> dof_ce(static_cast<size_t>(B.vec_total), -1), dof_cl(static_cast<size_t>(B.vec_total), -1),
                     dof_z(static_cast<size_t>(B.vec_total), -1);
+                std::vector<int> tree_block(6 * B.block_n.size(), 0), tv_size, tv_row_off, tv_rows, tv_parent, tv_child_off,
+                    tv_children, t_elim, t_elim_lvl, t_back, t_back_lvl;
+                std::vector<int> block_color(2 * B.block_n.size(), 0), color_off{0}, color_ce;
                 std::vector<int> nz_of(B.block_n.size(), 0), cond_of(B.block_n.size(), 0), nce_of(B.block_n.size(), 0);
                 for (int b : sys->large_blocks) {
                     bool ok = true;
@@ -1263,6 +1405,32 @@ kyn_system kyn_system_create(const kyn_model* models, size_t num_models, const k
                         }
                     nz_of[b] = nz;
                     for (int i = 0; i < B.block_n[b]; ++i) dof_z[B.block_vec_off[b] + i] = zidx[i];
+                    {   // tree schedule of the interface system, if its graph is a forest
+                        TreeSchedule T;
+                        if (build_interface_tree(B, b, owner, elam, zidx, T)) {
+                            const int vb = static_cast<int>(tv_size.size());
+                            tree_block[6 * b + 0] = vb;
+                            tree_block[6 * b + 1] = static_cast<int>(T.size.size());
+                            for (size_t v = 0; v < T.size.size(); ++v) {
+                                tv_size.push_back(T.size[v]);
+                                tv_row_off.push_back(static_cast<int>(tv_rows.size()));
+                                tv_rows.insert(tv_rows.end(), T.rows.begin() + T.row_off[v], T.rows.begin() + T.row_off[v + 1]);
+                                tv_parent.push_back(T.parent[v] >= 0 ? vb + T.parent[v] : -1);
+                                tv_child_off.push_back(static_cast<int>(tv_children.size()));
+                                for (int k = T.child_off[v]; k < T.child_off[v + 1]; ++k) tv_children.push_back(vb + T.children[k]);
+                            }
+                            auto put = [&](const std::vector<int>& ids, const std::vector<int>& off, std::vector<int>& gids,
+                                           std::vector<int>& goff, int slot) {
+                                tree_block[6 * b + slot] = static_cast<int>(goff.size());
+                                tree_block[6 * b + slot + 1] = static_cast<int>(off.size()) - 1;
+                                const int base = static_cast<int>(gids.size());
+                                for (int o : off) goff.push_back(base + o);
+                                for (int v : ids) gids.push_back(vb + v);
+                            };
+                            put(T.elim, T.elim_lvl, t_elim, t_elim_lvl, 2);
+                            put(T.back, T.back_lvl, t_back, t_back_lvl, 4);
+                        }
+                    }
                     Z_off[b] = Z_tot;
                     Z_tot += static_cast<int64_t>(nz) * nz;
                     int idx = 0;
@@ -1290,6 +1458,31 @@ kyn_system kyn_system_create(const kyn_model* models, size_t num_models, const k
                         S_tot += static_cast<int64_t>(me) * (me + 1);
                     }
                     nce_of[b] = B.block_elem_end[b] - B.block_elem_begin[b];
+                    {   // colour the block's condensed elements: no two of one colour share a constraint row
+                        const int ce0 = static_cast<int>(ce_elem.size()) - nce_of[b];
+                        std::vector<int> colour(nce_of[b], -1);
+                        int ncol = 0;
+                        for (int a = 0; a < nce_of[b]; ++a) {
+                            std::set<int> used;
+                            for (int c = 0; c < a; ++c) {
+                                bool share = false;
+                                for (int r1 : elam[a])
+                                    for (int r2 : elam[c]) share |= r1 == r2;
+                                if (share) used.insert(colour[c]);
+                            }
+                            int k = 0;
+                            while (used.count(k)) ++k;
+                            colour[a] = k;
+                            ncol = std::max(ncol, k + 1);
+                        }
+                        block_color[2 * b] = static_cast<int>(color_off.size()) - 1;
+                        block_color[2 * b + 1] = ncol;
+                        for (int k = 0; k < ncol; ++k) {
+                            for (int a = 0; a < nce_of[b]; ++a)
+                                if (colour[a] == k) color_ce.push_back(ce0 + a);
+                            color_off.push_back(static_cast<int>(color_ce.size()));
+                        }
+                    }
                 }
                 // per-block ranges (blocks in order; only condensed blocks are non-empty)
                 for (size_t b = 0; b + 1 < z_off.size(); ++b) {
@@ -1313,6 +1506,24 @@ kyn_system kyn_system_create(const kyn_model* models, size_t num_models, const k
                 S.block_ce_begin = Ar.up(ce_begin);
                 S.block_Z_off = Ar.up(Z_off);
                 S.Zbuf = Ar.zeros<double>(static_cast<size_t>(Z_tot));
+                tv_row_off.push_back(static_cast<int>(tv_rows.size()));
+                tv_child_off.push_back(static_cast<int>(tv_children.size()));
+                S.block_color = Ar.up(block_color);
+                S.cond_color_off = Ar.up(color_off);
+                S.cond_color_ce = Ar.up(color_ce);
+                S.tree_block = Ar.up(tree_block);
+                S.tree_vsize = Ar.up(tv_size);
+                S.tree_vrow_off = Ar.up(tv_row_off);
+                S.tree_vrows = Ar.up(tv_rows);
+                S.tree_vparent = Ar.up(tv_parent);
+                S.tree_vchild_off = Ar.up(tv_child_off);
+                S.tree_vchildren = Ar.up(tv_children);
+                S.tree_elim = Ar.up(t_elim);
+                S.tree_elim_lvl = Ar.up(t_elim_lvl);
+                S.tree_back = Ar.up(t_back);
+                S.tree_back_lvl = Ar.up(t_back_lvl);
+                sys->tree_nvert.assign(B.block_n.size(), 0);
+                for (size_t bb = 0; bb < B.block_n.size(); ++bb) sys->tree_nvert[bb] = tree_block[6 * bb + 1];
                 S.dof_cond_elem = Ar.up(dof_ce);
                 S.dof_cond_local = Ar.up(dof_cl);
                 S.dof_zidx = Ar.up(dof_z);

@@ -2534,7 +2534,7 @@ __global__ void __launch_bounds__(kThreads) k_cond_factor_f32(DevSys S, const do
     cond_factor_body<float>(S, rin);
 }
 
-// Interface system of block b: Z = A[z, z] - sum_e S_e (elements in order), r_z - sum_e t_e
+// Interface system of block b: Z = A[z, z] - sum_e S_e, r_z - sum_e t_e
 __global__ void __launch_bounds__(kThreads) k_cond_interface(DevSys S, int b, const double* rin) {
     const int t = threadIdx.x;
     if (!S.model_active[S.block_model[b]]) return;
@@ -2545,7 +2545,26 @@ __global__ void __launch_bounds__(kThreads) k_cond_interface(DevSys S, int b, co
     const double* V = S.crs_val + S.crs_nz_off[b];
     double* Z = S.Zbuf + S.block_Z_off[b];
     double* rz = S.zrhs + z0;
-    for (int k = t; k < nz * nz; k += blockDim.x) Z[k] = 0.;
+    const int nv = S.tree_block[6 * b + 1];
+    if (nv > 0) {
+        // tree-structured interface: every entry the assembly below and the tree solve touch lies
+        // in a vertex's diagonal block or a parent-child block; only those are cleared
+        const int v0 = S.tree_block[6 * b];
+        for (int q = t; q < nv * kTreeMaxBlock * kTreeMaxBlock; q += blockDim.x) {
+            const int v = v0 + q / (kTreeMaxBlock * kTreeMaxBlock), e = q % (kTreeMaxBlock * kTreeMaxBlock);
+            const int i = e / kTreeMaxBlock, j = e % kTreeMaxBlock;
+            const int s = S.tree_vsize[v], p = S.tree_vparent[v];
+            const int* rv = S.tree_vrows + S.tree_vrow_off[v];
+            if (i < s && j < s) Z[(size_t)rv[j] * nz + rv[i]] = 0.;
+            if (p >= 0 && i < s && j < S.tree_vsize[p]) {
+                const int* rp = S.tree_vrows + S.tree_vrow_off[p];
+                Z[(size_t)rp[j] * nz + rv[i]] = 0.;
+                Z[(size_t)rv[i] * nz + rp[j]] = 0.;
+            }
+        }
+    } else {
+        for (int k = t; k < nz * nz; k += blockDim.x) Z[k] = 0.;
+    }
     __syncthreads();
     for (int a = t; a < nz; a += blockDim.x) {  // interface row a: its CRS row's interface columns
         const int row = zr[a];
@@ -2556,27 +2575,36 @@ __global__ void __launch_bounds__(kThreads) k_cond_interface(DevSys S, int b, co
         rz[a] = rin[vb + row];
     }
     __syncthreads();
-    for (int ce = S.block_ce_begin[b]; ce < S.block_ce_begin[b + 1]; ++ce) {
-        const int l0 = S.cond_lam_off[ce], me = S.cond_lam_off[ce + 1] - l0;
-        const int* lz = S.cond_lam_z + l0;
-        const double* Sm = S.Sbuf + S.cond_S_off[ce];
-        for (int q = t; q < me * me; q += blockDim.x) Z[(size_t)lz[q / me] * nz + lz[q % me]] -= Sm[q];
-        if (t < me) rz[lz[t]] -= Sm[(size_t)me * me + t];
+    // the elements' S_e by colour: elements of one colour share no constraint row, so their
+    // updates touch disjoint entries; every entry receives its updates in colour order
+    constexpr int kStride = kCondMaxLam * (kCondMaxLam + 1);
+    for (int col = 0; col < S.block_color[2 * b + 1]; ++col) {
+        const int c0 = S.cond_color_off[S.block_color[2 * b] + col], c1 = S.cond_color_off[S.block_color[2 * b] + col + 1];
+        for (int q = t; q < (c1 - c0) * kStride; q += blockDim.x) {
+            const int ce = S.cond_color_ce[c0 + q / kStride], e = q % kStride;
+            const int l0 = S.cond_lam_off[ce], me = S.cond_lam_off[ce + 1] - l0;
+            const int* lz = S.cond_lam_z + l0;
+            const double* Sm = S.Sbuf + S.cond_S_off[ce];
+            if (e < me * me) Z[(size_t)lz[e / me] * nz + lz[e % me]] -= Sm[e];
+            else if (e < me * me + me) rz[lz[e - me * me]] -= Sm[(size_t)me * me + (e - me * me)];
+        }
         __syncthreads();
     }
 }
 
-// r = b - A x over one large block's rows (CRS, thread per row, columns ascending)
+// r = b - A x over one large block's rows (CRS): one wave per row, lanes over the row's
+// entries, a wave reduction (fixed shape, so the sum is deterministic)
 __global__ void k_crs_residual(DevSys S, int b, const double* x, double* r) {
-    const int row = blockIdx.x * blockDim.x + threadIdx.x;
+    const int row = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6), lane = threadIdx.x & 63;
     if (row >= S.block_n[b] || !S.model_active[S.block_model[b]]) return;
     const int vb = S.block_vec_off[b];
     const int* rp = S.crs_rp + S.crs_rp_off[b];
     const int* ci = S.crs_col + S.crs_nz_off[b];
     const double* V = S.crs_val + S.crs_nz_off[b];
     double acc = 0.;
-    for (int p = rp[row]; p < rp[row + 1]; ++p) acc += V[p] * x[vb + ci[p]];
-    r[vb + row] = S.rhs[vb + row] - acc;
+    for (int p = rp[row] + lane; p < rp[row + 1]; p += 64) acc = fma(V[p], x[vb + ci[p]], acc);
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+    if (lane == 0) r[vb + row] = S.rhs[vb + row] - acc;
 }
 
 // A block's CRS values into its dense column-major matrix (zeroed beforehand): the library-LU
@@ -2590,6 +2618,139 @@ __global__ void k_crs_to_dense(DevSys S, int b) {
     const double* V = S.crs_val + S.crs_nz_off[b];
     double* A = S.Abig + S.block_big_off[b];
     for (int p = rp[row]; p < rp[row + 1]; ++p) A[(size_t)ci[p] * n + row] = V[p];
+}
+
+// Tree-structured interface system of a condensed block (see the host's tree schedule): block
+// elimination from the leaves, one thread per vertex of a level (vertices of one height are
+// independent), then substitution from the roots.  Per vertex v with s <= kTreeMaxBlock rows:
+//   S_v = Z_vv - sum_children Z_vc X_c,  [X_v | zhat_v] = S_v^-1 [Z_vp | r_v - sum_children Z_vc zhat_c]
+// (LU with partial pivoting of S_v: largest |a|, lowest row on ties, multipliers and the back
+// substitution scaled by the pivot's reciprocal; rows past s padded with the identity), then
+// z_v = zhat_v - X_v z_p.  Z is the dense column-major interface matrix of
+// k_cond_interface, zrhs its right-hand side, overwritten by z.  X, zhat and z live in LDS.
+constexpr int kTreeThreads = 256;
+
+__global__ void __launch_bounds__(kTreeThreads) k_tree_interface(DevSys S, int b) {
+    extern __shared__ __align__(16) char sm[];
+    if (!S.model_active[S.block_model[b]]) return;
+    constexpr int K = kTreeMaxBlock;
+    const int* tb = S.tree_block + 6 * b;
+    const int vb = tb[0], nv = tb[1];
+    const int z0 = S.block_z_off[b], nz = S.block_z_off[b + 1] - z0;
+    const double* Z = S.Zbuf + S.block_Z_off[b];
+    double* r = S.zrhs + z0;
+    double* Xs = reinterpret_cast<double*>(sm);  // [nv][K][K]: X_v[i][j]
+    double* xh = Xs + K * K * nv;                // [nv][K]
+    double* zs = xh + K * nv;                    // [nv][K]
+    // ---- elimination, leaves first
+    for (int l = 0; l < tb[3]; ++l) {
+        const int lo = S.tree_elim_lvl[tb[2] + l], hi = S.tree_elim_lvl[tb[2] + l + 1];
+        for (int q = lo + (int)threadIdx.x; q < hi; q += blockDim.x) {
+            const int v = S.tree_elim[q], lv = v - vb;
+            const int s = S.tree_vsize[v];
+            const int* rv = S.tree_vrows + S.tree_vrow_off[v];
+            const int p = S.tree_vparent[v];
+            const int sp = p >= 0 ? S.tree_vsize[p] : 0;
+            const int* rp = p >= 0 ? S.tree_vrows + S.tree_vrow_off[p] : rv;
+            double M[K][K], R[K][K + 1];
+#pragma unroll
+            for (int i = 0; i < K; ++i) {
+#pragma unroll
+                for (int j = 0; j < K; ++j) {
+                    M[i][j] = (i < s && j < s) ? Z[(size_t)rv[j] * nz + rv[i]] : (i == j ? 1. : 0.);
+                    R[i][j] = (i < s && j < sp) ? Z[(size_t)rp[j] * nz + rv[i]] : 0.;
+                }
+                R[i][K] = i < s ? r[rv[i]] : 0.;
+            }
+            for (int cq = S.tree_vchild_off[v]; cq < S.tree_vchild_off[v + 1]; ++cq) {
+                const int c = S.tree_vchildren[cq], lc = c - vb, sc = S.tree_vsize[c];
+                const int* rc = S.tree_vrows + S.tree_vrow_off[c];
+                for (int k = 0; k < sc; ++k) {
+                    const double* Xk = Xs + (size_t)lc * K * K + k * K;  // row k of X_c (columns: v's rows)
+                    const double hk = xh[lc * K + k];
+#pragma unroll
+                    for (int i = 0; i < K; ++i) {
+                        const double zik = i < s ? Z[(size_t)rc[k] * nz + rv[i]] : 0.;
+#pragma unroll
+                        for (int j = 0; j < K; ++j) M[i][j] = fma(-zik, Xk[j], M[i][j]);
+                        R[i][K] = fma(-zik, hk, R[i][K]);
+                    }
+                }
+            }
+            // LU with partial pivoting, applied to R
+#pragma unroll
+            for (int j = 0; j < K; ++j) {
+                int pr = j;
+                double best = fabs(M[j][j]);
+#pragma unroll
+                for (int i = j + 1; i < K; ++i)
+                    if (fabs(M[i][j]) > best) {
+                        best = fabs(M[i][j]);
+                        pr = i;
+                    }
+#pragma unroll
+                for (int i = j + 1; i < K; ++i)
+                    if (i == pr) {
+#pragma unroll
+                        for (int c = 0; c < K; ++c) {
+                            const double t = M[j][c];
+                            M[j][c] = M[i][c];
+                            M[i][c] = t;
+                        }
+#pragma unroll
+                        for (int c = 0; c <= K; ++c) {
+                            const double t = R[j][c];
+                            R[j][c] = R[i][c];
+                            R[i][c] = t;
+                        }
+                    }
+                const double inv = 1. / M[j][j];  // one division per pivot; multipliers and the
+                M[j][j] = inv;                     // back substitution scale by it
+#pragma unroll
+                for (int i = j + 1; i < K; ++i) {
+                    const double lij = M[i][j] * inv;
+#pragma unroll
+                    for (int c = j + 1; c < K; ++c) M[i][c] = fma(-lij, M[j][c], M[i][c]);
+#pragma unroll
+                    for (int c = 0; c <= K; ++c) R[i][c] = fma(-lij, R[j][c], R[i][c]);
+                }
+            }
+#pragma unroll
+            for (int j = K - 1; j >= 0; --j) {
+#pragma unroll
+                for (int c = 0; c <= K; ++c) R[j][c] = R[j][c] * M[j][j];  // M[j][j] holds 1 / u_jj
+#pragma unroll
+                for (int i = 0; i < j; ++i)
+#pragma unroll
+                    for (int c = 0; c <= K; ++c) R[i][c] = fma(-M[i][j], R[j][c], R[i][c]);
+            }
+#pragma unroll
+            for (int i = 0; i < K; ++i) {
+#pragma unroll
+                for (int j = 0; j < K; ++j) Xs[(size_t)lv * K * K + i * K + j] = R[i][j];
+                xh[lv * K + i] = R[i][K];
+            }
+        }
+        __syncthreads();
+    }
+    // ---- substitution, roots first
+    for (int l = 0; l < tb[5]; ++l) {
+        const int lo = S.tree_back_lvl[tb[4] + l], hi = S.tree_back_lvl[tb[4] + l + 1];
+        for (int q = lo + (int)threadIdx.x; q < hi; q += blockDim.x) {
+            const int v = S.tree_back[q], lv = v - vb;
+            const int s = S.tree_vsize[v];
+            const int* rv = S.tree_vrows + S.tree_vrow_off[v];
+            const int p = S.tree_vparent[v];
+            const int sp = p >= 0 ? S.tree_vsize[p] : 0;
+            for (int i = 0; i < s; ++i) {
+                double zi = xh[lv * K + i];
+                for (int j = 0; j < sp; ++j) zi = fma(-Xs[(size_t)lv * K * K + i * K + j], zs[(p - vb) * K + j], zi);
+                zs[lv * K + i] = zi;
+                r[rv[i]] = zi;
+            }
+        }
+        __syncthreads();
+    }
 }
 
 // Element dofs from the interface solution: x_e = Y_b - Y_B lambda_e; interface rows copied
@@ -2717,6 +2878,8 @@ hipError_t set_lds_limits(size_t bytes) {
     if (e != hipSuccess) return e;
     e = hipFuncSetAttribute((const void*)k_plain_element_f32, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
     if (e != hipSuccess) return e;
+    e = hipFuncSetAttribute((const void*)k_tree_interface, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (e != hipSuccess) return e;
     e = hipFuncSetAttribute((const void*)k_cond_factor_f64, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
     if (e != hipSuccess) return e;
     e = hipFuncSetAttribute((const void*)k_cond_factor_f32, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
@@ -2821,8 +2984,15 @@ void launch_cond_scatter_z(const DevSys& S, int b, int nz, double* xout, hipStre
 void launch_block_axpy_copy(const DevSys& S, int b, int n, const double* d, double* x, double* r, hipStream_t st) {
     hipLaunchKernelGGL(k_block_axpy_copy, dim3((n + 255) / 256), dim3(256), 0, st, S, b, d, x, r);
 }
+void launch_tree_interface(const DevSys& S, int b, hipStream_t st) {
+    // LDS: X (K x K), zhat and z (K each) per vertex of the block's tree; the vertex count is
+    // read on the host from the same schedule (at most kTreeMaxVerts)
+    hipLaunchKernelGGL(k_tree_interface, dim3(1), dim3(kTreeThreads),
+                       (size_t)kTreeMaxVerts * (kTreeMaxBlock * kTreeMaxBlock + 2 * kTreeMaxBlock) * sizeof(double), st,
+                       S, b);
+}
 void launch_crs_residual(const DevSys& S, int b, int n, const double* x, double* r, hipStream_t st) {
-    hipLaunchKernelGGL(k_crs_residual, dim3((n + 255) / 256), dim3(256), 0, st, S, b, x, r);
+    hipLaunchKernelGGL(k_crs_residual, dim3((n + 3) / 4), dim3(256), 0, st, S, b, x, r);  // 4 rows per workgroup
 }
 void launch_crs_to_dense(const DevSys& S, int b, int n, hipStream_t st) {
     hipLaunchKernelGGL(k_crs_to_dense, dim3((n + 255) / 256), dim3(256), 0, st, S, b);

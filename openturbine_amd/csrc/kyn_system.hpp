@@ -19,6 +19,8 @@ constexpr int kMaxElemQps = 64;
 constexpr int kThreads = 256;      // workgroup size of every per-element / per-block kernel
 constexpr int kMaxBlockDofs = 143; // register LU tile holds n + 1 <= 144 columns ([A | b])
 constexpr int kCondMaxLam = 16;    // condensed solve: constraint rows touching one element
+constexpr int kTreeMaxBlock = 6;   // tree interface solve: rows of one vertex (constraint / free node)
+constexpr int kTreeMaxVerts = 256; // tree interface solve: vertices of one block's interface
 // scratch of the no-allocation element-terms entry point: residual [N][6] + matrix [N][N][36]
 constexpr int kElemTermsScratch = kMaxElemNodes * 6 + kMaxElemNodes * kMaxElemNodes * 36;
 constexpr int kMaxScratchDevices = 64;
@@ -182,6 +184,15 @@ struct DevSys {
     const int* dof_cond_elem;
     const int* dof_cond_local;
     const int* dof_zidx;
+    // tree-structured interface systems (block elimination, no fill): per block [6] = first
+    // vertex, vertices (0: dense solve), first elimination level, elimination levels, first
+    // substitution level, substitution levels; per vertex its rows (interface indices), parent
+    // (-1: root) and children; vertex ids by level with level offsets
+    const int* block_color;           // per block [2]: first colour (into cond_color_off), colours
+    const int *cond_color_off, *cond_color_ce;  // condensed elements grouped by colour (no shared constraint row)
+    const int* tree_block;
+    const int *tree_vsize, *tree_vrow_off, *tree_vrows, *tree_vparent, *tree_vchild_off, *tree_vchildren;
+    const int *tree_elim, *tree_elim_lvl, *tree_back, *tree_back_lvl;
 
     // ---- solver
     void* A;              // blocks' dense KKT matrices (double, or float for FP32 tangent)
