@@ -43,7 +43,8 @@ void launch_crs_to_dense(const DevSys& S, int b, int n, hipStream_t st);
 void launch_tree_interface(const DevSys& S, int b, hipStream_t st);
 size_t cond_lds(int ne, int me);
 void launch_cond_factor(const DevSys& S, size_t lds, bool fp32, const double* rin, hipStream_t st);
-void launch_cond_interface(const DevSys& S, int b, const double* rin, hipStream_t st);
+void launch_cond_interface(const DevSys& S, int b, int nz, const int* colour_sizes, int ncolour, const double* rin,
+                           hipStream_t st);
 void launch_cond_backsub(const DevSys& S, double* xout, hipStream_t st);
 void launch_cond_scatter_z(const DevSys& S, int b, int nz, double* xout, hipStream_t st);
 void launch_block_axpy_copy(const DevSys& S, int b, int n, const double* d, double* x, double* r, hipStream_t st);
@@ -249,6 +250,7 @@ struct kyn_system_s {
     std::vector<int> large_cond, large_nz;  // condensed solve (else dense), interface size
     std::vector<int64_t> crs_rp_off, crs_nz_off;  // per block (large blocks only)
     std::vector<int> tree_nvert;                  // per block: vertices of its interface tree (0: dense solve)
+    std::vector<std::vector<int>> colour_sizes;   // per block: condensed elements of each colour
     std::vector<int64_t> large_Z_off;
     size_t cond_lds_bytes{0};
     rocblas_handle blas{};
@@ -873,7 +875,8 @@ void solve_large_blocks(kyn_system_s* s) {
             for (size_t k = 0; k < s->large_blocks.size(); ++k) {
                 const int b = s->large_blocks[k], nz = s->large_nz[k];
                 if (!s->large_cond[k] || !s->model_active_host[s->block_model[b]]) continue;
-                kyn::launch_cond_interface(S, b, rin, st);
+                kyn::launch_cond_interface(S, b, nz, s->colour_sizes[b].data(), static_cast<int>(s->colour_sizes[b].size()),
+                                           rin, st);
                 if (s->tree_nvert[b] > 0) {  // tree-structured interface: block elimination in one launch
                     kyn::launch_tree_interface(S, b, st);
                     hip_check(hipGetLastError(), "k_tree_interface launch");
@@ -1477,9 +1480,11 @@ kyn_system kyn_system_create(const kyn_model* models, size_t num_models, const k
                         }
                         block_color[2 * b] = static_cast<int>(color_off.size()) - 1;
                         block_color[2 * b + 1] = ncol;
+                        sys->colour_sizes.resize(B.block_n.size());
                         for (int k = 0; k < ncol; ++k) {
                             for (int a = 0; a < nce_of[b]; ++a)
                                 if (colour[a] == k) color_ce.push_back(ce0 + a);
+                            sys->colour_sizes[b].push_back(static_cast<int>(color_ce.size()) - color_off.back());
                             color_off.push_back(static_cast<int>(color_ce.size()));
                         }
                     }

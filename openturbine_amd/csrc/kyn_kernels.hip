@@ -2534,24 +2534,26 @@ __global__ void __launch_bounds__(kThreads) k_cond_factor_f32(DevSys S, const do
     cond_factor_body<float>(S, rin);
 }
 
-// Interface system of block b: Z = A[z, z] - sum_e S_e, r_z - sum_e t_e
-__global__ void __launch_bounds__(kThreads) k_cond_interface(DevSys S, int b, const double* rin) {
-    const int t = threadIdx.x;
+// Interface system of block b: Z = A[z, z] - sum_e S_e, r_z - sum_e t_e, in three launches:
+//  k_cond_interface_clear  (grid-stride) the entries the assembly and the solve touch: for a
+//                          tree-structured interface its vertices' diagonal and parent-child
+//                          blocks, else the whole dense matrix;
+//  k_cond_interface_fill   one wave per interface row: the row's CRS entries in interface columns,
+//                          and its rhs entry (rows are disjoint, so no ordering);
+//  k_cond_interface        the elements' S_e and t_e, one launch per colour (elements of one
+//                          colour share no constraint row; every entry receives its updates in
+//                          colour order), thread per entry.
+__global__ void k_cond_interface_clear(DevSys S, int b) {
     if (!S.model_active[S.block_model[b]]) return;
-    const int z0 = S.block_z_off[b], nz = S.block_z_off[b + 1] - z0, vb = S.block_vec_off[b];
-    const int* zr = S.cond_z + z0;
-    const int* rp = S.crs_rp + S.crs_rp_off[b];
-    const int* ci = S.crs_col + S.crs_nz_off[b];
-    const double* V = S.crs_val + S.crs_nz_off[b];
+    const int z0 = S.block_z_off[b], nz = S.block_z_off[b + 1] - z0;
     double* Z = S.Zbuf + S.block_Z_off[b];
-    double* rz = S.zrhs + z0;
     const int nv = S.tree_block[6 * b + 1];
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t q0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (nv > 0) {
-        // tree-structured interface: every entry the assembly below and the tree solve touch lies
-        // in a vertex's diagonal block or a parent-child block; only those are cleared
         const int v0 = S.tree_block[6 * b];
-        for (int q = t; q < nv * kTreeMaxBlock * kTreeMaxBlock; q += blockDim.x) {
-            const int v = v0 + q / (kTreeMaxBlock * kTreeMaxBlock), e = q % (kTreeMaxBlock * kTreeMaxBlock);
+        for (int64_t q = q0; q < (int64_t)nv * kTreeMaxBlock * kTreeMaxBlock; q += stride) {
+            const int v = v0 + (int)(q / (kTreeMaxBlock * kTreeMaxBlock)), e = (int)(q % (kTreeMaxBlock * kTreeMaxBlock));
             const int i = e / kTreeMaxBlock, j = e % kTreeMaxBlock;
             const int s = S.tree_vsize[v], p = S.tree_vparent[v];
             const int* rv = S.tree_vrows + S.tree_vrow_off[v];
@@ -2563,33 +2565,42 @@ __global__ void __launch_bounds__(kThreads) k_cond_interface(DevSys S, int b, co
             }
         }
     } else {
-        for (int k = t; k < nz * nz; k += blockDim.x) Z[k] = 0.;
+        for (int64_t q = q0; q < (int64_t)nz * nz; q += stride) Z[q] = 0.;
     }
-    __syncthreads();
-    for (int a = t; a < nz; a += blockDim.x) {  // interface row a: its CRS row's interface columns
-        const int row = zr[a];
-        for (int p = rp[row]; p < rp[row + 1]; ++p) {
-            const int zc = S.dof_zidx[vb + ci[p]];
-            if (zc >= 0) Z[(size_t)zc * nz + a] = V[p];
-        }
-        rz[a] = rin[vb + row];
+}
+
+__global__ void k_cond_interface_fill(DevSys S, int b, const double* rin) {
+    if (!S.model_active[S.block_model[b]]) return;
+    const int a = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6), lane = threadIdx.x & 63;
+    const int z0 = S.block_z_off[b], nz = S.block_z_off[b + 1] - z0, vb = S.block_vec_off[b];
+    if (a >= nz) return;
+    const int row = S.cond_z[z0 + a];
+    const int* rp = S.crs_rp + S.crs_rp_off[b];
+    const int* ci = S.crs_col + S.crs_nz_off[b];
+    const double* V = S.crs_val + S.crs_nz_off[b];
+    double* Z = S.Zbuf + S.block_Z_off[b];
+    for (int p = rp[row] + lane; p < rp[row + 1]; p += 64) {
+        const int zc = S.dof_zidx[vb + ci[p]];
+        if (zc >= 0) Z[(size_t)zc * nz + a] = V[p];
     }
-    __syncthreads();
-    // the elements' S_e by colour: elements of one colour share no constraint row, so their
-    // updates touch disjoint entries; every entry receives its updates in colour order
+    if (lane == 0) S.zrhs[z0 + a] = rin[vb + row];
+}
+
+__global__ void k_cond_interface(DevSys S, int b, int col) {
+    if (!S.model_active[S.block_model[b]]) return;
+    const int z0 = S.block_z_off[b], nz = S.block_z_off[b + 1] - z0;
+    double* Z = S.Zbuf + S.block_Z_off[b];
+    double* rz = S.zrhs + z0;
     constexpr int kStride = kCondMaxLam * (kCondMaxLam + 1);
-    for (int col = 0; col < S.block_color[2 * b + 1]; ++col) {
-        const int c0 = S.cond_color_off[S.block_color[2 * b] + col], c1 = S.cond_color_off[S.block_color[2 * b] + col + 1];
-        for (int q = t; q < (c1 - c0) * kStride; q += blockDim.x) {
-            const int ce = S.cond_color_ce[c0 + q / kStride], e = q % kStride;
-            const int l0 = S.cond_lam_off[ce], me = S.cond_lam_off[ce + 1] - l0;
-            const int* lz = S.cond_lam_z + l0;
-            const double* Sm = S.Sbuf + S.cond_S_off[ce];
-            if (e < me * me) Z[(size_t)lz[e / me] * nz + lz[e % me]] -= Sm[e];
-            else if (e < me * me + me) rz[lz[e - me * me]] -= Sm[(size_t)me * me + (e - me * me)];
-        }
-        __syncthreads();
-    }
+    const int c0 = S.cond_color_off[S.block_color[2 * b] + col], c1 = S.cond_color_off[S.block_color[2 * b] + col + 1];
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;  // entry of one element of this colour
+    if (q >= (c1 - c0) * kStride) return;
+    const int ce = S.cond_color_ce[c0 + q / kStride], e = q % kStride;
+    const int l0 = S.cond_lam_off[ce], me = S.cond_lam_off[ce + 1] - l0;
+    const int* lz = S.cond_lam_z + l0;
+    const double* Sm = S.Sbuf + S.cond_S_off[ce];
+    if (e < me * me) Z[(size_t)lz[e / me] * nz + lz[e % me]] -= Sm[e];
+    else if (e < me * me + me) rz[lz[e - me * me]] -= Sm[(size_t)me * me + (e - me * me)];
 }
 
 // r = b - A x over one large block's rows (CRS): one wave per row, lanes over the row's
@@ -2627,50 +2638,70 @@ __global__ void k_crs_to_dense(DevSys S, int b) {
 // (LU with partial pivoting of S_v: largest |a|, lowest row on ties, multipliers and the back
 // substitution scaled by the pivot's reciprocal; rows past s padded with the identity), then
 // z_v = zhat_v - X_v z_p.  Z is the dense column-major interface matrix of
-// k_cond_interface, zrhs its right-hand side, overwritten by z.  X, zhat and z live in LDS.
+// k_cond_interface, zrhs its right-hand side, overwritten by z.  Every vertex's three blocks
+// (Z_vv, Z_v,parent, Z_parent,v) and rhs are first copied to LDS in one parallel pass; X, zhat
+// and z live in LDS too.
 constexpr int kTreeThreads = 256;
 
 __global__ void __launch_bounds__(kTreeThreads) k_tree_interface(DevSys S, int b) {
     extern __shared__ __align__(16) char sm[];
     if (!S.model_active[S.block_model[b]]) return;
-    constexpr int K = kTreeMaxBlock;
+    constexpr int K = kTreeMaxBlock, KK = kTreeMaxBlock * kTreeMaxBlock;
     const int* tb = S.tree_block + 6 * b;
     const int vb = tb[0], nv = tb[1];
     const int z0 = S.block_z_off[b], nz = S.block_z_off[b + 1] - z0;
     const double* Z = S.Zbuf + S.block_Z_off[b];
     double* r = S.zrhs + z0;
-    double* Xs = reinterpret_cast<double*>(sm);  // [nv][K][K]: X_v[i][j]
-    double* xh = Xs + K * K * nv;                // [nv][K]
-    double* zs = xh + K * nv;                    // [nv][K]
+    double* Ds = reinterpret_cast<double*>(sm);  // [nv][K][K] Z_vv
+    double* Up = Ds + KK * nv;                   // [nv][K][K] Z_v,parent
+    double* Lp = Up + KK * nv;                   // [nv][K][K] Z_parent,v
+    double* Xs = Lp + KK * nv;                   // [nv][K][K] X_v
+    double* xh = Xs + KK * nv;                   // [nv][K]: the rhs, then zhat_v
+    double* zs = xh + K * nv;                    // [nv][K]: z_v
+    // ---- every vertex's blocks and rhs from Z into LDS, all at once (independent loads)
+    for (int q = threadIdx.x; q < nv * (3 * KK + K); q += blockDim.x) {
+        const int lv = q / (3 * KK + K), e = q % (3 * KK + K), v = vb + lv;
+        const int s = S.tree_vsize[v], p = S.tree_vparent[v];
+        const int sp = p >= 0 ? S.tree_vsize[p] : 0;
+        const int* rv = S.tree_vrows + S.tree_vrow_off[v];
+        const int* rp = p >= 0 ? S.tree_vrows + S.tree_vrow_off[p] : rv;
+        if (e < 3 * KK) {
+            const int which = e / KK, i = (e % KK) / K, j = e % K;
+            double val = 0.;
+            if (which == 0 && i < s && j < s) val = Z[(size_t)rv[j] * nz + rv[i]];
+            if (which == 1 && i < s && j < sp) val = Z[(size_t)rp[j] * nz + rv[i]];
+            if (which == 2 && i < sp && j < s) val = Z[(size_t)rv[j] * nz + rp[i]];
+            (which == 0 ? Ds : which == 1 ? Up : Lp)[lv * KK + i * K + j] = val;
+        } else {
+            const int i = e - 3 * KK;
+            xh[lv * K + i] = i < s ? r[rv[i]] : 0.;
+        }
+    }
+    __syncthreads();
     // ---- elimination, leaves first
     for (int l = 0; l < tb[3]; ++l) {
         const int lo = S.tree_elim_lvl[tb[2] + l], hi = S.tree_elim_lvl[tb[2] + l + 1];
         for (int q = lo + (int)threadIdx.x; q < hi; q += blockDim.x) {
             const int v = S.tree_elim[q], lv = v - vb;
             const int s = S.tree_vsize[v];
-            const int* rv = S.tree_vrows + S.tree_vrow_off[v];
-            const int p = S.tree_vparent[v];
-            const int sp = p >= 0 ? S.tree_vsize[p] : 0;
-            const int* rp = p >= 0 ? S.tree_vrows + S.tree_vrow_off[p] : rv;
             double M[K][K], R[K][K + 1];
 #pragma unroll
             for (int i = 0; i < K; ++i) {
 #pragma unroll
                 for (int j = 0; j < K; ++j) {
-                    M[i][j] = (i < s && j < s) ? Z[(size_t)rv[j] * nz + rv[i]] : (i == j ? 1. : 0.);
-                    R[i][j] = (i < s && j < sp) ? Z[(size_t)rp[j] * nz + rv[i]] : 0.;
+                    M[i][j] = (i < s && j < s) ? Ds[lv * KK + i * K + j] : (i == j ? 1. : 0.);
+                    R[i][j] = Up[lv * KK + i * K + j];
                 }
-                R[i][K] = i < s ? r[rv[i]] : 0.;
+                R[i][K] = xh[lv * K + i];
             }
             for (int cq = S.tree_vchild_off[v]; cq < S.tree_vchild_off[v + 1]; ++cq) {
-                const int c = S.tree_vchildren[cq], lc = c - vb, sc = S.tree_vsize[c];
-                const int* rc = S.tree_vrows + S.tree_vrow_off[c];
+                const int lc = S.tree_vchildren[cq] - vb, sc = S.tree_vsize[lc + vb];
                 for (int k = 0; k < sc; ++k) {
-                    const double* Xk = Xs + (size_t)lc * K * K + k * K;  // row k of X_c (columns: v's rows)
+                    const double* Xk = Xs + lc * KK + k * K;  // row k of X_c (columns: v's rows)
                     const double hk = xh[lc * K + k];
 #pragma unroll
                     for (int i = 0; i < K; ++i) {
-                        const double zik = i < s ? Z[(size_t)rc[k] * nz + rv[i]] : 0.;
+                        const double zik = Lp[lc * KK + i * K + k];  // Z_v,c[i][k]: zero past v's rows
 #pragma unroll
                         for (int j = 0; j < K; ++j) M[i][j] = fma(-zik, Xk[j], M[i][j]);
                         R[i][K] = fma(-zik, hk, R[i][K]);
@@ -2727,7 +2758,7 @@ __global__ void __launch_bounds__(kTreeThreads) k_tree_interface(DevSys S, int b
 #pragma unroll
             for (int i = 0; i < K; ++i) {
 #pragma unroll
-                for (int j = 0; j < K; ++j) Xs[(size_t)lv * K * K + i * K + j] = R[i][j];
+                for (int j = 0; j < K; ++j) Xs[lv * KK + i * K + j] = R[i][j];
                 xh[lv * K + i] = R[i][K];
             }
         }
@@ -2744,7 +2775,7 @@ __global__ void __launch_bounds__(kTreeThreads) k_tree_interface(DevSys S, int b
             const int sp = p >= 0 ? S.tree_vsize[p] : 0;
             for (int i = 0; i < s; ++i) {
                 double zi = xh[lv * K + i];
-                for (int j = 0; j < sp; ++j) zi = fma(-Xs[(size_t)lv * K * K + i * K + j], zs[(p - vb) * K + j], zi);
+                for (int j = 0; j < sp; ++j) zi = fma(-Xs[lv * KK + i * K + j], zs[(p - vb) * K + j], zi);
                 zs[lv * K + i] = zi;
                 r[rv[i]] = zi;
             }
@@ -2972,8 +3003,13 @@ void launch_cond_factor(const DevSys& S, size_t lds, bool fp32, const double* ri
     else
         hipLaunchKernelGGL(k_cond_factor_f64, dim3(S.num_cond_elems), dim3(kThreads), lds, st, S, rin);
 }
-void launch_cond_interface(const DevSys& S, int b, const double* rin, hipStream_t st) {
-    hipLaunchKernelGGL(k_cond_interface, dim3(1), dim3(kThreads), 0, st, S, b, rin);
+void launch_cond_interface(const DevSys& S, int b, int nz, const int* colour_sizes, int ncolour, const double* rin,
+                           hipStream_t st) {
+    hipLaunchKernelGGL(k_cond_interface_clear, dim3(64), dim3(256), 0, st, S, b);
+    hipLaunchKernelGGL(k_cond_interface_fill, dim3((nz + 3) / 4), dim3(256), 0, st, S, b, rin);  // 4 rows per workgroup
+    constexpr int kStride = kCondMaxLam * (kCondMaxLam + 1);
+    for (int col = 0; col < ncolour; ++col)
+        hipLaunchKernelGGL(k_cond_interface, dim3((colour_sizes[col] * kStride + 255) / 256), dim3(256), 0, st, S, b, col);
 }
 void launch_cond_backsub(const DevSys& S, double* xout, hipStream_t st) {
     hipLaunchKernelGGL(k_cond_backsub, dim3(S.num_cond_elems), dim3(kThreads), 0, st, S, xout);
@@ -2985,10 +3021,10 @@ void launch_block_axpy_copy(const DevSys& S, int b, int n, const double* d, doub
     hipLaunchKernelGGL(k_block_axpy_copy, dim3((n + 255) / 256), dim3(256), 0, st, S, b, d, x, r);
 }
 void launch_tree_interface(const DevSys& S, int b, hipStream_t st) {
-    // LDS: X (K x K), zhat and z (K each) per vertex of the block's tree; the vertex count is
-    // read on the host from the same schedule (at most kTreeMaxVerts)
+    // LDS per vertex of the block's tree: Z_vv, Z_v,parent, Z_parent,v and X (K x K each), the
+    // rhs / zhat and z (K each); the host admits a tree only up to kTreeMaxVerts vertices
     hipLaunchKernelGGL(k_tree_interface, dim3(1), dim3(kTreeThreads),
-                       (size_t)kTreeMaxVerts * (kTreeMaxBlock * kTreeMaxBlock + 2 * kTreeMaxBlock) * sizeof(double), st,
+                       (size_t)kTreeMaxVerts * (4 * kTreeMaxBlock * kTreeMaxBlock + 2 * kTreeMaxBlock) * sizeof(double), st,
                        S, b);
 }
 void launch_crs_residual(const DevSys& S, int b, int n, const double* x, double* r, hipStream_t st) {

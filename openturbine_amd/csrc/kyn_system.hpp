@@ -20,7 +20,7 @@ constexpr int kThreads = 256;      // workgroup size of every per-element / per-
 constexpr int kMaxBlockDofs = 143; // register LU tile holds n + 1 <= 144 columns ([A | b])
 constexpr int kCondMaxLam = 16;    // condensed solve: constraint rows touching one element
 constexpr int kTreeMaxBlock = 6;   // tree interface solve: rows of one vertex (constraint / free node)
-constexpr int kTreeMaxVerts = 256; // tree interface solve: vertices of one block's interface
+constexpr int kTreeMaxVerts = 128; // tree interface solve: vertices of one block's interface (LDS)
 // scratch of the no-allocation element-terms entry point: residual [N][6] + matrix [N][N][36]
 constexpr int kElemTermsScratch = kMaxElemNodes * 6 + kMaxElemNodes * kMaxElemNodes * 36;
 constexpr int kMaxScratchDevices = 64;
