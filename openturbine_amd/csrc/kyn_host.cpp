@@ -1281,8 +1281,9 @@ kyn_system kyn_system_create(const kyn_model* models, size_t num_models, const k
             // allocated below, once the condensed blocks are known
             sys->crs_rp_off.assign(B.block_n.size(), 0);
             sys->crs_nz_off.assign(B.block_n.size(), 0);
+            std::vector<int> rp_all, col_all;  // kept for the condensed elements' gather maps
             {
-                std::vector<int> rp_all, col_all, rp, cols;
+                std::vector<int> rp, cols;
                 for (int b : sys->large_blocks) {
                     block_crs_graph(B, b, rp, cols);
                     sys->crs_rp_off[b] = static_cast<int64_t>(rp_all.size());
@@ -1356,6 +1357,7 @@ kyn_system kyn_system_create(const kyn_model* models, size_t num_models, const k
                     tv_children, t_elim, t_elim_lvl, t_back, t_back_lvl;
                 std::vector<int> block_color(2 * B.block_n.size(), 0), color_off{0}, color_ce;
                 std::vector<int> nz_of(B.block_n.size(), 0), cond_of(B.block_n.size(), 0), nce_of(B.block_n.size(), 0);
+                std::vector<int> gat_off{0}, gat_src, gat_dst, bt_off{0}, bt_src, bt_col;
                 for (int b : sys->large_blocks) {
                     bool ok = true;
                     for (size_t k = 0; k < B.spring_n1.size() && ok; ++k)
@@ -1455,6 +1457,35 @@ kyn_system kyn_system_create(const kyn_model* models, size_t num_models, const k
                         }
                         lam_off.push_back(static_cast<int>(lam.size()));
                         const int ne = 6 * B.elem_N[e], me = static_cast<int>(elam[idx].size());
+                        {   // gather maps: the element's rows into [K_e | B_e^T], and B_e's entries
+                            const int* rpb = rp_all.data() + sys->crs_rp_off[b];
+                            const int* cib = col_all.data() + sys->crs_nz_off[b];
+                            const int vb = B.block_vec_off[b], ce = static_cast<int>(ce_elem.size()) - 1;
+                            const int lda = (ne + me + 1) | 1;
+                            const int* er = rows.data() + rows_off[rows_off.size() - 2];
+                            for (int i = 0; i < ne; ++i)
+                                for (int p = rpb[er[i]]; p < rpb[er[i] + 1]; ++p) {
+                                    const int c = cib[p];
+                                    int dst = -1;
+                                    if (dof_ce[vb + c] == ce) dst = i * lda + dof_cl[vb + c];
+                                    for (int k = 0; k < me && dst < 0; ++k)
+                                        if (elam[idx][k] == c) dst = i * lda + ne + k;
+                                    if (dst >= 0) {
+                                        gat_src.push_back(p);
+                                        gat_dst.push_back(dst);
+                                    }
+                                }
+                            gat_off.push_back(static_cast<int>(gat_src.size()));
+                            for (int k = 0; k < me; ++k) {
+                                const int row = elam[idx][k];
+                                for (int p = rpb[row]; p < rpb[row + 1]; ++p)
+                                    if (dof_ce[vb + cib[p]] == ce) {
+                                        bt_src.push_back(p);
+                                        bt_col.push_back(dof_cl[vb + cib[p]]);
+                                    }
+                                bt_off.push_back(static_cast<int>(bt_src.size()));
+                            }
+                        }
                         Y_off.push_back(Y_tot);
                         Y_tot += static_cast<int64_t>(ne) * (me + 1);
                         S_off.push_back(S_tot);
@@ -1531,6 +1562,12 @@ kyn_system kyn_system_create(const kyn_model* models, size_t num_models, const k
                 for (size_t bb = 0; bb < B.block_n.size(); ++bb) sys->tree_nvert[bb] = tree_block[6 * bb + 1];
                 S.dof_cond_elem = Ar.up(dof_ce);
                 S.dof_cond_local = Ar.up(dof_cl);
+                S.cond_gat_off = Ar.up(gat_off);
+                S.cond_gat_src = Ar.up(gat_src);
+                S.cond_gat_dst = Ar.up(gat_dst);
+                S.cond_bt_off = Ar.up(bt_off);
+                S.cond_bt_src = Ar.up(bt_src);
+                S.cond_bt_col = Ar.up(bt_col);
                 S.dof_zidx = Ar.up(dof_z);
                 S.zrhs = Ar.zeros<double>(static_cast<size_t>(z_off.back()));
                 for (size_t k = 0; k < sys->large_blocks.size(); ++k) {

@@ -2474,13 +2474,10 @@ __device__ __forceinline__ void cond_factor_body(DevSys& S, const double* rin) {
     const int b = S.cond_elem_block[ce];
     if (!S.model_active[S.block_model[b]]) return;
     const int vb = S.block_vec_off[b];
-    const int* rp = S.crs_rp + S.crs_rp_off[b];
-    const int* ci = S.crs_col + S.crs_nz_off[b];
     const double* V = S.crs_val + S.crs_nz_off[b];
     const int r0 = S.cond_rows_off[ce], ne = S.cond_rows_off[ce + 1] - r0;
     const int l0 = S.cond_lam_off[ce], me = S.cond_lam_off[ce + 1] - l0;
     const int* rows = S.cond_rows + r0;
-    const int* lam = S.cond_lam + l0;
     const int nrhs = me + 1;
     const int lda = (ne + nrhs) | 1;
     TT* M = reinterpret_cast<TT*>(sm);
@@ -2495,20 +2492,28 @@ __device__ __forceinline__ void cond_factor_body(DevSys& S, const double* rin) {
     const double* rhs = rin + vb;
     for (int k = t; k < ne * lda; k += blockDim.x) M[k] = TT(0);
     __syncthreads();
-    // thread per element row: its CRS row's entries go to the element's own columns (K_e) or to
-    // the constraint-row columns of the element (B_e^T); the rhs entry last
-    for (int i = t; i < ne; i += blockDim.x) {
-        const int row = rows[i];
-        for (int p = rp[row]; p < rp[row + 1]; ++p) {
-            const int c = ci[p];
-            if (S.dof_cond_elem[vb + c] == ce) {
-                M[i * lda + S.dof_cond_local[vb + c]] = (TT)V[p];
-            } else {
-                for (int k = 0; k < me; ++k)
-                    if (lam[k] == c) M[i * lda + ne + k] = (TT)V[p];
+    // the element rows' CRS entries into [K_e | B_e^T] through the static gather map (thread per
+    // entry, four independent loads in flight per thread); the rhs column last
+    {
+        const int g0 = S.cond_gat_off[ce], g1 = S.cond_gat_off[ce + 1];
+        const int* src = S.cond_gat_src;
+        const int* dst = S.cond_gat_dst;
+        for (int q = g0 + t; q < g1; q += 4 * blockDim.x) {
+            int sq[4], dq[4];
+            double vq[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int qq = q + u * (int)blockDim.x;
+                sq[u] = qq < g1 ? src[qq] : src[q];
+                dq[u] = qq < g1 ? dst[qq] : -1;
             }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) vq[u] = V[sq[u]];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (dq[u] >= 0) M[dq[u]] = (TT)vq[u];
         }
-        M[i * lda + ne + me] = (TT)rhs[row];
+        for (int i = t; i < ne; i += blockDim.x) M[i * lda + ne + me] = (TT)rhs[rows[i]];
     }
     __syncthreads();
     const LuScratch W{ph, lc, sync, S.lu_wait_limit, S.dev_error};
@@ -2520,10 +2525,8 @@ __device__ __forceinline__ void cond_factor_body(DevSys& S, const double* rin) {
     for (int q = t; q < me * nrhs; q += blockDim.x) {  // S_e = B_e Y: constraint row lam[k]'s entries in this element
         const int k = q % me, c = q / me;
         double acc = 0.;
-        for (int p = rp[lam[k]]; p < rp[lam[k] + 1]; ++p) {
-            const int col = ci[p];
-            if (S.dof_cond_elem[vb + col] == ce) acc += V[p] * Y[(size_t)c * ne + S.dof_cond_local[vb + col]];
-        }
+        for (int u = S.cond_bt_off[l0 + k]; u < S.cond_bt_off[l0 + k + 1]; ++u)
+            acc += V[S.cond_bt_src[u]] * Y[(size_t)c * ne + S.cond_bt_col[u]];
         Sm[(size_t)c * me + k] = acc;
     }
 }

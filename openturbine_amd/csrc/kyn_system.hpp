@@ -184,6 +184,17 @@ struct DevSys {
     const int* dof_cond_elem;
     const int* dof_cond_local;
     const int* dof_zidx;
+    // gather maps of the condensed elements (static: the CRS graph is fixed at setup).  Per
+    // element [ce+1] into cond_gat_*: every CRS entry of the element's rows as (block-relative
+    // nonzero index, LDS index i * lda + column of [K_e | B_e^T]); per constraint row of an
+    // element (cond_lam entry) [l+1] into cond_bt_*: that row's entries in the element's columns
+    // as (nonzero index, element-local column)
+    const int* cond_gat_off;
+    const int* cond_gat_src;
+    const int* cond_gat_dst;
+    const int* cond_bt_off;
+    const int* cond_bt_src;
+    const int* cond_bt_col;
     // tree-structured interface systems (block elimination, no fill): per block [6] = first
     // vertex, vertices (0: dense solve), first elimination level, elimination levels, first
     // substitution level, substitution levels; per vertex its rows (interface indices), parent
