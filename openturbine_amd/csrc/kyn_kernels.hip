@@ -908,11 +908,11 @@ __device__ __forceinline__ void lu_panel_ns(TT* A, int n, int lda, int c0, int w
         lg[s] = ok ? li : -1;
         rr[s] = ph_in[ok ? li : c0];  // clamped: loads are unconditional, values selected after
 #pragma unroll
-        for (int jj = 0; jj < kPanel; ++jj) {
-            P[s][jj] = A[rr[s] * lda + c0 + (jj < w ? jj : 0)];
-            pin(P[s][jj]);
-            P[s][jj] = (ok && jj < w) ? P[s][jj] : TT(0);
-        }
+        for (int jj = 0; jj < kPanel; ++jj) P[s][jj] = A[rr[s] * lda + c0 + (jj < w ? jj : 0)];
+#pragma unroll
+        for (int jj = 0; jj < kPanel; ++jj) pin(P[s][jj]);  // after all loads: one wait
+#pragma unroll
+        for (int jj = 0; jj < kPanel; ++jj) P[s][jj] = (ok && jj < w) ? P[s][jj] : TT(0);
     }
     // Columns as compile-time steps (constant register indices without relying on unrolling:
     // the tile stays in registers).  A single wave issues about one instruction per four cycles,
@@ -1019,10 +1019,9 @@ __device__ __forceinline__ void lu_apply_panel(TT* A, int n, int lda, int c0, in
         const int j = ja + g + gs * t;
         TT u[kPanel];
 #pragma unroll
-        for (int i = 0; i < kPanel; ++i) {
-            u[i] = A[pr[i] * lda + j];
-            pin(u[i]);
-        }
+        for (int i = 0; i < kPanel; ++i) u[i] = A[pr[i] * lda + j];
+#pragma unroll
+        for (int i = 0; i < kPanel; ++i) pin(u[i]);
         u_rows_group<TT, 0>(Lc, u);
         u_rows_group<TT, 4>(Lc, u);
         u_rows_group<TT, 8>(Lc, u);
@@ -1044,11 +1043,11 @@ __device__ __forceinline__ void lu_apply_panel(TT* A, int n, int lda, int c0, in
         ok[s] = li < n;
         prw[s] = ph[ok[s] ? li : 0];
 #pragma unroll
-        for (int i = 0; i < kPanel; ++i) {
-            L[s][i] = A[prw[s] * lda + c0 + (i < w ? i : 0)];
-            pin(L[s][i]);
-            L[s][i] = (ok[s] && (kFull || i < w)) ? L[s][i] : TT(0);
-        }
+        for (int i = 0; i < kPanel; ++i) L[s][i] = A[prw[s] * lda + c0 + (i < w ? i : 0)];
+#pragma unroll
+        for (int i = 0; i < kPanel; ++i) pin(L[s][i]);
+#pragma unroll
+        for (int i = 0; i < kPanel; ++i) L[s][i] = (ok[s] && (kFull || i < w)) ? L[s][i] : TT(0);
     }
     for (int t = 0; t < m; t += 2) {
         const int j = ja + g + gs * t;
@@ -1059,9 +1058,15 @@ __device__ __forceinline__ void lu_apply_panel(TT* A, int n, int lda, int c0, in
         for (int i = 0; i < kPanel; ++i) {
             u0[i] = A[pr[i] * lda + j];
             u1[i] = A[pr[i] * lda + jq];
-            if (!kFull) {
+        }
+        if (!kFull) {
+#pragma unroll
+            for (int i = 0; i < kPanel; ++i) {
                 pin(u0[i]);
                 pin(u1[i]);
+            }
+#pragma unroll
+            for (int i = 0; i < kPanel; ++i) {
                 u0[i] = i < w ? u0[i] : TT(0);
                 u1[i] = i < w ? u1[i] : TT(0);
             }
@@ -1224,11 +1229,11 @@ __device__ __forceinline__ void lu_factor_nopiv_f32(float* A, int n, int lda, co
             const bool ok = lane < kPanel ? lane < w : row < n;
             const int rs = ok ? row : k0;  // clamped: loads unconditional, values selected after
 #pragma unroll
-            for (int c = 0; c < kPanel; ++c) {
-                a[c] = A[rs * lda + k0 + (c < w ? c : 0)];
-                pin(a[c]);
-                a[c] = (ok && c < w) ? a[c] : 0.f;
-            }
+            for (int c = 0; c < kPanel; ++c) a[c] = A[rs * lda + k0 + (c < w ? c : 0)];
+#pragma unroll
+            for (int c = 0; c < kPanel; ++c) pin(a[c]);  // after all loads: one wait
+#pragma unroll
+            for (int c = 0; c < kPanel; ++c) a[c] = (ok && c < w) ? a[c] : 0.f;
             np_tile_steps<0>(a, w, lane);
             if (ok && lane >= kPanel) {
 #pragma unroll
@@ -1248,11 +1253,11 @@ __device__ __forceinline__ void lu_factor_nopiv_f32(float* A, int n, int lda, co
             const int js = okj ? j : k0;  // clamped: loads unconditional
             float u[kPanel];
 #pragma unroll
-            for (int i = 0; i < kPanel; ++i) {
-                u[i] = A[(k0 + (i < w ? i : 0)) * lda + js];
-                pin(u[i]);
-                u[i] = (okj && i < w) ? u[i] : 0.f;
-            }
+            for (int i = 0; i < kPanel; ++i) u[i] = A[(k0 + (i < w ? i : 0)) * lda + js];
+#pragma unroll
+            for (int i = 0; i < kPanel; ++i) pin(u[i]);
+#pragma unroll
+            for (int i = 0; i < kPanel; ++i) u[i] = (okj && i < w) ? u[i] : 0.f;
 #pragma unroll
             for (int i = 1; i < kPanel; ++i)
 #pragma unroll
@@ -1290,13 +1295,13 @@ __device__ __forceinline__ void lu_factor_nopiv_f32(float* A, int n, int lda, co
                 }
                 f32x4 acc;
                 const int ic = r0 + 16 * ti + 4 * lk;  // first output row of this lane
+                float v[4];
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const bool okc = okb && ic + r < n;
-                    float v = A[(okc ? ic + r : r0) * lda + (okb ? jb : r0)];
-                    pin(v);
-                    acc[r] = okc ? v : 0.f;
-                }
+                for (int r = 0; r < 4; ++r) v[r] = A[((okb && ic + r < n) ? ic + r : r0) * lda + (okb ? jb : r0)];
+#pragma unroll
+                for (int r = 0; r < 4; ++r) pin(v[r]);
+#pragma unroll
+                for (int r = 0; r < 4; ++r) acc[r] = (okb && ic + r < n) ? v[r] : 0.f;
 #pragma unroll
                 for (int ks = 0; ks < 4; ++ks) {
                     const float a_ = oka ? -ao[ks] : 0.f, b_ = okb ? bo[ks] : 0.f;
