@@ -1282,10 +1282,29 @@ __device__ __forceinline__ void lu_factor_nopiv_f32(float* A, int n, int lda, co
         if (w == kPanel && m > 0) {
             const int TR = (m + 15) >> 4, TC = (mc + 15) >> 4;
             const int li = lane & 15, lk = lane >> 4;  // operand row/column and k within a step
+            const int fr = m >> 4, fc = mc >> 4;       // tile rows / columns with no masked entry
             for (int t = wv; t < TR * TC; t += nw) {
                 const int ti = t / TC, tj = t - TC * (t / TC);
                 const int ia = r0 + 16 * ti + li;  // A-operand row (L21)
                 const int jb = r0 + 16 * tj + li;  // B-operand / output column
+                if (ti < fr && tj < fc) {  // uniform: a full tile, pointer steps and no guards
+                    const float* pa = A + ia * lda + k0 + lk;
+                    const float* pb = A + (k0 + lk) * lda + jb;
+                    float* pc = A + (r0 + 16 * ti + 4 * lk) * lda + jb;
+                    float ao[4], bo[4];
+                    f32x4 acc;
+#pragma unroll
+                    for (int ks = 0; ks < 4; ++ks) {
+                        ao[ks] = pa[4 * ks];
+                        bo[ks] = pb[4 * ks * lda];
+                        acc[ks] = pc[ks * lda];
+                    }
+#pragma unroll
+                    for (int ks = 0; ks < 4; ++ks) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(-ao[ks], bo[ks], acc, 0, 0, 0);
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) pc[r * lda] = acc[r];
+                    continue;
+                }
                 const bool oka = ia < n, okb = jb <= n;
                 float ao[4], bo[4];
 #pragma unroll
@@ -1393,7 +1412,9 @@ __device__ __forceinline__ double back_block_solve(const TT* A, int lda, const i
 #pragma unroll
     for (int c = kBackBlock - 1; c >= 0; --c) {
         if (c < nb) {  // uniform
-            const double xk = rdlane(div_by(r, d, dinv), c);
+            // FP64 factors: the correctly rounded quotient (the oracle's division); FP32 factors:
+            // the product with the reciprocal (one rounding more, far below the factors' own error)
+            const double xk = rdlane(sizeof(TT) == 8 ? div_by(r, d, dinv) : r * dinv, c);
             const double upd = fma(-(double)u[c], xk, r);
             r = lane < c ? upd : (lane == c ? xk : r);
         }
