@@ -11,9 +11,12 @@ rho_inf = 0.9, Newton with the reference's increment criterion (atol 1e-10, rtol
 Timed legs (all on the one GPU; each the median of --repeats runs of --steps steps after the
 warm-up, every run timed with a synchronize on both sides of its step loop):
   value            fused FP32-tangent form with --refine FP64 iterative-refinement sweeps per
-                   solve (in every Newton iteration unless --refine-bound), one
-                   hipGraph-replayed kernel per step; null (with `error` set) if any model-step
-                   of this leg failed to converge
+                   solve in the Newton iterations whose previous convergence error is below
+                   --refine-bound (default 10: in practice the iteration that converges, the
+                   only one whose solve error survives into the state), one hipGraph-replayed
+                   kernel per step; null (with `error` set) if any model-step of this leg failed
+                   to converge
+  fp32_refine_every_iteration  the same with the sweeps in every Newton iteration
   fp32_without_refinement  the same without refinement
   fused_fp64       fused form, FP64 tangent
   plain_fp64       the reference's launch structure (one kernel per stage, host reads the
@@ -23,8 +26,9 @@ per element and Newton iteration (s = 4), summed over the iterations actually ta
 HIP-event-timed duration; the kernel is latency-bound (the `bound` field names the byte
 roofline it is measured against), so the matrix-core rate of its MFMA work (contraction
 144 N^2 Q, LU trailing updates) is reported beside it against the FP32 MFMA peak.
-accuracy (extra field, untimed): the FP32-tangent and FP64 fused forms against the CPU FP64
-oracle on the first --acc-rotors rotors over --acc-steps steps (BASELINE: 100 steps): final q
+accuracy (extra field, untimed): the FP32-tangent form (the benched refinement policy, and
+refinement in every iteration) and the FP64 fused form against the CPU FP64 oracle on the first
+--acc-rotors rotors over --acc-steps steps (100 steps, the ensemble's length): final q
 per model relative to that model's max |q|, worst and median, models above 1e-9, and Newton
 iteration totals that differ from the oracle's.  The FP32 form is an inexact-tangent Newton
 method: where its path takes one iteration more or less than the oracle's in some step, the
@@ -113,19 +117,22 @@ def lu_update_flops(specs):
 
 
 def measured_traffic(refine, refine_bound):
-    """HBM bytes per launch of the FP32 step kernel from the committed counter summary for this
-    refinement setting (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs, copy-calibrated), if present.
-    The summaries are taken with every Newton iteration refined (bound 0)."""
-    if refine_bound > 0:
-        return None
-    name = f"k_fused_step_f32_refine{refine}_ensemble4096_traffic.txt"
-    path = ROOT / "profiles" / name
+    """HBM bytes per launch of the FP32 step kernel from the committed counter summary
+    (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs, copy-calibrated), if it was taken with this
+    refinement setting."""
+    path = ROOT / "profiles" / "k_fused_step_f32_ensemble4096_traffic.txt"
     if not path.exists():
         return None
+    fields = {}
     for line in path.read_text().splitlines():
-        if line.startswith("traffic_bytes_per_launch:"):
-            return {"bytes_per_launch": float(line.split(":", 1)[1]), "source": str(path.relative_to(ROOT))}
-    return None
+        if ":" in line and line.split(":", 1)[0] in ("refine_sweeps", "refine_err_bound", "traffic_bytes_per_launch"):
+            k, v = line.split(":", 1)
+            fields[k] = float(v)
+    if fields.get("refine_sweeps") != refine or fields.get("refine_err_bound") != refine_bound:
+        return None
+    if "traffic_bytes_per_launch" not in fields:
+        return None
+    return {"bytes_per_launch": fields["traffic_bytes_per_launch"], "source": str(path.relative_to(ROOT))}
 
 
 def run_leg(lib, specs, sched, *, exec_mode, precision, warmup, steps, repeats=1, refine=0, refine_bound=0.0,
@@ -237,7 +244,8 @@ def cpu_baseline(specs_sample, steps, lib_path, threads=None):
 
 
 def accuracy_leg(lib, lib_path, rotors, steps, refine, refine_bound):
-    """FP32-tangent (refine sweeps) and FP64 fused forms against the FP64 oracle, per model."""
+    """FP32-tangent (refine sweeps under the benched bound, and in every iteration) and FP64 fused
+    forms against the FP64 oracle, per model."""
     sys.path.insert(0, str(ROOT / "tests"))
     from oracle_lib import OracleLib, OracleSystem, step_many
 
@@ -245,9 +253,13 @@ def accuracy_leg(lib, lib_path, rotors, steps, refine, refine_bound):
     sched = W.ensemble_schedule(specs, steps)
     out = {"rotors": rotors, "steps": steps}
     res = {}
-    for name, prec, rf in (("fp32", ota.TANGENT_FP32, refine), ("fp64", ota.TANGENT_FP64, 0)):
+    forms = [("fp32", ota.TANGENT_FP32, refine, refine_bound)]
+    if refine > 0 and refine_bound > 0:
+        forms.append(("fp32_refine_every_iteration", ota.TANGENT_FP32, refine, 0.0))
+    forms.append(("fp64", ota.TANGENT_FP64, 0, 0.0))
+    for name, prec, rf, bound in forms:
         g = ota.KynSystem(lib, specs, exec_mode=ota.EXEC_FUSED, precision=prec, refine_sweeps=rf,
-                          refine_err_bound=refine_bound)
+                          refine_err_bound=bound)
         g.set_schedule(steps, *sched)
         g.step_many(steps, use_graph=True)
         it, fl = g.totals()
@@ -271,9 +283,9 @@ def accuracy_leg(lib, lib_path, rotors, steps, refine, refine_bound):
         out[name] = {"q_rel_worst_model": float(np.max(per_model)), "q_rel_median_model": float(np.median(per_model)),
                      "models_above_1e-9": int(np.sum(per_model > 1e-9)),
                      "models_with_other_newton_total": int(np.sum(it != its)), "failed_model_steps": fails}
-        if name == "fp32":
+        if name.startswith("fp32"):
             out[name]["refine_sweeps"] = refine
-            out[name]["refine_err_bound"] = refine_bound
+            out[name]["refine_err_bound"] = refine_bound if name == "fp32" else 0.0
         log(f"  {name} vs oracle after {steps} steps: worst model {out[name]['q_rel_worst_model']:.2e}, median "
             f"{out[name]['q_rel_median_model']:.2e}, {out[name]['models_above_1e-9']} of {rotors} above 1e-9, "
             f"{out[name]['models_with_other_newton_total']} with other Newton totals")
@@ -327,11 +339,12 @@ def main():
     ap.add_argument("--skip-plain", action="store_true")
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--refine", type=int, default=1, help="FP64 refinement sweeps of the FP32-tangent solve")
-    ap.add_argument("--refine-bound", type=float, default=0.0,
-                    help="refine only Newton iterations whose previous convergence error is below this (0: all; "
-                         "the default: with a bound of 1e3 the worst of 512 rotors ends 3.5e-9 from the oracle "
-                         "after 100 steps, with every iteration refined 2.8e-10)")
+    ap.add_argument("--refine-bound", type=float, default=10.0,
+                    help="refine only Newton iterations whose previous convergence error is below this (0: every "
+                         "iteration, timed beside the default as fp32_refine_every_iteration)")
     ap.add_argument("--skip-unrefined", action="store_true", help="skip the FP32 leg without refinement")
+    ap.add_argument("--skip-refine-all", action="store_true",
+                    help="skip the FP32 leg with refinement in every Newton iteration")
     ap.add_argument("--structure-steps", type=int, default=10, help="steps of the large-structure legs (0: skip)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run one device copy of BYTES (counter calibration) and exit")
@@ -370,6 +383,12 @@ def main():
               warmup=args.warmup, steps=args.steps, repeats=args.repeats, refine=args.refine,
               refine_bound=args.refine_bound)
     del r32["system"]
+    rall = None
+    if args.refine > 0 and args.refine_bound > 0 and not args.skip_refine_all:
+        rall = leg("fused FP32 tangent, refinement in every Newton iteration (hipGraph)", exec_mode=ota.EXEC_FUSED,
+                   precision=ota.TANGENT_FP32, warmup=args.warmup, steps=args.steps, repeats=args.repeats,
+                   refine=args.refine, refine_bound=0.0)
+        del rall["system"]
     r0 = None
     if args.refine > 0 and not args.skip_unrefined:
         r0 = leg("fused FP32 tangent without refinement (hipGraph)", exec_mode=ota.EXEC_FUSED,
@@ -442,6 +461,9 @@ def main():
         "newton_iters_per_model_step": {"fp32": round(float(r32["iters"].mean()) / args.steps, 3),
                                          "fp64": round(float(r64["iters"].mean()) / args.steps, 3) if r64 else None},
         "failed_model_steps": {"fp32": r32["fails"], "fp64": r64["fails"] if r64 else None},
+        "fp32_refine_every_iteration": {"steps_per_s": round(args.steps / rall["wall_s"], 4),
+                                        "kernel_ms": round(rall["kernel_ms"], 4),
+                                        "failed_model_steps": rall["fails"]} if rall else None,
         "fp32_without_refinement": {"steps_per_s": round(args.steps / r0["wall_s"], 4),
                                     "kernel_ms": round(r0["kernel_ms"], 4)} if r0 else None,
         "accuracy_vs_oracle": accuracy,

@@ -43,7 +43,7 @@ def main():
     ap.add_argument("--steps", type=int, default=6)
     ap.add_argument("--repeats", type=int, default=3)
     ap.add_argument("--refine", type=int, default=1)
-    ap.add_argument("--refine-bound", type=float, default=0.0)
+    ap.add_argument("--refine-bound", type=float, default=10.0)
     ap.add_argument("--write", action="store_true", help="write csrc/kyn_tuning.inc and the profile summary")
     ap.add_argument("--out", default=None, help="directory for both files instead of their places in the tree")
     args = ap.parse_args()

@@ -991,6 +991,29 @@ __device__ __forceinline__ void u_rows_group(const TT* Lc, TT* u) {
         for (int i = KG + kk + 1; i < kPanel; ++i) u[i] -= lk[kk][i] * u[KG + kk];
 }
 
+// The same four columns for the unpivoted FP32 LU, with explicit FMAs: u_i = fma(-l_ik, u_k,
+// u_i), k ascending for every i (the order of the row-oriented loop).
+template <int KG>
+__device__ __forceinline__ void u_rows_group_fma(const float* Lc, float (&u)[kPanel]) {
+    typedef float vec_t __attribute__((ext_vector_type(4)));
+    float lk[4][kPanel];
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk) {
+#pragma unroll
+        for (int q = 0; q < kPanel / 4; ++q) {
+            if (KG + kk < kPanel - 1 && (q + 1) * 4 > KG + kk + 1) {
+                const vec_t v = *reinterpret_cast<const vec_t*>(Lc + (KG + kk) * kPanel + q * 4);
+#pragma unroll
+                for (int e = 0; e < 4; ++e) lk[kk][q * 4 + e] = v[e];
+            }
+        }
+    }
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk)
+#pragma unroll
+        for (int i = KG + kk + 1; i < kPanel; ++i) u[i] = __builtin_fmaf(-lk[kk][i], u[KG + kk], u[i]);
+}
+
 // Applies panel [c0, c0 + w) to this wave's share of the columns [ja, jb): columns
 // ja + g, ja + g + gs, ... (wave g of gs).  First the panel's U rows (U12 = L11^-1 A12, lane <->
 // column, written in place into the pivot rows), then the trailing rows a_rj -= sum_k l_rk u_kj
@@ -1185,8 +1208,9 @@ __device__ __forceinline__ void lu_factor_augmented(TT* A, int n, int lda, int* 
 //      and applies it, as it goes, to up to 48 rows below in lanes 16..63 (L21 = A21 U11^-1,
 //      rows split over the waves);
 //  P2  in the same wave, with no barrier: U12 = L11^-1 A12 for 48 columns per wave in lanes
-//      16..63 (right-hand side column n included), L11 read by readlane from the wave's own
-//      copy of the factored tile; then the barrier, after which wave 0 writes the tile back;
+//      16..63 (right-hand side column n included), L11 broadcast from a column-major LDS copy
+//      of the factored tile that every wave writes identically; then the barrier, after which
+//      wave 0 writes the tile back;
 //  P3  A22 -= L21 U12 on the matrix cores: 16 x 16 output tiles, each the sum of four
 //      v_mfma_f32_16x16x4f32 products over the panel's 16 columns, tiles round-robin over the
 //      waves, rows past n - 1 and columns past n masked.
@@ -1239,14 +1263,25 @@ __device__ __forceinline__ void lu_factor_nopiv_f32(float* A, int n, int lda, co
 #pragma unroll
                 for (int c = 0; c < kPanel; ++c) A[row * lda + k0 + c] = a[c];
             }
+            // the factored tile, column-major, for P2's broadcast reads: every wave computed the
+            // same tile, so every wave writes the same values here and reads its own writes
+            if (lane < kPanel) {
+                float* Lt = panel_l11<float>(W, k0);
+#pragma unroll
+                for (int c = 0; c < kPanel; ++c) Lt[c * kPanel + lane] = a[c];
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         }
         KYN_SPAN_END(7, t_p1, 0);
         KYN_SPAN_BEGIN(t_p2);
         // ---- P2, before the barrier: U12 = L11^-1 A12 for the columns after the panel (the rhs
         // column n included; a partial last panel has only the rhs after it), 48 columns per wave
-        // in lanes 16..63.  L11 is this wave's own copy of the factored tile (lane i holds row i:
-        // L[i][k] = a[k] for k < i), read by readlane.  These columns are disjoint from every
-        // entry P1 reads or writes in any wave.
+        // in lanes 16..63.  L11 comes from the column-major tile written at the end of P1 (16-byte
+        // loads, one address for all lanes).  These columns are disjoint from every entry P1
+        // reads or writes in any wave.
+        const float* Lt = panel_l11<float>(W, k0);
         for (int jb = k0 + w + wv * (64 - kPanel); jb <= n; jb += nw * (64 - kPanel)) {
             const int j = jb + lane - kPanel;
             const bool okj = lane >= kPanel && j <= n;
@@ -1258,10 +1293,10 @@ __device__ __forceinline__ void lu_factor_nopiv_f32(float* A, int n, int lda, co
             for (int i = 0; i < kPanel; ++i) pin(u[i]);
 #pragma unroll
             for (int i = 0; i < kPanel; ++i) u[i] = (okj && i < w) ? u[i] : 0.f;
-#pragma unroll
-            for (int i = 1; i < kPanel; ++i)
-#pragma unroll
-                for (int k = 0; k < i; ++k) u[i] = __builtin_fmaf(-rdlane(a[k], i), u[k], u[i]);
+            u_rows_group_fma<0>(Lt, u);
+            u_rows_group_fma<4>(Lt, u);
+            u_rows_group_fma<8>(Lt, u);
+            u_rows_group_fma<12>(Lt, u);
             if (okj) {
 #pragma unroll
                 for (int i = 1; i < kPanel; ++i)
