@@ -1252,12 +1252,22 @@ __device__ __forceinline__ void lu_factor_nopiv_f32(float* A, int n, int lda, co
             const int row = lane < kPanel ? k0 + lane : r0 + wv * (64 - kPanel) + (lane - kPanel);
             const bool ok = lane < kPanel ? lane < w : row < n;
             const int rs = ok ? row : k0;  // clamped: loads unconditional, values selected after
+            const float* ar = A + rs * lda + k0;
+            if (w == kPanel) {  // uniform: every panel but a partial last one, constant offsets
 #pragma unroll
-            for (int c = 0; c < kPanel; ++c) a[c] = A[rs * lda + k0 + (c < w ? c : 0)];
+                for (int c = 0; c < kPanel; ++c) a[c] = ar[c];
 #pragma unroll
-            for (int c = 0; c < kPanel; ++c) pin(a[c]);  // after all loads: one wait
+                for (int c = 0; c < kPanel; ++c) pin(a[c]);  // after all loads: one wait
 #pragma unroll
-            for (int c = 0; c < kPanel; ++c) a[c] = (ok && c < w) ? a[c] : 0.f;
+                for (int c = 0; c < kPanel; ++c) a[c] = ok ? a[c] : 0.f;
+            } else {
+#pragma unroll
+                for (int c = 0; c < kPanel; ++c) a[c] = ar[c < w ? c : 0];
+#pragma unroll
+                for (int c = 0; c < kPanel; ++c) pin(a[c]);
+#pragma unroll
+                for (int c = 0; c < kPanel; ++c) a[c] = (ok && c < w) ? a[c] : 0.f;
+            }
             np_tile_steps<0>(a, w, lane);
             if (ok && lane >= kPanel) {
 #pragma unroll
@@ -1862,6 +1872,48 @@ __device__ __forceinline__ double block_error_and_update(const DevSys& S, int b,
     const int nb0 = S.block_node_begin[b];
     const int t = threadIdx.x;
     double term = 0.;
+    if (n <= (int)blockDim.x) {  // uniform: one row per thread, the update's loads issued with the error's
+        const int i = t;
+        double qd = 0., v = 0., vd = 0.;
+        int k = 0;
+        if (i < nsys) {
+            k = S.block_nodes[nb0 + i / 6] * 6 + i % 6;
+            qd = S.q_delta[k];
+            if (p.is_dynamic) {
+                v = S.v[k];
+                vd = S.vd[k];
+            }
+        }
+        if (i < nsys) {
+            const double e = xs[i] / (p.atol + p.rtol * fabs(qd * p.h));
+            term = e * e;
+        } else if (i < n) {
+            double e = 0.;
+            for (int q = S.block_con_begin[b]; q < S.block_con_end[b]; ++q) {
+                const int c = S.block_cons[q];
+                const int r = i - S.con_local[c];
+                if (r >= 0 && r < S.con_rows[c]) e = xs[i] / (p.atol + p.rtol * fabs(S.con_lambda[c * 6 + r]));
+            }
+            term = e * e;
+        }
+        const double sum = block_sum(term, red);
+        if (i < nsys) {  // the same update as below, from the values already in registers
+            const double dx = xs[i];
+            S.q_delta[k] = qd + dx / p.h;
+            if (p.is_dynamic) {
+                S.v[k] = v + p.gamma_prime * dx;
+                S.vd[k] = vd + p.beta_prime * dx;
+            }
+        }
+        for (int q = S.block_con_begin[b] + t; q < S.block_con_end[b]; q += blockDim.x) {
+            const int c = S.block_cons[q];
+            for (int r = 0; r < S.con_rows[c]; ++r) S.con_lambda[c * 6 + r] += xs[S.con_local[c] + r];
+        }
+        __syncthreads();
+        for (int q = nb0 + t; q < S.block_node_end[b]; q += blockDim.x) displacement_node(S, S.block_nodes[q]);
+        __syncthreads();
+        return sum;
+    }
     for (int i = t; i < n; i += blockDim.x) {
         double e;
         if (i < nsys) {
