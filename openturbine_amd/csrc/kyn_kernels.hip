@@ -1332,6 +1332,41 @@ __device__ __forceinline__ void lu_factor_nopiv_f32(float* A, int n, int lda, co
                 const int ti = t / TC, tj = t - TC * (t / TC);
                 const int ia = r0 + 16 * ti + li;  // A-operand row (L21)
                 const int jb = r0 + 16 * tj + li;  // B-operand / output column
+                const int t2 = t + nw, ti2 = t2 / TC, tj2 = t2 - TC * (t2 / TC);
+                if (ti < fr && tj < fc && t2 < TR * TC && ti2 < fr && tj2 < fc) {
+                    // uniform: this wave's next tile is full too: both at once, their loads
+                    // behind one wait and their two MFMA chains interleaved
+                    const int ib = r0 + 16 * ti2 + li, jb2 = r0 + 16 * tj2 + li;
+                    const float* pa0 = A + ia * lda + k0 + lk;
+                    const float* pb0 = A + (k0 + lk) * lda + jb;
+                    float* pc0 = A + (r0 + 16 * ti + 4 * lk) * lda + jb;
+                    const float* pa1 = A + ib * lda + k0 + lk;
+                    const float* pb1 = A + (k0 + lk) * lda + jb2;
+                    float* pc1 = A + (r0 + 16 * ti2 + 4 * lk) * lda + jb2;
+                    float ao0[4], bo0[4], ao1[4], bo1[4];
+                    f32x4 acc0, acc1;
+#pragma unroll
+                    for (int ks = 0; ks < 4; ++ks) {
+                        ao0[ks] = pa0[4 * ks];
+                        bo0[ks] = pb0[4 * ks * lda];
+                        acc0[ks] = pc0[ks * lda];
+                        ao1[ks] = pa1[4 * ks];
+                        bo1[ks] = pb1[4 * ks * lda];
+                        acc1[ks] = pc1[ks * lda];
+                    }
+#pragma unroll
+                    for (int ks = 0; ks < 4; ++ks) {
+                        acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(-ao0[ks], bo0[ks], acc0, 0, 0, 0);
+                        acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(-ao1[ks], bo1[ks], acc1, 0, 0, 0);
+                    }
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        pc0[r * lda] = acc0[r];
+                        pc1[r * lda] = acc1[r];
+                    }
+                    t = t2;
+                    continue;
+                }
                 if (ti < fr && tj < fc) {  // uniform: a full tile, pointer steps and no guards
                     const float* pa = A + ia * lda + k0 + lk;
                     const float* pb = A + (k0 + lk) * lda + jb;
