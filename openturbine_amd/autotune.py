@@ -3,7 +3,8 @@
 For each bucket N in {6, 11, 16, 21} (Q = 2N - 2, the ensemble's element sizes) it times the
 fused FP32 step on a single-size ensemble (every blade N nodes) with the contraction on the
 matrix cores and on the VALU register tile (kyn_debug_set_contraction_mask), at the bench
-default refinement, and keeps the faster engine.  Sizes between buckets take the nearest
+default refinement policy (one sweep, in the Newton iterations whose previous convergence error is
+below 10), and keeps the faster engine.  Sizes between buckets take the nearest
 bucket's choice.  With --write it writes the table the library compiles in
 (csrc/kyn_tuning.inc) and a summary under profiles/; rebuild the library afterwards.
 
@@ -87,9 +88,10 @@ def main():
         "}  // namespace kyn\n")
     prof.write_text(
         "Contraction engine per element-size bucket — fused FP32 step kernel (k_fused_step_f32),\n"
-        f"single-size ensembles of {args.rotors} rotors (three N-node blades each, Q = 2N - 2), one FP64\n"
-        f"refinement sweep per Newton iteration (bound {args.refine_bound:g}), median of {args.repeats} runs of "
-        f"{args.steps} steps\n(HIP events around the graph launches), MI355X.\n\n"
+        f"single-size ensembles of {args.rotors} rotors (three N-node blades each, Q = 2N - 2), {args.refine} FP64\n"
+        f"refinement sweep(s) in the Newton iterations whose previous convergence error is below "
+        f"{args.refine_bound:g} (0: every iteration),\nmedian of {args.repeats} runs of {args.steps} steps "
+        f"(HIP events around the graph launches), MI355X.\n\n"
         "Command (repository root):\n"
         f"  python3 -m openturbine_amd.autotune --rotors {args.rotors} --steps {args.steps} --repeats {args.repeats} "
         "--write\n\n"
