@@ -306,7 +306,7 @@ def accuracy_leg(lib, lib_path, rotors, steps, refine, refine_bound):
 
 
 def config3_leg(lib, lib_path, refine, refine_bound, steps=200):
-    """the FP32-tangent form of the config-1 cantilever against the oracle."""
+    """The FP32-tangent form of the follower-load cantilever (200 steps) against the oracle."""
     sys.path.insert(0, str(ROOT / "tests"))
     from oracle_lib import OracleLib, OracleSystem
 

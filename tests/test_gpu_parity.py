@@ -181,7 +181,7 @@ def check_histories(hist, label):
 
 @pytest.mark.parametrize("mode", MODES)
 def test_config1_cantilever_follower_load(kyn, orc, orc_nc, mode):
-    """BASELINE config 1 at its full length: 200 steps, h = 0.005."""
+    """The follower-load cantilever at its full length: 200 steps, h = 0.005."""
     spec = W.cantilever_config_spec(num_steps=200)
     g, o, hist = run_pair(kyn, orc, orc_nc, spec, 200, mode)
     for sg, so, _, _ in hist:
@@ -193,7 +193,7 @@ def test_config1_cantilever_follower_load(kyn, orc, orc_nc, mode):
 
 @pytest.mark.parametrize("mode", MODES)
 def test_config2_rotor(kyn, orc, orc_nc, mode):
-    """BASELINE config 2 at its full length: 500 steps, h = 0.01."""
+    """The IEA-15 rotor at its full length: 500 steps, h = 0.01."""
     spec = W.rotor_config_spec(num_steps=500)
     g, o, hist = run_pair(kyn, orc, orc_nc, spec, 500, mode)
     for sg, so, _, _ in hist:
@@ -205,7 +205,7 @@ def test_config2_rotor(kyn, orc, orc_nc, mode):
 
 @pytest.mark.parametrize("refine", [0, 1])
 def test_config3_fp32_tangent_cantilever(kyn, orc, refine):
-    """config 1 with the tangent and its factorization in FP32 (FP64 residual,
+    """The follower-load cantilever with the tangent and its factorization in FP32 (FP64 residual,
     rotations and update; `refine` FP64 refinement sweeps per solve), 200 steps, against the FP64
     oracle: max-norm tip displacement and rotation-vector error over the whole history, Newton
     iteration counts."""
