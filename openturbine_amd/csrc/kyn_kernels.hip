@@ -63,6 +63,14 @@ struct Lay {
 
 __host__ __device__ inline int align16(int x) { return (x + 15) & ~15; }
 
+// Empty asm use: the value must exist here (keeps a load unconditional instead of being sunk
+// into a branch by the select that follows it).  Placed after a batch of loads, it costs one
+// wait for the batch.
+template <typename TT>
+__device__ __forceinline__ void pin(TT& v) {
+    asm volatile("" : "+v"(v));
+}
+
 // cm: most constraints in one block; refine: FP64 refinement scratch needed.
 __host__ __device__ inline Lay make_layout(int Nm, int Qm, int nm, int cm, bool refine, int tt_size, bool with_matrix) {
     Lay L{};
@@ -772,12 +780,6 @@ __device__ __forceinline__ double rdlane(double v, int l) {
     const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), l);
     const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
-}
-// Empty asm use: the value must exist here (keeps a load unconditional instead of being sunk
-// into a branch by the select that follows it).
-template <typename TT>
-__device__ __forceinline__ void pin(TT& v) {
-    asm volatile("" : "+v"(v));
 }
 // Empty asm use tagged with the register slot: keeps per-slot branches from being merged into
 // a dynamically indexed (scratch) register read.
@@ -1656,24 +1658,35 @@ __device__ __forceinline__ void lu_solve(const TT* A, int n, int lda, const int*
 // PredictNextState (src/step/predict_next_state.hpp:14-47, calculate_next_state.hpp:26-37)
 __device__ __forceinline__ void predict_node(const DevSys& S, int node) {
     const StepConsts& p = S.p;
+    // every load first (the state arrays may alias as far as the compiler knows, so loads
+    // interleaved with stores would each wait for HBM), then the arithmetic, then the stores
+    double qp[7], v_p[6], vd_p[6], a_p[6];
 #pragma unroll
-    for (int c = 0; c < 7; ++c) S.q_prev[node * 7 + c] = S.q[node * 7 + c];
+    for (int c = 0; c < 7; ++c) qp[c] = S.q[node * 7 + c];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+        v_p[c] = S.v[node * 6 + c];
+        vd_p[c] = S.vd[node * 6 + c];
+        a_p[c] = S.a[node * 6 + c];
+    }
+    double an[6], vn[6], qd[6], qn[7];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+        an[c] = (p.alpha_f * vd_p[c] - p.alpha_m * a_p[c]) / (1. - p.alpha_m);
+        vn[c] = v_p[c] + p.h * (1. - p.gamma) * a_p[c] + p.gamma * p.h * an[c];
+        qd[c] = v_p[c] + (0.5 - p.beta) * p.h * a_p[c] + p.beta * p.h * an[c];
+    }
+    calc_displacement(p.h, qd, qp, qn);
+#pragma unroll
+    for (int c = 0; c < 7; ++c) S.q_prev[node * 7 + c] = qp[c];
 #pragma unroll
     for (int c = 0; c < 6; ++c) {
         const int k = node * 6 + c;
-        const double v_p = S.v[k], vd_p = S.vd[k], a_p = S.a[k];
-        const double an = (p.alpha_f * vd_p - p.alpha_m * a_p) / (1. - p.alpha_m);
         S.vd[k] = 0.;
-        S.a[k] = an;
-        S.v[k] = v_p + p.h * (1. - p.gamma) * a_p + p.gamma * p.h * an;
-        S.q_delta[k] = v_p + (0.5 - p.beta) * p.h * a_p + p.beta * p.h * an;
+        S.a[k] = an[c];
+        S.v[k] = vn[c];
+        S.q_delta[k] = qd[c];
     }
-    double qd[6], qp[7], qn[7];
-#pragma unroll
-    for (int c = 0; c < 6; ++c) qd[c] = S.q_delta[node * 6 + c];
-#pragma unroll
-    for (int c = 0; c < 7; ++c) qp[c] = S.q_prev[node * 7 + c];
-    calc_displacement(p.h, qd, qp, qn);
 #pragma unroll
     for (int c = 0; c < 7; ++c) S.q[node * 7 + c] = qn[c];
 }

@@ -121,7 +121,8 @@ def run_pair(kyn, orc, orc_nc, spec, steps, mode, inputs_of=None, **kw):
     """Steps the GPU form, the oracle and the no-contraction oracle side by side; per step: the
     GPU's and the oracle's statuses, the history deviation GPU vs oracle and the history
     deviation of the two oracle builds (the run's rounding sensitivity).  Inputs come from
-    `inputs_of(k)`, else from the spec (follower loads read each system's own state)."""
+    `inputs_of(k)`, else from the spec (follower loads read each system's own state).  Returns
+    the GPU system, the oracle, the per-step records and the no-contraction oracle."""
     g = ota.KynSystem(kyn, spec, exec_mode=mode, **kw)
     o = OracleSystem(orc, spec)
     o2 = OracleSystem(orc_nc, spec)
@@ -138,7 +139,7 @@ def run_pair(kyn, orc, orc_nc, spec, steps, mode, inputs_of=None, **kw):
         o2.step(ins[2])
         hist.append((sg, so, history_deviation(g.convergence(), o.convergence()),
                      history_deviation(o2.convergence(), o.convergence())))
-    return g, o, hist
+    return g, o, hist, o2
 
 
 def history_deviation(hg, ho):
@@ -179,28 +180,42 @@ def check_histories(hist, label):
     assert worst_g[0] <= bars[0] and worst_g[1] <= bars[1], (worst_g, worst_r)
 
 
+# Final states: the fixed bars of STATE_TOL, widened to STATE_FACTOR times the two oracle
+# builds' own distance where a long run amplifies rounding beyond them (the IEA-15 rotor over
+# 500 steps: the builds differ by 1.2e-9 in vd and 4e-10 in a).
+STATE_FACTOR = 10.0
+
+
+def check_states(g, o, o2, label, fields=STATE_FIELDS):
+    for f in fields:
+        d, r = rel(g.get(f), o.get(f)), rel(o2.get(f), o.get(f))
+        bar = max(STATE_TOL[f], STATE_FACTOR * r)
+        print(f"{label}: final {f} GPU vs oracle {d:.2e} (bar {bar:.1e}; the two oracle builds {r:.2e})", flush=True)
+        assert d < bar, (f, d, r)
+
+
 @pytest.mark.parametrize("mode", MODES)
 def test_config1_cantilever_follower_load(kyn, orc, orc_nc, mode):
     """The follower-load cantilever at its full length: 200 steps, h = 0.005."""
     spec = W.cantilever_config_spec(num_steps=200)
-    g, o, hist = run_pair(kyn, orc, orc_nc, spec, 200, mode)
+    g, o, hist, o2 = run_pair(kyn, orc, orc_nc, spec, 200, mode)
     for sg, so, _, _ in hist:
         assert sg == 0 and so == 0
-    check_histories(hist, f"config 1 ({'fused' if mode == ota.EXEC_FUSED else 'plain'})")
-    for f in STATE_FIELDS:
-        assert rel(g.get(f), o.get(f)) < STATE_TOL[f], f
+    label = f"config 1 ({'fused' if mode == ota.EXEC_FUSED else 'plain'})"
+    check_histories(hist, label)
+    check_states(g, o, o2, label)
 
 
 @pytest.mark.parametrize("mode", MODES)
 def test_config2_rotor(kyn, orc, orc_nc, mode):
     """The IEA-15 rotor at its full length: 500 steps, h = 0.01."""
     spec = W.rotor_config_spec(num_steps=500)
-    g, o, hist = run_pair(kyn, orc, orc_nc, spec, 500, mode)
+    g, o, hist, o2 = run_pair(kyn, orc, orc_nc, spec, 500, mode)
     for sg, so, _, _ in hist:
         assert sg == so == 0
-    check_histories(hist, f"config 2 ({'fused' if mode == ota.EXEC_FUSED else 'plain'})")
-    for f in STATE_FIELDS:
-        assert rel(g.get(f), o.get(f)) < STATE_TOL[f], f
+    label = f"config 2 ({'fused' if mode == ota.EXEC_FUSED else 'plain'})"
+    check_histories(hist, label)
+    check_states(g, o, o2, label)
 
 
 @pytest.mark.parametrize("refine", [0, 1])
@@ -381,7 +396,7 @@ def test_static_solve(kyn, orc, orc_nc, mode):
             f[tip, 2] = P * (k + 1) / steps
             return {"nodal_loads": f}
 
-        g, o, hist = run_pair(kyn, orc, orc_nc, spec, steps, mode, inputs_of=loads)
+        g, o, hist, _ = run_pair(kyn, orc, orc_nc, spec, steps, mode, inputs_of=loads)
         for sg, so, _, _ in hist:
             assert sg == 0 and so == 0
         check_histories(hist, f"static P={P:g} ({'fused' if mode == ota.EXEC_FUSED else 'plain'})")
