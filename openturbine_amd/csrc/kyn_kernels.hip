@@ -85,7 +85,7 @@ __host__ __device__ inline Lay make_layout(int Nm, int Qm, int nm, int cm, bool 
     L.xs = take(nm * 8);
     L.xs2 = take(refine ? nm * 8 : 0);
     L.piv = take(nm * 4);
-    L.red = take(16 * 8);
+    L.red = take(24 * 8);  // block reductions; in the constraint phase two 3 x 3 tangent blocks
     L.cres = take(cm * 24 * 8);  // per constraint: res, lres_b, lres_t, sys_res
     L.misc = take(16 * 4);
     L.phys_of = take(2 * kLuTile * 4);
@@ -1811,32 +1811,63 @@ __device__ __forceinline__ void block_constraints(const DevSys& S, int b, char* 
     const int c0 = S.block_con_begin[b], c1 = S.block_con_end[b];
     const int nc = c1 - c0;
     const int t = threadIdx.x;
+    // One constraint (every fused blade block): lane 0 of wave 1 forms its target's rotational
+    // tangent block and lane 0 of wave 2 its base's, into the reduction scratch (free in this
+    // phase), while thread 0 evaluates the constraint; one barrier joins them.  Otherwise thread
+    // c does all of constraint c.
+    const bool split = nc == 1;  // uniform
+    double* trr_sh = reinterpret_cast<double*>(sm + L.red);  // [2][9]: target, base
+    if (split && (t == 64 || t == 128)) {
+        const int c = S.block_cons[c0];
+        const bool base = t == 128;
+        if (!base || con_two_node(S.con_type[c])) {
+            const int nd = base ? S.con_base[c] : S.con_target[c];
+            double qd[6], trr[9];
+#pragma unroll
+            for (int k = 0; k < 6; ++k) qd[k] = S.q_delta[nd * 6 + k];
+            tangent_rot_block(S.p.h, qd, trr);
+#pragma unroll
+            for (int k = 0; k < 9; ++k) trr_sh[(base ? 9 : 0) + k] = trr[k];
+        }
+    }
+    int c = 0, type = 0, tn = 0, bn = 0;
+    bool two = false;
+    double res[6], bg[36], tg[36], sres[6] = {0., 0., 0., 0., 0., 0.}, lam[6];
     if (t < nc) {
-        const int c = S.block_cons[c0 + t];
-        const int type = S.con_type[c];
-        const bool two = con_two_node(type);
-        const int tn = S.con_target[c], bn = S.con_base[c];
+        c = S.block_cons[c0 + t];
+        type = S.con_type[c];
+        two = con_two_node(type);
+        tn = S.con_target[c];
+        bn = S.con_base[c];
         double input[7];
 #pragma unroll
         for (int k = 0; k < 7; ++k) input[k] = S.con_input[c * 7 + k];
         if (con_controlled(type)) input[0] = S.con_control[c];  // Constraints::UpdateViews
+#pragma unroll
+        for (int k = 0; k < 6; ++k) lam[k] = S.con_lambda[c * 6 + k];
         double tq[7], bq[7] = {0., 0., 0., 1., 0., 0., 0.};
 #pragma unroll
         for (int k = 0; k < 7; ++k) tq[k] = S.q[tn * 7 + k];
         if (two)
 #pragma unroll
             for (int k = 0; k < 7; ++k) bq[k] = S.q[bn * 7 + k];
-        double res[6], bg[36], tg[36], sres[6] = {0., 0., 0., 0., 0., 0.};
         constraint_eval(type, S.con_X0 + c * 3, S.con_axes + c * 9, input, bq, tq, res, bg, tg, sres);
+    }
+    if (split) __syncthreads();
+    if (t < nc) {
         const int rows = S.con_rows[c];
         const int lr = S.con_local[c];
         double trr[9], qd[6], gt[36];
+        if (split) {
 #pragma unroll
-        for (int k = 0; k < 6; ++k) qd[k] = S.q_delta[tn * 6 + k];
-        tangent_rot_block(S.p.h, qd, trr);
+            for (int k = 0; k < 9; ++k) trr[k] = trr_sh[k];
+        } else {
+#pragma unroll
+            for (int k = 0; k < 6; ++k) qd[k] = S.q_delta[tn * 6 + k];
+            tangent_rot_block(S.p.h, qd, trr);
+        }
         times_tangent(tg, trr, gt);
         const int lt = S.node_local[tn];
-        const double* lam = S.con_lambda + c * 6;
         double* cr = cres + t * 24;
 #pragma unroll
         for (int i = 0; i < 6; ++i)
@@ -1854,9 +1885,14 @@ __device__ __forceinline__ void block_constraints(const DevSys& S, int b, char* 
             cr[12 + i] = acc;  // target lambda residual
         }
         if (two) {
+            if (split) {
 #pragma unroll
-            for (int k = 0; k < 6; ++k) qd[k] = S.q_delta[bn * 6 + k];
-            tangent_rot_block(S.p.h, qd, trr);
+                for (int k = 0; k < 9; ++k) trr[k] = trr_sh[9 + k];
+            } else {
+#pragma unroll
+                for (int k = 0; k < 6; ++k) qd[k] = S.q_delta[bn * 6 + k];
+                tangent_rot_block(S.p.h, qd, trr);
+            }
             times_tangent(bg, trr, gt);
             const int lb = S.node_local[bn];
 #pragma unroll
