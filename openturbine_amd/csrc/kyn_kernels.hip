@@ -1790,8 +1790,8 @@ __global__ void k_advance_slot(DevSys S) {
 // rotational rows and columns hold nothing but a unit diagonal and a zero rhs, so their
 // increments are exactly zero and the elimination of every other row is unchanged.
 template <typename TT>
-__device__ __forceinline__ void pin_rotations(const DevSys& S, int b, TT* A, int lda, double* rhs) {
-    if (S.block_npin[b] == 0) return;  // uniform over the workgroup
+__device__ __forceinline__ void pin_rotations(const DevSys& S, int b, TT* A, int lda, double* rhs, int npin = -1) {
+    if ((npin < 0 ? S.block_npin[b] : npin) == 0) return;  // uniform over the workgroup
     for (int k = S.block_node_begin[b] + threadIdx.x; k < S.block_node_end[b]; k += blockDim.x) {
         const int node = S.block_nodes[k];
         if (!S.node_pin[node]) continue;
@@ -1816,6 +1816,7 @@ __device__ __forceinline__ void block_constraints(const DevSys& S, int b, char* 
     // phase), while thread 0 evaluates the constraint; one barrier joins them.  Otherwise thread
     // c does all of constraint c.
     const bool split = nc == 1;  // uniform
+    const int npin = S.block_npin[b];  // loaded now, used after the last barrier
     double* trr_sh = reinterpret_cast<double*>(sm + L.red);  // [2][9]: target, base
     if (split && (t == 64 || t == 128)) {
         const int c = S.block_cons[c0];
@@ -1830,7 +1831,7 @@ __device__ __forceinline__ void block_constraints(const DevSys& S, int b, char* 
             for (int k = 0; k < 9; ++k) trr_sh[(base ? 9 : 0) + k] = trr[k];
         }
     }
-    int c = 0, type = 0, tn = 0, bn = 0;
+    int c = 0, type = 0, tn = 0, bn = 0, rows = 0, lr = 0, lt = 0, lb = 0;
     bool two = false;
     double res[6], bg[36], tg[36], sres[6] = {0., 0., 0., 0., 0., 0.}, lam[6];
     if (t < nc) {
@@ -1839,6 +1840,10 @@ __device__ __forceinline__ void block_constraints(const DevSys& S, int b, char* 
         two = con_two_node(type);
         tn = S.con_target[c];
         bn = S.con_base[c];
+        rows = S.con_rows[c];
+        lr = S.con_local[c];
+        lt = S.node_local[tn];
+        lb = two ? S.node_local[bn] : 0;
         double input[7];
 #pragma unroll
         for (int k = 0; k < 7; ++k) input[k] = S.con_input[c * 7 + k];
@@ -1855,8 +1860,6 @@ __device__ __forceinline__ void block_constraints(const DevSys& S, int b, char* 
     }
     if (split) __syncthreads();
     if (t < nc) {
-        const int rows = S.con_rows[c];
-        const int lr = S.con_local[c];
         double trr[9], qd[6], gt[36];
         if (split) {
 #pragma unroll
@@ -1867,7 +1870,6 @@ __device__ __forceinline__ void block_constraints(const DevSys& S, int b, char* 
             tangent_rot_block(S.p.h, qd, trr);
         }
         times_tangent(tg, trr, gt);
-        const int lt = S.node_local[tn];
         double* cr = cres + t * 24;
 #pragma unroll
         for (int i = 0; i < 6; ++i)
@@ -1894,7 +1896,6 @@ __device__ __forceinline__ void block_constraints(const DevSys& S, int b, char* 
                 tangent_rot_block(S.p.h, qd, trr);
             }
             times_tangent(bg, trr, gt);
-            const int lb = S.node_local[bn];
 #pragma unroll
             for (int i = 0; i < 6; ++i)
                 if (i < rows)
@@ -1918,10 +1919,18 @@ __device__ __forceinline__ void block_constraints(const DevSys& S, int b, char* 
             S.con_res[c * 6 + k] = res[k];
             S.con_sys_res[c * 6 + k] = sres[k];
         }
+        if (split) {  // the one constraint's rhs terms, in the order of the loop below
+            double* rhs = reinterpret_cast<double*>(sm + L.rhs);
+            for (int j = 0; j < 6; ++j) rhs[lt + j] += cr[18 + j];
+            if (two)
+                for (int j = 0; j < 6; ++j) rhs[lb + j] += cr[6 + j];
+            for (int j = 0; j < 6; ++j) rhs[lt + j] += cr[12 + j];
+            for (int i = 0; i < rows; ++i) rhs[lr + i] = 0. + cr[i];
+        }
     }
     __syncthreads();
     // rhs in the reference's order (assemble_constraints_residual.hpp:27-50)
-    if (t == 0 && nc > 0) {
+    if (t == 0 && nc > 0 && !split) {
         double* rhs = reinterpret_cast<double*>(sm + L.rhs);
         for (int k = 0; k < nc; ++k) {
             const int c = S.block_cons[c0 + k];
@@ -1945,7 +1954,7 @@ __device__ __forceinline__ void block_constraints(const DevSys& S, int b, char* 
         }
     }
     __syncthreads();
-    pin_rotations<TT>(S, b, A, L.lda, reinterpret_cast<double*>(sm + L.rhs));
+    pin_rotations<TT>(S, b, A, L.lda, reinterpret_cast<double*>(sm + L.rhs), npin);
 }
 
 // Convergence-error terms and state update of one block after its solve
