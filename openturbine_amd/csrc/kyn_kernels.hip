@@ -2867,108 +2867,127 @@ __global__ void __launch_bounds__(kTreeThreads) k_tree_interface(DevSys S, int b
     double* Xs = Lp + KK * nv;                   // [nv][K][K] X_v
     double* xh = Xs + KK * nv;                   // [nv][K]: the rhs, then zhat_v
     double* zs = xh + K * nv;                    // [nv][K]: z_v
-    // ---- every vertex's blocks and rhs from Z into LDS, all at once (independent loads)
-    for (int q = threadIdx.x; q < nv * (3 * KK + K); q += blockDim.x) {
-        const int lv = q / (3 * KK + K), e = q % (3 * KK + K), v = vb + lv;
+    // ---- every vertex's blocks and rhs from Z into LDS: thread per vertex, its row lists read
+    // once, then each block's 36 entries as one batch of loads (indices clamped, values selected
+    // after one wait)
+    for (int lv = threadIdx.x; lv < nv; lv += blockDim.x) {
+        const int v = vb + lv;
         const int s = S.tree_vsize[v], p = S.tree_vparent[v];
         const int sp = p >= 0 ? S.tree_vsize[p] : 0;
-        const int* rv = S.tree_vrows + S.tree_vrow_off[v];
-        const int* rp = p >= 0 ? S.tree_vrows + S.tree_vrow_off[p] : rv;
-        if (e < 3 * KK) {
-            const int which = e / KK, i = (e % KK) / K, j = e % K;
-            double val = 0.;
-            if (which == 0 && i < s && j < s) val = Z[(size_t)rv[j] * nz + rv[i]];
-            if (which == 1 && i < s && j < sp) val = Z[(size_t)rp[j] * nz + rv[i]];
-            if (which == 2 && i < sp && j < s) val = Z[(size_t)rv[j] * nz + rp[i]];
-            (which == 0 ? Ds : which == 1 ? Up : Lp)[lv * KK + i * K + j] = val;
-        } else {
-            const int i = e - 3 * KK;
-            xh[lv * K + i] = i < s ? r[rv[i]] : 0.;
+        const int* rvp = S.tree_vrows + S.tree_vrow_off[v];
+        const int* rpp = p >= 0 ? S.tree_vrows + S.tree_vrow_off[p] : rvp;
+        int rv[K], rp[K];
+#pragma unroll
+        for (int i = 0; i < K; ++i) {
+            rv[i] = i < s ? rvp[i] : 0;
+            rp[i] = i < sp ? rpp[i] : 0;
         }
+#pragma unroll
+        for (int which = 0; which < 3; ++which) {
+            const int* ra = which == 2 ? rp : rv;  // block rows
+            const int* ca = which == 1 ? rp : rv;  // block columns
+            const int na = which == 2 ? sp : s, nc = which == 1 ? sp : s;
+            double val[KK];
+#pragma unroll
+            for (int i = 0; i < K; ++i)
+#pragma unroll
+                for (int j = 0; j < K; ++j) val[i * K + j] = Z[(size_t)ca[j] * nz + ra[i]];
+#pragma unroll
+            for (int e = 0; e < KK; ++e) pin(val[e]);
+            double* dst = (which == 0 ? Ds : which == 1 ? Up : Lp) + lv * KK;
+#pragma unroll
+            for (int i = 0; i < K; ++i)
+#pragma unroll
+                for (int j = 0; j < K; ++j) dst[i * K + j] = (i < na && j < nc) ? val[i * K + j] : 0.;
+        }
+        double rr[K];
+#pragma unroll
+        for (int i = 0; i < K; ++i) rr[i] = r[rv[i]];
+#pragma unroll
+        for (int i = 0; i < K; ++i) xh[lv * K + i] = i < s ? rr[i] : 0.;
     }
     __syncthreads();
-    // ---- elimination, leaves first
+    // ---- elimination, leaves first: one wave per vertex, lane i < K holding row i of [M | R]
+    // (R: the K columns of Z_v,parent, then the rhs); every entry receives the operations of the
+    // row-by-row elimination in the same order, lanes >= K compute on a copy of row 0 and store
+    // nothing
+    const int wv = (int)threadIdx.x >> 6, lane = (int)threadIdx.x & 63, nwv = (int)blockDim.x >> 6;
+    const int li = lane < K ? lane : 0;
     for (int l = 0; l < tb[3]; ++l) {
         const int lo = S.tree_elim_lvl[tb[2] + l], hi = S.tree_elim_lvl[tb[2] + l + 1];
-        for (int q = lo + (int)threadIdx.x; q < hi; q += blockDim.x) {
+        for (int q = lo + wv; q < hi; q += nwv) {  // uniform per wave
             const int v = S.tree_elim[q], lv = v - vb;
             const int s = S.tree_vsize[v];
-            double M[K][K], R[K][K + 1];
+            double M[K], R[K + 1];
 #pragma unroll
-            for (int i = 0; i < K; ++i) {
-#pragma unroll
-                for (int j = 0; j < K; ++j) {
-                    M[i][j] = (i < s && j < s) ? Ds[lv * KK + i * K + j] : (i == j ? 1. : 0.);
-                    R[i][j] = Up[lv * KK + i * K + j];
-                }
-                R[i][K] = xh[lv * K + i];
+            for (int j = 0; j < K; ++j) {
+                M[j] = (li < s && j < s) ? Ds[lv * KK + li * K + j] : (li == j ? 1. : 0.);
+                R[j] = Up[lv * KK + li * K + j];
             }
+            R[K] = xh[lv * K + li];
             for (int cq = S.tree_vchild_off[v]; cq < S.tree_vchild_off[v + 1]; ++cq) {
                 const int lc = S.tree_vchildren[cq] - vb, sc = S.tree_vsize[lc + vb];
                 for (int k = 0; k < sc; ++k) {
-                    const double* Xk = Xs + lc * KK + k * K;  // row k of X_c (columns: v's rows)
+                    const double* Xk = Xs + lc * KK + k * K;  // row k of X_c (one address for all lanes)
                     const double hk = xh[lc * K + k];
+                    const double zik = Lp[lc * KK + li * K + k];  // Z_v,c[i][k]: zero past v's rows
 #pragma unroll
-                    for (int i = 0; i < K; ++i) {
-                        const double zik = Lp[lc * KK + i * K + k];  // Z_v,c[i][k]: zero past v's rows
-#pragma unroll
-                        for (int j = 0; j < K; ++j) M[i][j] = fma(-zik, Xk[j], M[i][j]);
-                        R[i][K] = fma(-zik, hk, R[i][K]);
-                    }
+                    for (int j = 0; j < K; ++j) M[j] = fma(-zik, Xk[j], M[j]);
+                    R[K] = fma(-zik, hk, R[K]);
                 }
             }
-            // LU with partial pivoting, applied to R
+            // LU with partial pivoting (largest |m_ij| over rows i >= j, the first on ties), the
+            // interchanges and eliminations applied to R
 #pragma unroll
             for (int j = 0; j < K; ++j) {
                 int pr = j;
-                double best = fabs(M[j][j]);
-#pragma unroll
-                for (int i = j + 1; i < K; ++i)
-                    if (fabs(M[i][j]) > best) {
-                        best = fabs(M[i][j]);
-                        pr = i;
-                    }
-#pragma unroll
-                for (int i = j + 1; i < K; ++i)
-                    if (i == pr) {
-#pragma unroll
-                        for (int c = 0; c < K; ++c) {
-                            const double t = M[j][c];
-                            M[j][c] = M[i][c];
-                            M[i][c] = t;
-                        }
-#pragma unroll
-                        for (int c = 0; c <= K; ++c) {
-                            const double t = R[j][c];
-                            R[j][c] = R[i][c];
-                            R[i][c] = t;
-                        }
-                    }
-                const double inv = 1. / M[j][j];  // one division per pivot; multipliers and the
-                M[j][j] = inv;                     // back substitution scale by it
+                double best = fabs(rdlane(M[j], j));
 #pragma unroll
                 for (int i = j + 1; i < K; ++i) {
-                    const double lij = M[i][j] * inv;
+                    const double cand = fabs(rdlane(M[j], i));
+                    if (cand > best) {
+                        best = cand;
+                        pr = i;
+                    }
+                }
+                if (pr != j) {  // uniform: rows j and pr trade places
+                    const int src = lane == j ? pr : (lane == pr ? j : lane);
 #pragma unroll
-                    for (int c = j + 1; c < K; ++c) M[i][c] = fma(-lij, M[j][c], M[i][c]);
+                    for (int c = 0; c < K; ++c) M[c] = __shfl(M[c], src, 64);
 #pragma unroll
-                    for (int c = 0; c <= K; ++c) R[i][c] = fma(-lij, R[j][c], R[i][c]);
+                    for (int c = 0; c <= K; ++c) R[c] = __shfl(R[c], src, 64);
+                }
+                double pj[K], rj[K + 1];
+#pragma unroll
+                for (int c = 0; c < K; ++c) pj[c] = rdlane(M[c], j);
+#pragma unroll
+                for (int c = 0; c <= K; ++c) rj[c] = rdlane(R[c], j);
+                const double inv = 1. / pj[j];  // one division per pivot; the multipliers and the
+                if (lane == j) M[j] = inv;        // back substitution scale by it
+                if (lane > j) {
+                    const double lij = M[j] * inv;
+#pragma unroll
+                    for (int c = j + 1; c < K; ++c) M[c] = fma(-lij, pj[c], M[c]);
+#pragma unroll
+                    for (int c = 0; c <= K; ++c) R[c] = fma(-lij, rj[c], R[c]);
                 }
             }
 #pragma unroll
             for (int j = K - 1; j >= 0; --j) {
+                if (lane == j)
 #pragma unroll
-                for (int c = 0; c <= K; ++c) R[j][c] = R[j][c] * M[j][j];  // M[j][j] holds 1 / u_jj
+                    for (int c = 0; c <= K; ++c) R[c] = R[c] * M[j];  // M[j] holds 1 / u_jj on lane j
+                double rj[K + 1];
 #pragma unroll
-                for (int i = 0; i < j; ++i)
+                for (int c = 0; c <= K; ++c) rj[c] = rdlane(R[c], j);
+                if (lane < j)
 #pragma unroll
-                    for (int c = 0; c <= K; ++c) R[i][c] = fma(-M[i][j], R[j][c], R[i][c]);
+                    for (int c = 0; c <= K; ++c) R[c] = fma(-M[j], rj[c], R[c]);
             }
+            if (lane < K) {
 #pragma unroll
-            for (int i = 0; i < K; ++i) {
-#pragma unroll
-                for (int j = 0; j < K; ++j) Xs[lv * KK + i * K + j] = R[i][j];
-                xh[lv * K + i] = R[i][K];
+                for (int j = 0; j < K; ++j) Xs[lv * KK + lane * K + j] = R[j];
+                xh[lv * K + lane] = R[K];
             }
         }
         __syncthreads();
