@@ -2867,20 +2867,48 @@ __global__ void __launch_bounds__(kTreeThreads) k_tree_interface(DevSys S, int b
     double* Xs = Lp + KK * nv;                   // [nv][K][K] X_v
     double* xh = Xs + KK * nv;                   // [nv][K]: the rhs, then zhat_v
     double* zs = xh + K * nv;                    // [nv][K]: z_v
-    // ---- every vertex's blocks and rhs from Z into LDS: thread per vertex, its row lists read
-    // once, then each block's 36 entries as one batch of loads (indices clamped, values selected
-    // after one wait)
+    // the block's schedule in LDS as 16-bit local ids (vertex sizes, parents, child lists, rows,
+    // elimination and substitution orders with their level offsets), read once from HBM
+    const int nle = tb[3], nlb = tb[5];
+    short* m_sz = reinterpret_cast<short*>(zs + K * nv);
+    short* m_par = m_sz + nv;
+    short* m_coff = m_par + nv;      // [nv + 1]
+    short* m_ch = m_coff + nv + 1;   // [nv]
+    short* m_row = m_ch + nv;        // [nv][K], zero past the vertex's size
+    short* m_el = m_row + nv * K;    // [nv]
+    short* m_ellv = m_el + nv;       // [nle + 1]
+    short* m_bk = m_ellv + nle + 1;  // [nv]
+    short* m_bklv = m_bk + nv;       // [nlb + 1]
+    {
+        const int c0 = S.tree_vchild_off[vb], nch = S.tree_vchild_off[vb + nv] - c0;
+        const int e0 = S.tree_elim_lvl[tb[2]], k0 = S.tree_back_lvl[tb[4]];
+        for (int q = threadIdx.x; q < nv; q += blockDim.x) {
+            const int v = vb + q, sz = S.tree_vsize[v], par = S.tree_vparent[v];
+            const int* rows = S.tree_vrows + S.tree_vrow_off[v];
+            m_sz[q] = (short)sz;
+            m_par[q] = (short)(par >= 0 ? par - vb : -1);
+            m_coff[q] = (short)(S.tree_vchild_off[v] - c0);
+#pragma unroll
+            for (int i = 0; i < K; ++i) m_row[q * K + i] = (short)(i < sz ? rows[i] : 0);
+            m_el[q] = (short)(S.tree_elim[e0 + q] - vb);
+            m_bk[q] = (short)(S.tree_back[k0 + q] - vb);
+        }
+        for (int q = threadIdx.x; q < nch; q += blockDim.x) m_ch[q] = (short)(S.tree_vchildren[c0 + q] - vb);
+        for (int q = threadIdx.x; q <= nle; q += blockDim.x) m_ellv[q] = (short)(S.tree_elim_lvl[tb[2] + q] - e0);
+        for (int q = threadIdx.x; q <= nlb; q += blockDim.x) m_bklv[q] = (short)(S.tree_back_lvl[tb[4] + q] - k0);
+        if (threadIdx.x == 0) m_coff[nv] = (short)nch;
+    }
+    __syncthreads();
+    // ---- every vertex's blocks and rhs from Z into LDS: thread per vertex, each block's 36
+    // entries as one batch of loads (indices clamped, values selected after one wait)
     for (int lv = threadIdx.x; lv < nv; lv += blockDim.x) {
-        const int v = vb + lv;
-        const int s = S.tree_vsize[v], p = S.tree_vparent[v];
-        const int sp = p >= 0 ? S.tree_vsize[p] : 0;
-        const int* rvp = S.tree_vrows + S.tree_vrow_off[v];
-        const int* rpp = p >= 0 ? S.tree_vrows + S.tree_vrow_off[p] : rvp;
+        const int s = m_sz[lv], p = m_par[lv];
+        const int sp = p >= 0 ? m_sz[p] : 0;
         int rv[K], rp[K];
 #pragma unroll
         for (int i = 0; i < K; ++i) {
-            rv[i] = i < s ? rvp[i] : 0;
-            rp[i] = i < sp ? rpp[i] : 0;
+            rv[i] = m_row[lv * K + i];
+            rp[i] = p >= 0 ? m_row[p * K + i] : 0;
         }
 #pragma unroll
         for (int which = 0; which < 3; ++which) {
@@ -2913,11 +2941,11 @@ __global__ void __launch_bounds__(kTreeThreads) k_tree_interface(DevSys S, int b
     // nothing
     const int wv = (int)threadIdx.x >> 6, lane = (int)threadIdx.x & 63, nwv = (int)blockDim.x >> 6;
     const int li = lane < K ? lane : 0;
-    for (int l = 0; l < tb[3]; ++l) {
-        const int lo = S.tree_elim_lvl[tb[2] + l], hi = S.tree_elim_lvl[tb[2] + l + 1];
+    for (int l = 0; l < nle; ++l) {
+        const int lo = m_ellv[l], hi = m_ellv[l + 1];
         for (int q = lo + wv; q < hi; q += nwv) {  // uniform per wave
-            const int v = S.tree_elim[q], lv = v - vb;
-            const int s = S.tree_vsize[v];
+            const int lv = m_el[q];
+            const int s = m_sz[lv];
             double M[K], R[K + 1];
 #pragma unroll
             for (int j = 0; j < K; ++j) {
@@ -2925,15 +2953,18 @@ __global__ void __launch_bounds__(kTreeThreads) k_tree_interface(DevSys S, int b
                 R[j] = Up[lv * KK + li * K + j];
             }
             R[K] = xh[lv * K + li];
-            for (int cq = S.tree_vchild_off[v]; cq < S.tree_vchild_off[v + 1]; ++cq) {
-                const int lc = S.tree_vchildren[cq] - vb, sc = S.tree_vsize[lc + vb];
-                for (int k = 0; k < sc; ++k) {
-                    const double* Xk = Xs + lc * KK + k * K;  // row k of X_c (one address for all lanes)
-                    const double hk = xh[lc * K + k];
-                    const double zik = Lp[lc * KK + li * K + k];  // Z_v,c[i][k]: zero past v's rows
+            for (int cq = m_coff[lv]; cq < m_coff[lv + 1]; ++cq) {
+                const int lc = m_ch[cq], sc = m_sz[lc];
 #pragma unroll
-                    for (int j = 0; j < K; ++j) M[j] = fma(-zik, Xk[j], M[j]);
-                    R[K] = fma(-zik, hk, R[K]);
+                for (int k = 0; k < K; ++k) {
+                    if (k < sc) {  // uniform
+                        const double* Xk = Xs + lc * KK + k * K;  // row k of X_c (one address for all lanes)
+                        const double hk = xh[lc * K + k];
+                        const double zik = Lp[lc * KK + li * K + k];  // Z_v,c[i][k]: zero past v's rows
+#pragma unroll
+                        for (int j = 0; j < K; ++j) M[j] = fma(-zik, Xk[j], M[j]);
+                        R[K] = fma(-zik, hk, R[K]);
+                    }
                 }
             }
             // LU with partial pivoting (largest |m_ij| over rows i >= j, the first on ties), the
@@ -2993,19 +3024,19 @@ __global__ void __launch_bounds__(kTreeThreads) k_tree_interface(DevSys S, int b
         __syncthreads();
     }
     // ---- substitution, roots first
-    for (int l = 0; l < tb[5]; ++l) {
-        const int lo = S.tree_back_lvl[tb[4] + l], hi = S.tree_back_lvl[tb[4] + l + 1];
-        for (int q = lo + (int)threadIdx.x; q < hi; q += blockDim.x) {
-            const int v = S.tree_back[q], lv = v - vb;
-            const int s = S.tree_vsize[v];
-            const int* rv = S.tree_vrows + S.tree_vrow_off[v];
-            const int p = S.tree_vparent[v];
-            const int sp = p >= 0 ? S.tree_vsize[p] : 0;
-            for (int i = 0; i < s; ++i) {
-                double zi = xh[lv * K + i];
-                for (int j = 0; j < sp; ++j) zi = fma(-Xs[lv * KK + i * K + j], zs[(p - vb) * K + j], zi);
-                zs[lv * K + i] = zi;
-                r[rv[i]] = zi;
+    for (int l = 0; l < nlb; ++l) {  // one wave per vertex, lane i < size its row i
+        const int lo = m_bklv[l], hi = m_bklv[l + 1];
+        for (int q = lo + wv; q < hi; q += nwv) {
+            const int lv = m_bk[q];
+            const int s = m_sz[lv], p = m_par[lv];
+            const int sp = p >= 0 ? m_sz[p] : 0;
+            if (lane < s) {
+                double zi = xh[lv * K + lane];
+#pragma unroll
+                for (int j = 0; j < K; ++j)
+                    if (j < sp) zi = fma(-Xs[lv * KK + lane * K + j], zs[p * K + j], zi);
+                zs[lv * K + lane] = zi;
+                r[m_row[lv * K + lane]] = zi;
             }
         }
         __syncthreads();
@@ -3248,11 +3279,18 @@ void launch_cond_scatter_z(const DevSys& S, int b, int nz, double* xout, hipStre
 void launch_block_axpy_copy(const DevSys& S, int b, int n, const double* d, double* x, double* r, hipStream_t st) {
     hipLaunchKernelGGL(k_block_axpy_copy, dim3((n + 255) / 256), dim3(256), 0, st, S, b, d, x, r);
 }
+// the blocks (4 K x K and 2 K doubles per vertex) and the 16-bit schedule (6 + K entries per
+// vertex, one more child offset, and two level-offset lists of at most kTreeMaxVerts + 1)
+constexpr size_t tree_lds_bytes() {
+    return (size_t)kTreeMaxVerts * (4 * kTreeMaxBlock * kTreeMaxBlock + 2 * kTreeMaxBlock) * sizeof(double) +
+           ((size_t)kTreeMaxVerts * (8 + kTreeMaxBlock) + 3) * sizeof(short);
+}
+static_assert(tree_lds_bytes() <= 160 * 1024, "tree interface solve: LDS");
 void launch_tree_interface(const DevSys& S, int b, hipStream_t st) {
     // LDS per vertex of the block's tree: Z_vv, Z_v,parent, Z_parent,v and X (K x K each), the
     // rhs / zhat and z (K each); the host admits a tree only up to kTreeMaxVerts vertices
     hipLaunchKernelGGL(k_tree_interface, dim3(1), dim3(kTreeThreads),
-                       (size_t)kTreeMaxVerts * (4 * kTreeMaxBlock * kTreeMaxBlock + 2 * kTreeMaxBlock) * sizeof(double), st,
+                       tree_lds_bytes(), st,
                        S, b);
 }
 void launch_crs_residual(const DevSys& S, int b, int n, const double* x, double* r, hipStream_t st) {
