@@ -1554,7 +1554,7 @@ __device__ __forceinline__ void back_substitute(const TT* A, int n, int lda, con
 // factored augmented matrix; waves take columns in turn; X column c -> out[c * ldo + i] (HBM).
 template <typename TT>
 __device__ __forceinline__ void back_substitute_cols(const TT* A, int n, int lda, const int* phys_of, int ncols,
-                                                     double* out, int ldo) {
+                                                     double* out, int ldo, double* lout = nullptr) {
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
     int pr[3];
     double dg[3], dinv[3];
@@ -1568,7 +1568,10 @@ __device__ __forceinline__ void back_substitute_cols(const TT* A, int n, int lda
         back_sweep<TT>(A, n, lda, pr, dg, dinv, r);
 #pragma unroll
         for (int a = 0; a < 3; ++a)
-            if (lane + 64 * a < n) out[(size_t)c * ldo + lane + 64 * a] = r[a];
+            if (lane + 64 * a < n) {
+                out[(size_t)c * ldo + lane + 64 * a] = r[a];
+                if (lout) lout[(size_t)c * ldo + lane + 64 * a] = r[a];
+            }
     }
 }
 
@@ -2667,7 +2670,7 @@ __global__ void __launch_bounds__(kThreads) k_large_assemble(DevSys S) {
 
 __host__ __device__ inline int cond_lds_bytes(int ne, int me) {
     const int lda = (ne + me + 1) | 1;
-    return align16(ne * lda * 8) + 2 * kLuTile * 4 + 2 * 256 * 8 + 16 + align16(ne * 4);
+    return align16(ne * lda * 8) + 2 * kLuTile * 4 + 2 * 256 * 8 + 16 + align16(ne * 4) + ne * (me + 1) * 8;
 }
 
 // rin: right-hand side vector in the layout of S.rhs (the conditioned rhs, or a refinement
@@ -2695,6 +2698,8 @@ __device__ __forceinline__ void cond_factor_body(DevSys& S, const double* rin) {
     int* sync = reinterpret_cast<int*>(sm + off);
     off += 16;
     int* piv = reinterpret_cast<int*>(sm + off);
+    off += align16(ne * 4);
+    double* Yl = reinterpret_cast<double*>(sm + off);  // LDS copy of Y for S_e
     const double* rhs = rin + vb;
     for (int k = t; k < ne * lda; k += blockDim.x) M[k] = TT(0);
     __syncthreads();
@@ -2704,19 +2709,19 @@ __device__ __forceinline__ void cond_factor_body(DevSys& S, const double* rin) {
         const int g0 = S.cond_gat_off[ce], g1 = S.cond_gat_off[ce + 1];
         const int* src = S.cond_gat_src;
         const int* dst = S.cond_gat_dst;
-        for (int q = g0 + t; q < g1; q += 4 * blockDim.x) {
-            int sq[4], dq[4];
-            double vq[4];
+        for (int q = g0 + t; q < g1; q += 8 * blockDim.x) {
+            int sq[8], dq[8];
+            double vq[8];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
+            for (int u = 0; u < 8; ++u) {
                 const int qq = q + u * (int)blockDim.x;
                 sq[u] = qq < g1 ? src[qq] : src[q];
                 dq[u] = qq < g1 ? dst[qq] : -1;
             }
 #pragma unroll
-            for (int u = 0; u < 4; ++u) vq[u] = V[sq[u]];
+            for (int u = 0; u < 8; ++u) vq[u] = V[sq[u]];
 #pragma unroll
-            for (int u = 0; u < 4; ++u)
+            for (int u = 0; u < 8; ++u)
                 if (dq[u] >= 0) M[dq[u]] = (TT)vq[u];
         }
         for (int i = t; i < ne; i += blockDim.x) M[i * lda + ne + me] = (TT)rhs[rows[i]];
@@ -2725,14 +2730,28 @@ __device__ __forceinline__ void cond_factor_body(DevSys& S, const double* rin) {
     const LuScratch W{ph, lc, sync, S.lu_wait_limit, S.dev_error};
     lu_factor_augmented<TT>(M, ne, lda, piv, W, nrhs);
     double* Y = S.Ybuf + S.cond_Y_off[ce];
-    back_substitute_cols<TT>(M, ne, lda, ph, nrhs, Y, ne);
+    back_substitute_cols<TT>(M, ne, lda, ph, nrhs, Y, ne, Yl);
     __syncthreads();
     double* Sm = S.Sbuf + S.cond_S_off[ce];
     for (int q = t; q < me * nrhs; q += blockDim.x) {  // S_e = B_e Y: constraint row lam[k]'s entries in this element
         const int k = q % me, c = q / me;
+        const int u0 = S.cond_bt_off[l0 + k], u1 = S.cond_bt_off[l0 + k + 1];
         double acc = 0.;
-        for (int u = S.cond_bt_off[l0 + k]; u < S.cond_bt_off[l0 + k + 1]; ++u)
-            acc += V[S.cond_bt_src[u]] * Y[(size_t)c * ne + S.cond_bt_col[u]];
+        for (int u = u0; u < u1; u += 4) {  // four entries' loads in flight, summed in order
+            int sq[4], cq[4];
+            double vq[4];
+#pragma unroll
+            for (int w = 0; w < 4; ++w) {
+                const int uu = u + w < u1 ? u + w : u;
+                sq[w] = S.cond_bt_src[uu];
+                cq[w] = S.cond_bt_col[uu];
+            }
+#pragma unroll
+            for (int w = 0; w < 4; ++w) vq[w] = V[sq[w]];
+#pragma unroll
+            for (int w = 0; w < 4; ++w)
+                if (u + w < u1) acc += vq[w] * Yl[(size_t)c * ne + cq[w]];
+        }
         Sm[(size_t)c * me + k] = acc;
     }
 }
