@@ -192,7 +192,7 @@ __device__ __forceinline__ void mfma_column(int ecol, int N, int& j, int& r, int
     sp = rem - 3 * r;
 }
 
-template <bool kFused>
+template <bool kFused, bool kTwoRows>
 __device__ __forceinline__ void contract_mfma_f32(const DevSys& S, char* sm, const Lay& L, int n, int N, int Q,
                                                   const int* loc, double* mat_out) {
     const int tid = threadIdx.x, lane = tid & 63;
@@ -204,7 +204,6 @@ __device__ __forceinline__ void contract_mfma_f32(const DevSys& S, char* sm, con
     const float* phiT = reinterpret_cast<const float*>(sm + L.phiT);
     const double* tr = reinterpret_cast<const double*>(sm + L.tr);
     const int TH = (18 * Nu + 15) >> 4;
-    const bool two_rows = Nu > 16;
     const int npairs = Qu >> 1;  // QP pairs with both QPs present; an odd Q adds one with QP Q - 1 only
     // A operand: row li (and 16 + li) of L at column k: phi (t = 0) or phi' (t = 1) of the lane's QP
     const int ia = 2 * li + (t ? 1 : 0);
@@ -216,11 +215,13 @@ __device__ __forceinline__ void contract_mfma_f32(const DevSys& S, char* sm, con
     const int qt = Qu - 1;
     const bool tail = (Qu & 1) != 0;
     const bool tail_valid = qo == 0;
-    f32x4 acc[kMfmaTilesPerWave][2];
+    // accumulators: one 16-row tile per column tile (N <= 16), two for N > 16
+    constexpr int kNB = kTwoRows ? 2 : 1;
+    f32x4 acc[kMfmaTilesPerWave][kNB];
 #pragma unroll
     for (int s = 0; s < kMfmaTilesPerWave; ++s)
 #pragma unroll
-        for (int b = 0; b < 2; ++b) acc[s][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+        for (int b = 0; b < kNB; ++b) acc[s][b] = f32x4{0.f, 0.f, 0.f, 0.f};
 
     // Tiles in pairs within one half: tile pair (s', s' + 1) of half h runs one QP loop for
     // both tiles -- two independent MFMA chains per QP pair, the loads of one tile overlapping
@@ -236,7 +237,7 @@ __device__ __forceinline__ void contract_mfma_f32(const DevSys& S, char* sm, con
                 int jA, rA, pA, jB, rB, pB;
                 mfma_column(16 * TA + li, Nu, jA, rA, pA);
                 mfma_column(16 * (TA + 4) + li, Nu, jB, rB, pB);
-                f32x4 a0 = acc[sA][0], a1 = acc[sA][1], b0 = acc[sA + 1][0], b1 = acc[sA + 1][1];
+                f32x4 a0 = acc[sA][0], a1 = acc[sA][kNB - 1], b0 = acc[sA + 1][0], b1 = acc[sA + 1][kNB - 1];
                 if (h == 0) {
                     for (int ks = 0; ks < npairs; ++ks) {
                         const float* ph = phq + ks * (2 * kPhiStride);
@@ -247,7 +248,7 @@ __device__ __forceinline__ void contract_mfma_f32(const DevSys& S, char* sm, con
                         const float x1 = row2 ? y1 : 0.f;
                         a0 = __builtin_amdgcn_mfma_f32_16x16x4f32(x0, RA, a0, 0, 0, 0);
                         b0 = __builtin_amdgcn_mfma_f32_16x16x4f32(x0, RB, b0, 0, 0, 0);
-                        if (two_rows) {
+                        if constexpr (kTwoRows) {
                             a1 = __builtin_amdgcn_mfma_f32_16x16x4f32(x1, RA, a1, 0, 0, 0);
                             b1 = __builtin_amdgcn_mfma_f32_16x16x4f32(x1, RB, b1, 0, 0, 0);
                         }
@@ -261,7 +262,7 @@ __device__ __forceinline__ void contract_mfma_f32(const DevSys& S, char* sm, con
                         const float x0 = tail_valid ? y0 : 0.f, x1 = tail_valid && row2 ? y1 : 0.f;
                         a0 = __builtin_amdgcn_mfma_f32_16x16x4f32(x0, RA, a0, 0, 0, 0);
                         b0 = __builtin_amdgcn_mfma_f32_16x16x4f32(x0, RB, b0, 0, 0, 0);
-                        if (two_rows) {
+                        if constexpr (kTwoRows) {
                             a1 = __builtin_amdgcn_mfma_f32_16x16x4f32(x1, RA, a1, 0, 0, 0);
                             b1 = __builtin_amdgcn_mfma_f32_16x16x4f32(x1, RB, b1, 0, 0, 0);
                         }
@@ -283,7 +284,7 @@ __device__ __forceinline__ void contract_mfma_f32(const DevSys& S, char* sm, con
                         const float x1 = row2 ? y1 : 0.f;
                         a0 = __builtin_amdgcn_mfma_f32_16x16x4f32(x0, RA, a0, 0, 0, 0);
                         b0 = __builtin_amdgcn_mfma_f32_16x16x4f32(x0, RB, b0, 0, 0, 0);
-                        if (two_rows) {
+                        if constexpr (kTwoRows) {
                             a1 = __builtin_amdgcn_mfma_f32_16x16x4f32(x1, RA, a1, 0, 0, 0);
                             b1 = __builtin_amdgcn_mfma_f32_16x16x4f32(x1, RB, b1, 0, 0, 0);
                         }
@@ -299,16 +300,18 @@ __device__ __forceinline__ void contract_mfma_f32(const DevSys& S, char* sm, con
                         const float x0 = tail_valid ? y0 : 0.f, x1 = tail_valid && row2 ? y1 : 0.f;
                         a0 = __builtin_amdgcn_mfma_f32_16x16x4f32(x0, RA, a0, 0, 0, 0);
                         b0 = __builtin_amdgcn_mfma_f32_16x16x4f32(x0, RB, b0, 0, 0, 0);
-                        if (two_rows) {
+                        if constexpr (kTwoRows) {
                             a1 = __builtin_amdgcn_mfma_f32_16x16x4f32(x1, RA, a1, 0, 0, 0);
                             b1 = __builtin_amdgcn_mfma_f32_16x16x4f32(x1, RB, b1, 0, 0, 0);
                         }
                     }
                 }
                 acc[sA][0] = a0;
-                acc[sA][1] = a1;
                 acc[sA + 1][0] = b0;
-                acc[sA + 1][1] = b1;
+                if constexpr (kTwoRows) {
+                    acc[sA][kNB - 1] = a1;
+                    acc[sA + 1][kNB - 1] = b1;
+                }
             }
         }
     }
@@ -342,14 +345,14 @@ __device__ __forceinline__ void contract_mfma_f32(const DevSys& S, char* sm, con
                 if (kFused) {
                     const int c = loc[j] + col;
 #pragma unroll
-                    for (int b = 0; b < 2; ++b)
+                    for (int b = 0; b < kNB; ++b)
 #pragma unroll
                         for (int r = 0; r < 4; ++r)
                             if (rowoff[b][r] >= 0) A[(rowoff[b][r] + r_) * L.lda + c] = acc[s][b][r] * cond;
                 } else {
                     double* out = mat_out + (size_t)j * 36 + r_ * 6 + col;
 #pragma unroll
-                    for (int b = 0; b < 2; ++b)
+                    for (int b = 0; b < kNB; ++b)
 #pragma unroll
                         for (int r = 0; r < 4; ++r)
                             if (rowoff[b][r] >= 0) out[(size_t)rowoff[b][r] * Nu * 36] = (double)acc[s][b][r];
@@ -645,9 +648,12 @@ __device__ __forceinline__ void element_phase(const DevSys& S, int e, char* sm, 
     if constexpr (sizeof(TT) == 4) {
         // FP32 tangent: the matrix cores or the VALU register tile, per element size (the
         // tuning table: bit N of contract_mfma_mask)
-        if ((S.contract_mfma_mask >> N) & 1u)
-            contract_mfma_f32<kFused>(S, sm, L, n, N, Q, loc, mat_out);
-        else
+        if ((S.contract_mfma_mask >> N) & 1u) {
+            if (N > 16)
+                contract_mfma_f32<kFused, true>(S, sm, L, n, N, Q, loc, mat_out);
+            else
+                contract_mfma_f32<kFused, false>(S, sm, L, n, N, Q, loc, mat_out);
+        } else
             contract_valu<TT, kFused>(S, sm, L, n, N, Q, loc, mat_out);
     } else {
         contract_valu<TT, kFused>(S, sm, L, n, N, Q, loc, mat_out);
