@@ -1432,53 +1432,6 @@ __device__ __forceinline__ double div_by(double y, double d, double d_inv) {
     const double e = fma(-d, q0, y);
     return fma(e, d_inv, q0);
 }
-__device__ __forceinline__ double sel3(const double* r, int a) { return a == 0 ? r[0] : (a == 1 ? r[1] : r[2]); }
-
-// Column-oriented back substitution U x = r by wave 0 (lane owns logical rows lane, +64, +128;
-// n <= 192), FP64, the oracle's operation order: x_k = r_k / u_kk, then r_i -= u_ik x_k for
-// i < k, k descending.  Columns go in blocks of 8 whose entries are loaded together; x_k goes
-// to every lane by readlane; the diagonal reciprocals are formed up front.  Row i of the
-// factors is at physical row pr[a] of A.  On return r holds x.
-constexpr int kSweepBlock = 8;
-
-template <typename TT>
-__device__ __forceinline__ void back_sweep(const TT* A, int n, int lda, const int* pr, const double* dg,
-                                           const double* dinv, double* r) {
-    const int lane = threadIdx.x & 63;
-    for (int kb = ((n - 1) / kSweepBlock) * kSweepBlock; kb >= 0; kb -= kSweepBlock) {
-        TT U[3][kSweepBlock];
-#pragma unroll
-        for (int a = 0; a < 3; ++a)
-#pragma unroll
-            for (int c = 0; c < kSweepBlock; ++c) U[a][c] = A[pr[a] * lda + (kb + c < n ? kb + c : n - 1)];
-#pragma unroll
-        for (int c = kSweepBlock - 1; c >= 0; --c) {
-            const int k = kb + c;
-            if (k < n) {
-                const int ka = k >> 6, kl = k & 63;
-                const double xk = rdlane(div_by(sel3(r, ka), sel3(dg, ka), sel3(dinv, ka)), kl);
-#pragma unroll
-                for (int a = 0; a < 3; ++a) {
-                    const int i = lane + 64 * a;
-                    const double upd = fma(-(double)U[a][c], xk, r[a]);
-                    r[a] = i < k ? upd : (i == k ? xk : r[a]);
-                }
-            }
-        }
-    }
-}
-
-template <typename TT>
-__device__ __forceinline__ void diag_of(const TT* A, int n, int lda, const int* pr, double* dg, double* dinv) {
-    const int lane = threadIdx.x & 63;
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        const int i = lane + 64 * a;
-        dg[a] = i < n ? (double)A[pr[a] * lda + i] : 1.;
-        dinv[a] = 1. / dg[a];
-    }
-}
-
 // Back substitution U x = y with y in column n of the factored augmented matrix (logical row
 // i at physical row phys_of[i]), in blocks of kBackBlock rows from the bottom.  Once a block's
 // x is known, wave 0 updates the rows of the block above with it and solves that block (lane <->
@@ -1557,27 +1510,59 @@ __device__ __forceinline__ void back_substitute(const TT* A, int n, int lda, con
 }
 
 // Back substitution U X = Y for the right-hand sides in columns n + c, c in [0, ncols), of the
-// factored augmented matrix; waves take columns in turn; X column c -> out[c * ldo + i] (HBM).
+// factored augmented matrix, all columns together in blocks of 16 rows from the bottom: per
+// block, each wave solves the diagonal block for four columns (16-lane group <-> column, lane
+// <-> row, x_j to its group by a lane shuffle), then every thread updates one (row, column) of
+// the rows above with the block's x.  Each entry receives r_i -= u_ij x_j for j descending and
+// x_j = r_j / u_jj (the IEEE quotient) as in the column-oriented loop.  X: LDS, [ncols][n]
+// doubles, the solution on return; also written to out[c * ldo + i] (HBM).
 template <typename TT>
-__device__ __forceinline__ void back_substitute_cols(const TT* A, int n, int lda, const int* phys_of, int ncols,
-                                                     double* out, int ldo, double* lout = nullptr) {
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
-    int pr[3];
-    double dg[3], dinv[3];
+__device__ __forceinline__ void back_substitute_cols_blocked(const TT* A, int n, int lda, const int* phys_of,
+                                                             int ncols, double* X, double* out, int ldo) {
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, nw = blockDim.x >> 6;
+    for (int q = tid; q < ncols * n; q += blockDim.x) {
+        const int c = q / n, i = q - n * (q / n);
+        X[q] = (double)A[phys_of[i] * lda + n + c];
+    }
+    __syncthreads();
+    const int g = lane >> 4, rl = lane & 15;
+    for (int kb = ((n - 1) / 16) * 16; kb >= 0; kb -= 16) {
+        const int nb = (n - kb) < 16 ? (n - kb) : 16;
+        for (int cb = 4 * wv; cb < ncols; cb += 4 * nw) {  // the same for every lane of a wave
+            const int c = cb + g;
+            const bool okc = c < ncols, okr = rl < nb;
+            const int i = kb + (okr ? rl : 0);
+            const int pr = phys_of[i];
+            double r = okc && okr ? X[c * n + i] : 0.;
+            const double d = okr ? (double)A[pr * lda + i] : 1.;
+            const double dinv = 1. / d;
+            TT u[16];
 #pragma unroll
-    for (int a = 0; a < 3; ++a) pr[a] = lane + 64 * a < n ? phys_of[lane + 64 * a] : 0;
-    diag_of<TT>(A, n, lda, pr, dg, dinv);
-    for (int c = wv; c < ncols; c += nw) {
-        double r[3];
+            for (int j = 0; j < 16; ++j) u[j] = A[pr * lda + kb + (j < nb ? j : 0)];
 #pragma unroll
-        for (int a = 0; a < 3; ++a) r[a] = lane + 64 * a < n ? (double)A[pr[a] * lda + n + c] : 0.;
-        back_sweep<TT>(A, n, lda, pr, dg, dinv, r);
-#pragma unroll
-        for (int a = 0; a < 3; ++a)
-            if (lane + 64 * a < n) {
-                out[(size_t)c * ldo + lane + 64 * a] = r[a];
-                if (lout) lout[(size_t)c * ldo + lane + 64 * a] = r[a];
+            for (int j = 15; j >= 0; --j) {
+                if (j < nb) {  // uniform
+                    const double xj = __shfl(div_by(r, d, dinv), (lane & ~15) | j, 64);
+                    const double upd = fma(-(double)u[j], xj, r);
+                    r = rl < j ? upd : (rl == j ? xj : r);
+                }
             }
+            if (okc && okr) X[c * n + i] = r;
+        }
+        __syncthreads();
+        for (int q = tid; q < kb * ncols; q += blockDim.x) {
+            const int c = q / kb, i = q - kb * (q / kb);
+            const TT* Ar = A + phys_of[i] * lda + kb;
+            const double* xb = X + c * n + kb;
+            double r = X[c * n + i];
+            for (int j = nb - 1; j >= 0; --j) r = fma(-(double)Ar[j], xb[j], r);
+            X[c * n + i] = r;
+        }
+        __syncthreads();
+    }
+    for (int q = tid; q < ncols * n; q += blockDim.x) {
+        const int c = q / n, i = q - n * (q / n);
+        out[(size_t)c * ldo + i] = X[q];
     }
 }
 
@@ -2736,7 +2721,7 @@ __device__ __forceinline__ void cond_factor_body(DevSys& S, const double* rin) {
     const LuScratch W{ph, lc, sync, S.lu_wait_limit, S.dev_error};
     lu_factor_augmented<TT>(M, ne, lda, piv, W, nrhs);
     double* Y = S.Ybuf + S.cond_Y_off[ce];
-    back_substitute_cols<TT>(M, ne, lda, ph, nrhs, Y, ne, Yl);
+    back_substitute_cols_blocked<TT>(M, ne, lda, ph, nrhs, Yl, Y, ne);
     __syncthreads();
     double* Sm = S.Sbuf + S.cond_S_off[ce];
     for (int q = t; q < me * nrhs; q += blockDim.x) {  // S_e = B_e Y: constraint row lam[k]'s entries in this element
