@@ -1363,6 +1363,11 @@ __device__ __forceinline__ void lu_factor_nopiv_f32(float* A, int n, int lda, co
                         acc1[ks] = pc1[ks * lda];
                     }
 #pragma unroll
+                    for (int ks = 0; ks < 4; ++ks) {  // all 24 loads issued before the one wait
+                        pin(ao0[ks]); pin(bo0[ks]); pin(ao1[ks]); pin(bo1[ks]);
+                    }
+                    pin(acc0); pin(acc1);
+#pragma unroll
                     for (int ks = 0; ks < 4; ++ks) {
                         acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(-ao0[ks], bo0[ks], acc0, 0, 0, 0);
                         acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(-ao1[ks], bo1[ks], acc1, 0, 0, 0);
@@ -1387,6 +1392,11 @@ __device__ __forceinline__ void lu_factor_nopiv_f32(float* A, int n, int lda, co
                         bo[ks] = pb[4 * ks * lda];
                         acc[ks] = pc[ks * lda];
                     }
+#pragma unroll
+                    for (int ks = 0; ks < 4; ++ks) {
+                        pin(ao[ks]); pin(bo[ks]);
+                    }
+                    pin(acc);
 #pragma unroll
                     for (int ks = 0; ks < 4; ++ks) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(-ao[ks], bo[ks], acc, 0, 0, 0);
 #pragma unroll
