@@ -508,15 +508,19 @@ __device__ __forceinline__ void contract_valu(const DevSys& S, char* sm, const L
 //   kOwnTangent: the nodes' rotational tangent blocks are formed here from q_delta
 //           (UpdateTangentOperator folded in) rather than read from tan_rot.
 // ---------------------------------------------------------------------------------------
+//   model: the element's model where the caller knows it (one load for the gravity vector
+//           instead of the element -> node -> block -> model chain), else -1.
 template <typename TT, bool kFused, bool kOwnTangent = kFused>
 __device__ __forceinline__ void element_phase(const DevSys& S, int e, char* sm, const Lay& L, int n,
-                                              double* res_out = nullptr, double* mat_out = nullptr) {
+                                              double* res_out = nullptr, double* mat_out = nullptr,
+                                              int model = -1) {
     const int tid = threadIdx.x;
     const int N = S.elem_N[e], Q = S.elem_Q[e];
     const int noff = S.elem_node_off[e];
     const int qoff = S.elem_qp_off[e];
     const int soff = S.elem_shape_off[e];
-    const double* grav = S.model_gravity + 3 * S.block_model[S.node_block[S.elem_nodes[noff]]];
+    const double* grav =
+        S.model_gravity + 3 * (model >= 0 ? model : S.block_model[S.node_block[S.elem_nodes[noff]]]);
     double* nq = reinterpret_cast<double*>(sm + L.nq);
     double* nv = reinterpret_cast<double*>(sm + L.nv);
     double* nvd = reinterpret_cast<double*>(sm + L.nvd);
@@ -532,8 +536,11 @@ __device__ __forceinline__ void element_phase(const DevSys& S, int e, char* sm, 
     double* rhs = reinterpret_cast<double*>(sm + L.rhs);
     const double h = S.p.h;
 
-    // ---- gather node state (update_node_state.hpp:26-35) and tangent blocks
-    for (int k = tid; k < N; k += blockDim.x) {
+    // ---- gather node state (update_node_state.hpp:26-35) and tangent blocks on the last wave,
+    // the shape and QP tables on the waves before it: the two load chains run side by side
+    const int nodeT0 = blockDim.x > 64 ? (int)blockDim.x - 64 : 0;  // first thread of the node gather
+    const int tabT = blockDim.x > 64 ? nodeT0 : (int)blockDim.x;     // threads of the table gather
+    for (int k = tid - nodeT0; k >= 0 && k < N; k += blockDim.x - nodeT0) {
         const int node = S.elem_nodes[noff + k];
 #pragma unroll
         for (int c = 0; c < 7; ++c) nq[k * 7 + c] = S.q[node * 7 + c];
@@ -554,7 +561,7 @@ __device__ __forceinline__ void element_phase(const DevSys& S, int e, char* sm, 
         }
         loc[k] = kFused ? S.node_local[node] : 0;
     }
-    for (int k = tid; k < (kPhiStride / 2) * Q; k += blockDim.x) {
+    for (int k = tid; tid < tabT && k < (kPhiStride / 2) * Q; k += tabT) {
         const int i = k / Q, q = k - Q * (k / Q);
         double p = 0., dp = 0.;
         if (i < N) {
@@ -566,7 +573,7 @@ __device__ __forceinline__ void element_phase(const DevSys& S, int e, char* sm, 
         phiT[q * kPhiStride + 2 * i] = (TT)p;
         phiT[q * kPhiStride + 2 * i + 1] = (TT)dp;
     }
-    for (int q = tid; q < Q; q += blockDim.x) {
+    for (int q = tid; tid < tabT && q < Q; q += tabT) {
         wq[q] = S.qp_weight[qoff + q];
         Jq[q] = S.qp_jac[qoff + q];
     }
@@ -627,6 +634,9 @@ __device__ __forceinline__ void element_phase(const DevSys& S, int e, char* sm, 
     // ---- residual (integrate_residual_vector.hpp:32-50)
     for (int t = tid; t < 6 * N; t += blockDim.x) {
         const int k = t / 6, c = t - 6 * (t / 6);
+        // HBM operands first: their latency runs under the QP loop
+        const double fx = S.node_FX[(size_t)(noff + k) * 6 + c];
+        const double fk = kFused ? S.f[S.elem_nodes[noff + k] * 6 + c] : 0.;
         double acc = 0.;
         for (int q = 0; q < Q; ++q) {
             const double w = wq[q];
@@ -635,10 +645,9 @@ __device__ __forceinline__ void element_phase(const DevSys& S, int e, char* sm, 
             const double* F = qpF + q * 24;
             acc += cc * F[c] + cd * ((F[6 + c] + F[12 + c]) - F[18 + c]);
         }
-        const double R = acc - S.node_FX[(size_t)(noff + k) * 6 + c];
+        const double R = acc - fx;
         if (kFused) {
-            const int node = S.elem_nodes[noff + k];
-            rhs[loc[k] + c] = -S.f[node * 6 + c] + R;
+            rhs[loc[k] + c] = -fk + R;
         } else {
             res_out[k * 6 + c] = R;
         }
@@ -2185,7 +2194,7 @@ __device__ __forceinline__ void fused_step_body(const DevSys& S, int Nm, int Qm,
             // -f + R after the element phase's first barrier
             for (int i = t; i < S.block_nsys[b]; i += blockDim.x)
                 rhs[i] = -S.f[S.block_nodes[S.block_node_begin[b] + i / 6] * 6 + i % 6];
-            element_phase<TT, true>(S, e, sm, L, n);
+            element_phase<TT, true>(S, e, sm, L, n, nullptr, nullptr, m);
             KYN_PHASE_MARK(5);
             block_constraints<TT>(S, b, sm, L);
             condition_rhs(S, b, rhs);
