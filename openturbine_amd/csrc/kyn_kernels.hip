@@ -1978,7 +1978,11 @@ __device__ __forceinline__ double block_error_and_update(const DevSys& S, int b,
     const int nb0 = S.block_node_begin[b];
     const int t = threadIdx.x;
     double term = 0.;
-    if (n <= (int)blockDim.x) {  // uniform: one row per thread, the update's loads issued with the error's
+    if (n <= (int)blockDim.x - 64) {
+        // uniform: one row per thread on the first waves, the update's loads issued with the
+        // error's; the last wave forms the nodes' new displacements from the old q_delta and x
+        // itself (the same q_delta + x / h the row threads store), so no barrier separates the
+        // two.  Every old value is read before the reduction's barriers, every store follows them.
         const int i = t;
         double qd = 0., v = 0., vd = 0.;
         int k = 0;
@@ -1988,6 +1992,35 @@ __device__ __forceinline__ double block_error_and_update(const DevSys& S, int b,
             if (p.is_dynamic) {
                 v = S.v[k];
                 vd = S.vd[k];
+            }
+        }
+        // constraint multipliers: operands loaded now, sums written after the reduction
+        const int qc = S.block_con_begin[b] + t;
+        const bool has_con = qc < S.block_con_end[b];
+        int cidx = 0, crows = 0, cloc = 0;
+        double lam[6];
+        if (has_con) {
+            cidx = S.block_cons[qc];
+            crows = S.con_rows[cidx];
+            cloc = S.con_local[cidx];
+#pragma unroll
+            for (int r = 0; r < 6; ++r) lam[r] = r < crows ? S.con_lambda[cidx * 6 + r] : 0.;
+        }
+        const int nodeT0 = blockDim.x - 64;
+        if (t >= nodeT0) {
+            for (int kn = nb0 + t - nodeT0; kn < S.block_node_end[b]; kn += 64) {
+                const int node = S.block_nodes[kn], r0 = 6 * (kn - nb0);
+                double qn_d[6], qp[7], qn[7];
+#pragma unroll
+                for (int c = 0; c < 6; ++c) {
+                    const double old = S.q_delta[node * 6 + c];
+                    qn_d[c] = r0 + c < nsys ? old + xs[r0 + c] / p.h : old;
+                }
+#pragma unroll
+                for (int c = 0; c < 7; ++c) qp[c] = S.q_prev[node * 7 + c];
+                calc_displacement(p.h, qn_d, qp, qn);
+#pragma unroll
+                for (int c = 0; c < 7; ++c) S.q[node * 7 + c] = qn[c];
             }
         }
         if (i < nsys) {
@@ -2011,12 +2044,15 @@ __device__ __forceinline__ double block_error_and_update(const DevSys& S, int b,
                 S.vd[k] = vd + p.beta_prime * dx;
             }
         }
-        for (int q = S.block_con_begin[b] + t; q < S.block_con_end[b]; q += blockDim.x) {
+        if (has_con) {
+#pragma unroll
+            for (int r = 0; r < 6; ++r)
+                if (r < crows) S.con_lambda[cidx * 6 + r] = lam[r] + xs[cloc + r];
+        }
+        for (int q = qc + blockDim.x; q < S.block_con_end[b]; q += blockDim.x) {
             const int c = S.block_cons[q];
             for (int r = 0; r < S.con_rows[c]; ++r) S.con_lambda[c * 6 + r] += xs[S.con_local[c] + r];
         }
-        __syncthreads();
-        for (int q = nb0 + t; q < S.block_node_end[b]; q += blockDim.x) displacement_node(S, S.block_nodes[q]);
         __syncthreads();
         return sum;
     }
